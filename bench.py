@@ -1,0 +1,144 @@
+#!/usr/bin/env python3
+"""Benchmark: Metropolis samples/s and SR-step seconds of the sampling + SR
+optimisation path on one MI355X, 16x16 t-t'-U Hubbard model (Nsize 224,
+8 QP planes, 16384 walkers; mvmc-rs_amd/lattice.py workload "square16").
+
+A step is one SR step in the steady state: every walker restores its burn
+configuration, runs (NVMCSample + 1) x NVMCInterval x Nsite proposals,
+saves NVMCSample samples, the samples are measured and reduced, and the
+SR-CG solve updates the parameters.  --warmup untimed steps come first (the
+first one thermalises with NVMCWarmUp outer steps).
+
+value  = saved samples / second of the whole timed SR steps (all walkers).
+roofline: dominant kernel = the sampling kernel k_block; algorithmic bytes
+per launch = proposals * NQP * 2n*8 (inverse row + Slater row) + accepted
+moves * NQP * 2n^2*8 (inverse read + write), divided by the launch's
+average device time (HIP events on the kernel's stream).
+cpu_baseline: the FP64 oracle (same algorithm and operation order) on a
+bounded walker subset of the same workload, one thread per walker.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for sub in ("mvmc-rs_amd", "oracle"):
+    sys.path.insert(0, os.path.join(ROOT, sub))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(workload_dir, walkers, threads):
+    """Oracle leg: one untimed warm-up SR step, then one timed steady step."""
+    import oracle_py
+    run = oracle_py.Run(os.path.join(workload_dir, "namelist.def"), os.path.join(workload_dir, "initial.def"),
+                        walkers=walkers, threads=threads)
+    t0 = time.time()
+    run.step(do_sr=True)
+    t1 = time.time()
+    run.step(do_sr=True)
+    t2 = time.time()
+    samples = walkers * run.nsample
+    return {"value": samples / (t2 - t1), "unit": "samples/s", "cores": threads, "kind": "port",
+            "sample": f"{walkers} walkers of the same workload, 1 steady-state SR step after 1 warm-up step "
+                      f"(warm-up {t1 - t0:.1f} s, timed {t2 - t1:.1f} s), one thread per walker"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="square16")
+    ap.add_argument("--walkers", type=int, default=0, help="override (not a valid headline number)")
+    ap.add_argument("--cpu-walkers", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); walkers shard across GPUs "
+              f"by construction but no multi-GPU path is built", file=sys.stderr)
+        sys.exit(2)
+
+    import lattice
+    import mvmc_hip
+    wl_dir = os.path.join(tempfile.gettempdir(), f"mvmc_bench_{args.workload}")
+    info = lattice.write_workload(args.workload, wl_dir, sr_steps=args.steps + args.warmup)
+    walkers = args.walkers or info["walkers"]
+    log(f"workload {args.workload}: Nsite={info['nsite']} Nsize={info['nsize']} NQP={info['nqp']} "
+        f"NPara={info['npara']} walkers={walkers}")
+    t = time.time()
+    ens = mvmc_hip.Ensemble(info["namelist"], info["initial"], walkers=walkers)
+    log(f"ensemble created in {time.time() - t:.1f} s (grid {ens.grid}, LDS {ens.lds_sampler} B)")
+    for s in range(args.warmup):
+        t = time.time()
+        row = ens.opt_step()
+        log(f"warmup step {s}: E/site = {row[0] / info['nsite']:.8f}  ({time.time() - t:.1f} s)")
+    tm0 = ens.timings()
+    step_times, rows, kernel = [], [], []
+    for s in range(args.steps):
+        before = ens.timings()
+        t = time.time()
+        row = ens.opt_step()
+        dt = time.time() - t
+        after = ens.timings()
+        step_times.append(dt)
+        rows.append(row)
+        kernel.append((after["block_ms"], after["block_launches"], after["proposals"] - before["proposals"],
+                       after["accepted"] - before["accepted"]))
+        log(f"step {s}: E/site = {row[0] / info['nsite']:.8f}  {dt:.2f} s  (sampling {after['sample_s'] - before['sample_s']:.2f} s, "
+            f"measure {after['measure_s'] - before['measure_s']:.2f} s, SR {after['sr_s'] - before['sr_s']:.2f} s)")
+    tm1 = ens.timings()
+    total = sum(step_times)
+    samples = walkers * ens.nsample * args.steps
+    n, nqp = ens.n, ens.nqp
+    # roofline of the sampling kernel (last timed step's launches)
+    block_ms, launches, props, accs = kernel[-1]
+    bytes_total = props * nqp * 2 * n * 8 + accs * nqp * 2 * n * n * 8
+    avg_launch_s = block_ms / 1e3 / max(launches, 1)
+    achieved = bytes_total / max(launches, 1) / avg_launch_s / 1e9
+    out = {
+        "metric": "Metropolis samples/sec + SR-step s, 16x16 Hubbard Nsize 224; |dE/site| vs FP64",
+        "value": samples / total,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * total / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {info['nsite']}-site t-t'-U Hubbard, Nsize {info['nsize']}, "
+                               f"NQP {nqp}, NPara {info['npara']}, {walkers} walkers, 1 sample/walker, "
+                               f"16 sweeps/sample, SR-CG",
+                   "walkers": walkers, "nsize": n, "nqp": nqp, "npara": info["npara"], "parallelism": "walkers"},
+        "sr_step_s": total / args.steps,
+        "sampling_s_per_step": (tm1["sample_s"] - tm0["sample_s"]) / args.steps,
+        "measure_s_per_step": (tm1["measure_s"] - tm0["measure_s"]) / args.steps,
+        "sr_cg_s_per_step": (tm1["sr_s"] - tm0["sr_s"]) / args.steps,
+        "proposals_per_s": (tm1["proposals"] - tm0["proposals"]) / total,
+        "acceptance": accs / max(props, 1),
+        "energy_per_site": [r[0] / info["nsite"] for r in rows],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_block", "avg_launch_ms": avg_launch_s * 1e3, "launches": launches},
+    }
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(wl_dir, args.cpu_walkers, args.cpu_threads)
+        except Exception as e:  # the baseline is reported, never the target
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

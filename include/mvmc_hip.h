@@ -1,0 +1,113 @@
+/* C ABI of libmvmc_hip.so: the MI355X sampling + SR-optimisation path of
+ * mvmc-rs behind the reference's backend seams.  Plain C types only; device
+ * buffers are caller-owned device pointers where noted; every function
+ * returns 0 on success and a negative code on error (message via
+ * mvmc_last_error).  Handles are single-owner and not thread-safe, like the
+ * reference's resident sampler (gpu/mvmc-gpu-cuda/src/interval/resident.rs:104-130).
+ * There is no CPU fallback: without a usable GPU every call fails.
+ */
+#ifndef MVMC_HIP_H
+#define MVMC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes. */
+#define MVMC_OK 0
+#define MVMC_E_INVALID_SHAPE (-1)   /* BatchedError::InvalidShape (crates/mvmc-gpu/src/lib.rs:83-101) */
+#define MVMC_E_UNAVAILABLE (-2)     /* BatchedError::BackendUnavailable: no GPU / HIP error at init */
+#define MVMC_E_BACKEND (-3)         /* BatchedError::Backend / StageError::Failed: HIP runtime error */
+#define MVMC_E_INPUT (-4)           /* input files / parameters rejected */
+#define MVMC_E_NUMERIC (-5)         /* sticky device status set (see mvmc_ens_status) */
+
+const char* mvmc_last_error(void);
+int mvmc_abi_version(void);
+
+/* (a) Batched Pfaffian + inverse of real skew-symmetric planes.
+ * Replaces mvmc_gpu::BatchedEngine::run_f64 (crates/mvmc-gpu/src/lib.rs:104-117,
+ * semantics lib.rs:1-80): planes [n,n,count] column-major (plane p = q + NQP*b),
+ * n even >= 2; outputs pf[count], inv [n,n,count] = the TRUE inverse A^{-1}
+ * (mVMC's invM = -inv is applied by the caller), status[count]: 0 ok,
+ * k > 0 zero pivot at 1-based row k, -1 non-finite Pfaffian; failed planes are
+ * zero-filled with pf = 0.  All pointers are device pointers; `stream` is a
+ * hipStream_t (NULL = default stream).  Asynchronous on `stream`. */
+int mvmc_pfinv_f64(const double* planes, int n, int count, double* pf, double* inv, int* status, void* stream);
+
+/* In-place form (crates/mvmc-core/src/stage_backend.rs:145-155): planes are
+ * overwritten with the true inverse. */
+int mvmc_pfinv_f64_inplace(double* planes, int n, int count, double* pf, int* status, void* stream);
+
+/* Walker ensemble: W independent chains (= W ungrouped C ranks, walker w
+ * seeded RndSeed + w; crates/mvmc-core/src/multichain.rs:88-96) of one
+ * Expert-mode model, resident in HBM.  Covers the SamplingEngine/RealPfStage
+ * seam (crates/mvmc-core/src/sampling/{engine.rs:134-168,stage.rs:56-128}),
+ * the MeasurementStages seam (crates/mvmc-core/src/stage_backend.rs:418-438)
+ * and the SrStages::cg_step seam (crates/mvmc-core/src/sr_backend.rs:128-292). */
+typedef struct mvmc_ens mvmc_ens;
+
+/* Load namelist.def (+ optional initial.def, NULL = none: C init_parameter
+ * draws) and allocate the ensemble.  seed_override is used when
+ * use_seed_override != 0, else RndSeed from modpara.def. */
+int mvmc_ens_create(const char* namelist, const char* initial_def, int walkers, long long seed_override,
+                    int use_seed_override, mvmc_ens** out);
+int mvmc_ens_destroy(mvmc_ens* e);
+
+/* dims[16]: Nsite, Ne, Nsize, NQP, NProj, NSlater, SROptSize, NVMCSample,
+ * walkers, NExUpdatePath, NVMCInterval, NVMCWarmUp, persistent grid, lds bytes
+ * (sampler), lds bytes (measure), reserved. */
+int mvmc_ens_dims(mvmc_ens* e, int* dims16);
+
+/* Variational parameters (host arrays): projection [NProj], Slater [NSlater]. */
+int mvmc_ens_get_params(mvmc_ens* e, double* proj, double* slater);
+int mvmc_ens_set_params(mvmc_ens* e, const double* proj, const double* slater);
+
+/* One SR step's sampling on every walker (vmc_make_sample,
+ * crates/mvmc-core/src/sampling/driver.rs:175-458): first call warm-up
+ * (NVMCWarmUp + NVMCSample outer steps), later calls restore the burn state
+ * (NVMCSample + 1 outer steps); NVMCInterval*Nsite proposals per outer step. */
+int mvmc_ens_sample(mvmc_ens* e);
+
+/* Measurement of the saved samples + rank-order reductions
+ * (crates/mvmc-core/src/run.rs:2937-3616, 1863-1903; average.rs:18-63). */
+int mvmc_ens_measure(mvmc_ens* e);
+
+/* SR-CG solve on the reduced operands, parameter update and gauge sync
+ * (crates/mvmc-core/src/sr_cg.rs:201-484, sync.rs:18-104).  cg_info gets the C
+ * status (iterations or -(iter+1)). */
+int mvmc_ens_sr_cg(mvmc_ens* e, long long* cg_info);
+
+/* sample + measure + SR-CG; energy_row6 gets the zvo_out.dat row
+ * (Etot, Im Etot, Etot2, (Etot2-Etot^2)/Etot^2, Sztot, Sztot2). */
+int mvmc_ens_opt_step(mvmc_ens* e, double* energy_row6);
+
+/* Results for parity checks (host arrays). */
+int mvmc_ens_get_energy(mvmc_ens* e, double* wc_etot_etot2_sz_sz2);
+int mvmc_ens_get_operands(mvmc_ens* e, double* oo_2size, double* ho_size);
+int mvmc_ens_get_o_store(mvmc_ens* e, int walker, double* o_nsample_size);
+int mvmc_ens_get_walker(mvmc_ens* e, int walker, int* saved_ele_idx, long long* counters4, uint32_t* rng625);
+int mvmc_ens_get_tables(mvmc_ens* e, int walker, double* inv, double* pf);
+int mvmc_ens_get_config(mvmc_ens* e, int walker, int* ele_idx, int* ele_cfg, int* ele_num, int* proj_cnt);
+int mvmc_ens_get_cg(mvmc_ens* e, double* solution, int* n_active);
+/* timings[8]: sampling s, measurement s, SR s (wall, summed over calls),
+ * proposals, accepted, hop-kernel device ms of the last k_block launch set,
+ * k_block launches, reserved. */
+int mvmc_ens_timings(mvmc_ens* e, double* timings8);
+/* Sticky device status word (0 ok; 1 init retries exhausted, 2 bad input,
+ * 3 rebuild failure, 4 non-finite, 5 CG failure). */
+int mvmc_ens_status(mvmc_ens* e);
+
+/* SrStages::cg_step on caller-provided device operands: samples packed
+ * [nrows][na] in walker order (nrows = W*nsample), mean/diag/g [na];
+ * solves (S + shift diag) x = g with S = O^T O*inv_w - mean mean^T
+ * (sr_cg.rs:611-880); variant 0 plain, 1 diagonal scaling. */
+int mvmc_sr_cg_solve(int na, int walkers, int nsample, const double* pack, const double* mean, const double* diag,
+                     const double* g, double inv_w, double shift, double tol, int max_it, int variant, double* x,
+                     long long* status, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

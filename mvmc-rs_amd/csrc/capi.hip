@@ -1,0 +1,832 @@
+// Host side of libmvmc_hip.so: the C ABI of include/mvmc_hip.h, the walker
+// ensemble's device memory, and the SR-step driver that sequences the
+// kernels of ensemble.hip (crates/mvmc-core/src/run.rs:1294-1475 for the
+// step order; multichain.rs:647-689 for the walker = rank mapping).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mvmc_hip.h"
+#include "../host/model.h"
+#include "ensemble.hip"
+
+using namespace mvmc_hip;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Fail : std::runtime_error {
+  int code;
+  Fail(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                                 \
+  do {                                                                                            \
+    hipError_t e_ = (x);                                                                          \
+    if (e_ != hipSuccess)                                                                         \
+      throw Fail(MVMC_E_BACKEND, std::string(#x) + ": " + hipGetErrorString(e_) + " (" + __FILE__ + \
+                                     ":" + std::to_string(__LINE__) + ")");                       \
+  } while (0)
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <class F>
+int guarded(F f) {
+  try {
+    f();
+    return MVMC_OK;
+  } catch (Fail& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return MVMC_E_INPUT;
+  }
+}
+
+void require_gpu() {
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) throw Fail(MVMC_E_UNAVAILABLE, "no HIP device available");
+}
+
+// Device scratch of the SR-CG solver.
+struct CgWork {
+  double *x = nullptr, *r = nullptr, *d = nullptr, *q = nullptr, *y = nullptr, *g = nullptr;
+  double* sc = nullptr;  // scalars: 0 delta, 1 dq, 2 alpha, 3 dnew, 4 beta, 5 coef
+  int* flag = nullptr;   // 0 alpha fail, 1 beta fail
+  int cap_n = 0, cap_rows = 0;
+  void ensure(int na, int rows) {
+    if (na <= cap_n && rows <= cap_rows) return;
+    release();
+    cap_n = std::max(na, 1);
+    cap_rows = std::max(rows, 1);
+    HIPCHK(hipMalloc(&x, sizeof(double) * cap_n));
+    HIPCHK(hipMalloc(&r, sizeof(double) * cap_n));
+    HIPCHK(hipMalloc(&d, sizeof(double) * cap_n));
+    HIPCHK(hipMalloc(&q, sizeof(double) * cap_n));
+    HIPCHK(hipMalloc(&g, sizeof(double) * cap_n));
+    HIPCHK(hipMalloc(&y, sizeof(double) * cap_rows));
+    HIPCHK(hipMalloc(&sc, sizeof(double) * 8));
+    HIPCHK(hipMalloc(&flag, sizeof(int) * 4));
+  }
+  void release() {
+    for (void* p : {(void*)x, (void*)r, (void*)d, (void*)q, (void*)y, (void*)g, (void*)sc, (void*)flag})
+      if (p) (void)hipFree(p);
+    x = r = d = q = y = g = sc = nullptr;
+    flag = nullptr;
+    cap_n = cap_rows = 0;
+  }
+};
+
+}  // namespace
+
+namespace mvmc_hip {
+
+__global__ void k_cg_precond(int n, double pfac, const double* diag, const double* r, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double EPS = 1.0e-12;
+  double dq = pfac * diag[i];
+  if (!(dq - dq == 0.0) || fabs(dq) < EPS) dq = dq < 0.0 ? -EPS : EPS;
+  out[i] = r[i] / dq;
+}
+
+// alpha step guarded by the device flag; x += alpha d; r -= alpha q unless
+// this is a refresh iteration (then r is recomputed by the caller).
+__global__ void k_cg_update_x(int n, const double* sc, const int* flag, const double* d, double* x, const double* q,
+                              double* r, int refresh) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || flag[0]) return;
+  const double a = sc[2];
+  x[i] += a * d[i];
+  if (!refresh) r[i] -= a * q[i];
+}
+
+__global__ void k_cg_alpha2(double* sc, int* flag) {
+  if (threadIdx.x || blockIdx.x) return;
+  const double q = sc[1];
+  flag[0] = 0;
+  if (!(q - q == 0.0) || fabs(q) < 1.0e-20) { flag[0] = 1; return; }
+  const double a = sc[0] / q;
+  if (!(a - a == 0.0)) { flag[0] = 1; return; }
+  sc[2] = a;
+}
+
+// beta = dnew/delta; plain: delta *= beta; diag-scale: delta = dnew.
+__global__ void k_cg_beta2(double* sc, int* flag, int diag_scale) {
+  if (threadIdx.x || blockIdx.x) return;
+  flag[1] = 0;
+  if (flag[0]) return;
+  const double dn = sc[3], dl = sc[0];
+  if (!(dn - dn == 0.0) || fabs(dl) < 1.0e-20) { flag[1] = 1; return; }
+  const double b = dn / dl;
+  if (!(b - b == 0.0)) { flag[1] = 1; return; }
+  sc[4] = b;
+  sc[0] = diag_scale ? dn : dl * b;
+}
+
+__global__ void k_cg_dir2(int n, const double* sc, const int* flag, const double* src, double* d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || flag[0] || flag[1]) return;
+  d[i] = src[i] + sc[4] * d[i];
+}
+
+__global__ void k_fill(double* p, int n, double v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace mvmc_hip
+
+namespace {
+
+inline int nblk(long long n, int b = 256) { return (int)((n + b - 1) / b); }
+
+// z = S x with the sampled operator (sr_cg.rs:952-1004)
+void cg_apply_dev(int na, int W, int NS, const double* pack, const double* mean, const double* diag,
+                  const double* xv, double* z, double inv_w, double shift, CgWork& wk, hipStream_t st) {
+  const int rows = W * NS;
+  hipLaunchKernelGGL(k_cg_y, dim3(nblk(rows)), dim3(256), 0, st, pack, rows, na, xv, wk.y);
+  hipLaunchKernelGGL(k_seq_dot, dim3(1), dim3(64), 0, st, mean, xv, na, wk.sc + 5);
+  hipLaunchKernelGGL(k_cg_z, dim3(nblk(na)), dim3(256), 0, st, pack, W, NS, na, wk.y, xv, mean, diag, wk.sc + 5,
+                     inv_w, shift, z);
+}
+
+// C 1.4.0 StochasticOptCG_Main / PreconCG_DiagScale (sr_cg.rs:700-880).
+long long cg_solve_dev(int na, int W, int NS, const double* pack, const double* mean, const double* diag,
+                       const double* g, double inv_w, double shift, double tol, int max_it, bool diag_scale,
+                       double* x_out, CgWork& wk, hipStream_t st) {
+  wk.ensure(na, W * NS);
+  const double threshold = tol * tol * (double)na * (double)na;
+  const double pfac = 1.0 + shift;
+  HIPCHK(hipMemcpyAsync(wk.g, g, sizeof(double) * na, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_fill, dim3(nblk(na)), dim3(256), 0, st, wk.x, na, 0.0);
+  HIPCHK(hipMemcpyAsync(wk.r, g, sizeof(double) * na, hipMemcpyDeviceToDevice, st));
+  if (diag_scale) {
+    cg_apply_dev(na, W, NS, pack, mean, diag, wk.x, wk.q, inv_w, shift, wk, st);
+    hipLaunchKernelGGL(k_cg_resid, dim3(nblk(na)), dim3(256), 0, st, na, wk.g, wk.q, wk.r);
+    hipLaunchKernelGGL(k_cg_precond, dim3(nblk(na)), dim3(256), 0, st, na, pfac, diag, wk.r, wk.d);
+    hipLaunchKernelGGL(k_seq_dot, dim3(1), dim3(64), 0, st, wk.r, wk.d, na, wk.sc + 0);
+  } else {
+    HIPCHK(hipMemcpyAsync(wk.d, g, sizeof(double) * na, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_seq_dot, dim3(1), dim3(64), 0, st, wk.r, wk.r, na, wk.sc + 0);
+  }
+  double delta;
+  HIPCHK(hipMemcpyAsync(&delta, wk.sc, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  auto finish = [&](long long status) {
+    HIPCHK(hipMemcpyAsync(x_out, wk.x, sizeof(double) * na, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return status;
+  };
+  if (!std::isfinite(delta)) return finish(-1);
+  int iter = 0;
+  while (iter < max_it) {
+    const long long failed = -(long long)(iter + 1);
+    if (!std::isfinite(delta)) return finish(failed);
+    if (std::fabs(delta) < threshold) break;
+    cg_apply_dev(na, W, NS, pack, mean, diag, wk.d, wk.q, inv_w, shift, wk, st);
+    hipLaunchKernelGGL(k_seq_dot, dim3(1), dim3(64), 0, st, wk.d, wk.q, na, wk.sc + 1);
+    hipLaunchKernelGGL(k_cg_alpha2, dim3(1), dim3(64), 0, st, wk.sc, wk.flag);
+    const int refresh = ((iter + 1) % 20 == 0);
+    hipLaunchKernelGGL(k_cg_update_x, dim3(nblk(na)), dim3(256), 0, st, na, wk.sc, wk.flag, wk.d, wk.x, wk.q, wk.r,
+                       refresh);
+    if (refresh) {
+      // r = g - S x (the product needs no guard: x is unchanged on failure and
+      // the result is discarded by the failure return below)
+      cg_apply_dev(na, W, NS, pack, mean, diag, wk.x, wk.q, inv_w, shift, wk, st);
+      hipLaunchKernelGGL(k_cg_resid, dim3(nblk(na)), dim3(256), 0, st, na, wk.g, wk.q, wk.r);
+    }
+    if (diag_scale) {
+      hipLaunchKernelGGL(k_cg_precond, dim3(nblk(na)), dim3(256), 0, st, na, pfac, diag, wk.r, wk.q);
+      hipLaunchKernelGGL(k_seq_dot, dim3(1), dim3(64), 0, st, wk.r, wk.q, na, wk.sc + 3);
+    } else {
+      hipLaunchKernelGGL(k_seq_dot, dim3(1), dim3(64), 0, st, wk.r, wk.r, na, wk.sc + 3);
+    }
+    hipLaunchKernelGGL(k_cg_beta2, dim3(1), dim3(64), 0, st, wk.sc, wk.flag, diag_scale ? 1 : 0);
+    hipLaunchKernelGGL(k_cg_dir2, dim3(nblk(na)), dim3(256), 0, st, na, wk.sc, wk.flag, diag_scale ? wk.q : wk.r,
+                       wk.d);
+    int flags[2];
+    double sc0;
+    HIPCHK(hipMemcpyAsync(flags, wk.flag, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&sc0, wk.sc, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (flags[0] || flags[1]) return finish(failed);
+    delta = sc0;
+    ++iter;
+  }
+  if (iter == max_it && std::fabs(delta) >= threshold) return finish(-(long long)(max_it + 1));
+  return finish(iter);
+}
+
+template <class T>
+T* dalloc(std::vector<void*>& list, size_t count) {
+  void* p = nullptr;
+  if (count == 0) count = 1;
+  HIPCHK(hipMalloc(&p, sizeof(T) * count));
+  list.push_back(p);
+  return (T*)p;
+}
+
+template <class T>
+T* dupload(std::vector<void*>& list, const std::vector<T>& h) {
+  T* p = dalloc<T>(list, h.size());
+  if (!h.empty()) HIPCHK(hipMemcpy(p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+  return p;
+}
+
+}  // namespace
+
+struct mvmc_ens {
+  mvmc::Model m;
+  EnsView v{};
+  std::vector<void*> allocs;
+  hipStream_t st = nullptr;
+  int grid_samp = 0, grid_meas = 0, slots = 0;
+  size_t lds_samp = 0, lds_meas = 0;
+  double* d_slater = nullptr;
+  double* d_proj = nullptr;
+  double* d_dg = nullptr;
+  double* d_dj = nullptr;
+  double* d_out5 = nullptr;
+  double *d_ho = nullptr, *d_mean = nullptr, *d_diag = nullptr;
+  double* d_gscr = nullptr;
+  int gstride = 1;
+  double* d_pack = nullptr;
+  size_t pack_cap = 0;
+  int* d_map = nullptr;
+  double *d_cmean = nullptr, *d_cdiag = nullptr, *d_cg = nullptr, *d_cx = nullptr;
+  CgWork cg;
+  // host results
+  double wc = 0, etot = 0, etot2 = 0, sztot = 0, sztot2 = 0;
+  std::vector<double> oo, ho;
+  std::vector<double> cg_x;
+  long long cg_info = 0;
+  double t_sample = 0, t_measure = 0, t_sr = 0, proposals = 0, accepted = 0;
+  float block_ms = 0;
+  int block_launches = 0;
+  ~mvmc_ens() {
+    cg.release();
+    for (void* p : allocs) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+  }
+  void upload_tables() {
+    std::vector<double> S, g, j;
+    mvmc::build_slater_table(m, S);
+    mvmc::direct_projection_tables(m, g, j);
+    HIPCHK(hipMemcpyAsync(d_slater, S.data(), sizeof(double) * S.size(), hipMemcpyHostToDevice, st));
+    if (!m.proj.empty())
+      HIPCHK(hipMemcpyAsync(d_proj, m.proj.data(), sizeof(double) * m.proj.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_dg, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_dj, j.data(), sizeof(double) * j.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  void check_status() {
+    int s = 0;
+    HIPCHK(hipMemcpy(&s, v.status, sizeof(int), hipMemcpyDeviceToHost));
+    if (s == ST_BAD_INPUT || s == ST_INIT_RETRY)
+      throw Fail(MVMC_E_NUMERIC, "device status " + std::to_string(s));
+  }
+};
+
+namespace {
+
+void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int walkers, long long seed_override,
+                 int use_seed) {
+  require_gpu();
+  if (walkers < 1) throw Fail(MVMC_E_INVALID_SHAPE, "walkers must be >= 1");
+  mvmc::Model& m = E->m;
+  m = mvmc::load_namelist(namelist);
+  if (m.mp.vmc_calc_mode != 0) throw Fail(MVMC_E_INPUT, "only NVMCCalMode 0 (optimisation) is supported");
+  if (m.ne() < 1 || m.ne() > m.nsite()) throw Fail(MVMC_E_INPUT, "Ne must be in [1, Nsite]");
+  if (m.nqp() < 1) throw Fail(MVMC_E_INPUT, "no quantum-projection planes (NMPTrans/TransSym)");
+  {
+    // every translated site must be a site (the kernels index tables with it)
+    std::vector<int> ts, tg;
+    mvmc::translation_maps(m, ts, tg);
+    for (int t : ts)
+      if (t < 0 || t >= m.nsite()) throw Fail(MVMC_E_INPUT, "TransSym maps a site outside [0, Nsite)");
+    // placement loops terminate: local spins fit, itinerant electrons find free sites
+    int nloc = 0;
+    for (int f : m.loc_spn) nloc += (f == 1);
+    if (nloc > 2 * m.ne()) throw Fail(MVMC_E_INPUT, "more local spins than electrons");
+    if (nloc < 2 * m.ne() && m.nsite() - nloc < m.ne())
+      throw Fail(MVMC_E_INPUT, "not enough itinerant sites for the itinerant electrons");
+    if (m.mp.nex_update_path < 0 || m.mp.nex_update_path > 2)
+      throw Fail(MVMC_E_INPUT, "NExUpdatePath must be 0, 1 or 2 for the normal real sampler");
+  }
+  const long long seed = use_seed ? seed_override : m.mp.rnd_seed;
+  // init_parameter (parameter_init.rs:63-160) on walker 0's stream; every
+  // walker consumes the same number of draws before sampling.
+  int n_active = 0;
+  {
+    SfmtLds r;
+    sfmt_seed_host(r.w, (uint32_t)seed);
+    r.idx = 624;
+    for (int i = 0; i < m.n_slater; ++i) {
+      const bool active = m.opt_flags[2 * (m.n_proj() + i)] > 0;
+      m.slater[i] = 0.0;
+      if (active) {
+        m.slater[i] = 2.0 * (sfmt_real2(r) - 0.5);
+        ++n_active;
+      }
+    }
+    for (double& p : m.proj) p = 0.0;
+  }
+  if (initial_def && *initial_def) {
+    if (!mvmc::load_initial_def(m, initial_def)) throw Fail(MVMC_E_INPUT, std::string("cannot read ") + initial_def);
+  }
+  mvmc::sync_modified_parameter(m, true);
+
+  HIPCHK(hipStreamCreateWithFlags(&E->st, hipStreamNonBlocking));
+  EnsView& v = E->v;
+  auto& A = E->allocs;
+  v.W = walkers;
+  v.n = m.nsize(); v.ne = m.ne(); v.ns = m.nsite(); v.n2 = m.nsite2();
+  v.nqp = m.nqp(); v.nleg = m.n_sp_gl(); v.nmp = m.n_mp_trans();
+  v.nproj = m.n_proj(); v.ngutz = m.n_gutz; v.nslater = m.n_slater; v.size = m.sr_opt_size();
+  v.nsample = m.mp.nvmc_sample; v.warmup = m.mp.nvmc_warmup; v.interval = m.mp.nvmc_interval;
+  v.ex_path = m.mp.nex_update_path;
+  if (v.nsample < 1) throw Fail(MVMC_E_INPUT, "NVMCSample must be >= 1");
+  // orbital rows injective -> ordered parallel scatter in the Slater derivative
+  int inj = 1;
+  for (int a = 0; a < v.ns && inj; ++a) {
+    std::set<int> seen;
+    for (int b = 0; b < v.ns; ++b) {
+      int o = m.orb_idx[(size_t)a * v.ns + b];
+      if (o >= 0 && !seen.insert(o).second) { inj = 0; break; }
+    }
+  }
+  v.orb_injective = inj;
+  v.exp_cr = 1;
+  // model tables
+  E->d_slater = dalloc<double>(A, (size_t)v.nqp * v.n2 * v.n2);
+  v.slater = E->d_slater;
+  v.qpw = dupload(A, m.qp_full_weight);
+  E->d_proj = dalloc<double>(A, std::max(v.nproj, 1));
+  v.proj = E->d_proj;
+  v.gutz_idx = dupload(A, m.gutz_idx);
+  v.jast_idx = dupload(A, m.jast_idx);
+  v.loc_spn = dupload(A, m.loc_spn);
+  {
+    std::vector<int> t;
+    std::vector<double> tv;
+    for (auto& x : m.transfer) {
+      t.insert(t.end(), {x.site1, x.spin1, x.site2, x.spin2});
+      tv.push_back(x.re);
+    }
+    v.ntrans = (int)m.transfer.size();
+    v.trans = dupload(A, t);
+    v.trans_v = dupload(A, tv);
+    std::vector<int> is;
+    std::vector<double> iv;
+    for (auto& x : m.coulomb_intra) { is.push_back(x.first); iv.push_back(x.second); }
+    v.nintra = (int)is.size();
+    v.intra_site = dupload(A, is);
+    v.intra_v = dupload(A, iv);
+    auto pairs = [&](const std::vector<mvmc::PairTerm>& P, const int*& site, const double*& val, int& cnt) {
+      std::vector<int> s;
+      std::vector<double> vv;
+      for (auto& x : P) { s.push_back(x.site1); s.push_back(x.site2); vv.push_back(x.value); }
+      cnt = (int)P.size();
+      site = dupload(A, s);
+      val = dupload(A, vv);
+    };
+    pairs(m.coulomb_inter, v.inter_site, v.inter_v, v.ninter);
+    pairs(m.hund, v.hund_site, v.hund_v, v.nhund);
+    pairs(m.exchange, v.exch_site, v.exch_v, v.nexch);
+  }
+  E->d_dg = dalloc<double>(A, v.ns);
+  E->d_dj = dalloc<double>(A, (size_t)v.ns * v.ns);
+  v.dproj_g = E->d_dg;
+  v.dproj_j = E->d_dj;
+  {
+    std::vector<int> ts, tg;
+    mvmc::translation_maps(m, ts, tg);
+    v.tsite = dupload(A, ts);
+    v.tsgn = dupload(A, tg);
+  }
+  v.orb_idx = dupload(A, m.orb_idx);
+  v.orb_sgn = dupload(A, m.orb_sgn);
+  v.spgl_cs = dupload(A, m.spgl_cos_sin);
+  v.spgl_cc = dupload(A, m.spgl_cos_cos);
+  v.spgl_ss = dupload(A, m.spgl_sin_sin);
+  // walker state
+  const size_t W = (size_t)walkers;
+  v.rng = dalloc<uint32_t>(A, W * 625);
+  v.ele_idx = dalloc<int>(A, W * v.n);
+  v.ele_cfg = dalloc<int>(A, W * v.n2);
+  v.ele_num = dalloc<int>(A, W * v.n2);
+  v.proj_cnt = dalloc<int>(A, W * std::max(v.nproj, 1));
+  v.inv = dalloc<double>(A, W * v.nqp * v.n * v.n);
+  v.pf = dalloc<double>(A, W * v.nqp);
+  v.log_ip = dalloc<double>(A, W);
+  v.wstate = dalloc<int>(A, W * 8);
+  v.counter = dalloc<long long>(A, W * 4);
+  const size_t WS = W * v.nsample;
+  v.s_idx = dalloc<int>(A, WS * v.n);
+  v.s_cfg = dalloc<int>(A, WS * v.n2);
+  v.s_num = dalloc<int>(A, WS * v.n2);
+  v.s_proj = dalloc<int>(A, WS * std::max(v.nproj, 1));
+  v.e_loc = dalloc<double>(A, WS);
+  v.s_valid = dalloc<int>(A, WS);
+  v.sz = dalloc<double>(A, WS);
+  v.o_store = dalloc<double>(A, WS * v.size);
+  v.status = dalloc<int>(A, 1);
+  HIPCHK(hipMemset(v.status, 0, sizeof(int)));
+  HIPCHK(hipMemset(v.wstate, 0, sizeof(int) * W * 8));
+  HIPCHK(hipMemset(v.counter, 0, sizeof(long long) * W * 4));
+  HIPCHK(hipMemset(v.ele_idx, 0, sizeof(int) * W * v.n));
+  HIPCHK(hipMemset(v.ele_cfg, 0, sizeof(int) * W * v.n2));
+  HIPCHK(hipMemset(v.ele_num, 0, sizeof(int) * W * v.n2));
+  HIPCHK(hipMemset(v.proj_cnt, 0, sizeof(int) * W * std::max(v.nproj, 1)));
+  // persistent grids sized to the resident capacity
+  E->lds_samp = lds_bytes_sampler(v, 256);
+  E->lds_meas = lds_bytes_measure(v, 256);
+  int dev = 0, ncu = 0, maxlds = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  if (E->lds_samp > (size_t)maxlds || E->lds_meas > (size_t)maxlds)
+    throw Fail(MVMC_E_INVALID_SHAPE, "model needs " + std::to_string(std::max(E->lds_samp, E->lds_meas)) +
+                                         " B of LDS per workgroup, device allows " + std::to_string(maxlds));
+  HIPCHK(hipFuncSetAttribute((const void*)k_block, hipFuncAttributeMaxDynamicSharedMemorySize, (int)E->lds_samp));
+  HIPCHK(hipFuncSetAttribute((const void*)k_begin, hipFuncAttributeMaxDynamicSharedMemorySize, (int)E->lds_samp));
+  HIPCHK(hipFuncSetAttribute((const void*)k_measure, hipFuncAttributeMaxDynamicSharedMemorySize, (int)E->lds_meas));
+  int occ_s = 0, occ_m = 0;
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, k_block, 256, E->lds_samp));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_m, k_measure, 256, E->lds_meas));
+  occ_s = std::max(1, occ_s);
+  occ_m = std::max(1, occ_m);
+  E->grid_samp = (int)std::min<long long>(walkers, (long long)occ_s * ncu);
+  E->grid_meas = (int)std::min<long long>(walkers, (long long)occ_m * ncu);
+  E->slots = std::max(E->grid_samp, E->grid_meas);
+  v.work = dalloc<double>(A, (size_t)E->slots * v.nqp * v.n * v.n);
+  v.qorb = dalloc<double>(A, (size_t)E->slots * v.nqp * std::max(v.nslater, 1));
+  E->gstride = std::max(1, std::max(v.ntrans, v.nexch));
+  E->d_gscr = dalloc<double>(A, (size_t)E->slots * E->gstride);
+  E->d_out5 = dalloc<double>(A, 8);
+  E->d_ho = dalloc<double>(A, v.size);
+  E->d_mean = dalloc<double>(A, v.size);
+  E->d_diag = dalloc<double>(A, v.size);
+  E->d_map = dalloc<int>(A, v.size);
+  E->d_cmean = dalloc<double>(A, v.size);
+  E->d_cdiag = dalloc<double>(A, v.size);
+  E->d_cg = dalloc<double>(A, v.size);
+  E->d_cx = dalloc<double>(A, v.size);
+  hipLaunchKernelGGL(k_seed, dim3(nblk(walkers, 64)), dim3(64), 0, E->st, v, seed, n_active);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(E->st));
+}
+
+void sample_impl(mvmc_ens* E) {
+  EnsView& v = E->v;
+  const double t0 = now_s();
+  E->upload_tables();
+  hipLaunchKernelGGL(k_begin, dim3(E->grid_samp), dim3(256), E->lds_samp, E->st, v);
+  HIPCHK(hipGetLastError());
+  std::vector<int> ws((size_t)v.W * 8);
+  HIPCHK(hipMemcpyAsync(ws.data(), v.wstate, sizeof(int) * ws.size(), hipMemcpyDeviceToHost, E->st));
+  HIPCHK(hipStreamSynchronize(E->st));
+  E->check_status();
+  int n_out = 0;
+  for (int w = 0; w < v.W; ++w) n_out = std::max(n_out, ws[(size_t)w * 8 + 2]);
+  const int n_in = v.interval * std::max(v.ns, 1);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, E->st));
+  for (int out = 0; out < n_out; ++out) {
+    hipLaunchKernelGGL(k_block, dim3(E->grid_samp), dim3(256), E->lds_samp, E->st, v, out, n_in);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipEventRecord(e1, E->st));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  E->block_ms = ms;
+  E->block_launches = n_out;
+  E->check_status();
+  std::vector<long long> c((size_t)v.W * 4);
+  HIPCHK(hipMemcpy(c.data(), v.counter, sizeof(long long) * c.size(), hipMemcpyDeviceToHost));
+  for (int w = 0; w < v.W; ++w) {
+    E->proposals += (double)(c[(size_t)w * 4 + 0] + c[(size_t)w * 4 + 2]);
+    E->accepted += (double)(c[(size_t)w * 4 + 1] + c[(size_t)w * 4 + 3]);
+  }
+  E->t_sample += now_s() - t0;
+}
+
+void measure_impl(mvmc_ens* E) {
+  EnsView& v = E->v;
+  const double t0 = now_s();
+  hipLaunchKernelGGL(k_measure, dim3(E->grid_meas), dim3(256), E->lds_meas, E->st, v, E->d_gscr, E->gstride, 1);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_energy, dim3(1), dim3(64), 0, E->st, v, E->d_out5);
+  hipLaunchKernelGGL(k_reduce_ops, dim3(nblk(v.size)), dim3(256), 0, E->st, v, E->d_ho, E->d_mean, E->d_diag);
+  HIPCHK(hipGetLastError());
+  double o5[5];
+  std::vector<double> ho(v.size), mean(v.size), diag(v.size);
+  HIPCHK(hipMemcpyAsync(o5, E->d_out5, sizeof(double) * 5, hipMemcpyDeviceToHost, E->st));
+  HIPCHK(hipMemcpyAsync(ho.data(), E->d_ho, sizeof(double) * v.size, hipMemcpyDeviceToHost, E->st));
+  HIPCHK(hipMemcpyAsync(mean.data(), E->d_mean, sizeof(double) * v.size, hipMemcpyDeviceToHost, E->st));
+  HIPCHK(hipMemcpyAsync(diag.data(), E->d_diag, sizeof(double) * v.size, hipMemcpyDeviceToHost, E->st));
+  HIPCHK(hipStreamSynchronize(E->st));
+  E->check_status();
+  E->wc = o5[0]; E->etot = o5[1]; E->etot2 = o5[2]; E->sztot = o5[3]; E->sztot2 = o5[4];
+  E->oo.assign(2 * (size_t)v.size, 0.0);
+  for (int i = 0; i < v.size; ++i) { E->oo[i] = mean[i]; E->oo[v.size + i] = diag[i]; }
+  E->ho = ho;
+  // weight_average_we / weight_average_sr_opt_real (average.rs:18-63)
+  if (std::fabs(E->wc) >= 1.0e-15) {
+    const double inv_c = E->wc / (E->wc * E->wc + 0.0 * 0.0);
+    E->etot *= inv_c; E->etot2 *= inv_c; E->sztot *= inv_c; E->sztot2 *= inv_c;
+    const double inv = 1.0 / E->wc;
+    for (auto& x : E->oo) x *= inv;
+    for (auto& x : E->ho) x *= inv;
+  }
+  E->t_measure += now_s() - t0;
+}
+
+// sr_cg.rs:201-484 (real parameters, offset 1, no RescaleSmat)
+int sr_impl(mvmc_ens* E, long long* info_out) {
+  mvmc::Model& m = E->m;
+  EnsView& v = E->v;
+  const double t0 = now_s();
+  const int np = m.n_para(), size = m.sr_opt_size();
+  if (m.mp.nsrcg == 0) throw Fail(MVMC_E_INPUT, "the device solver implements SR-CG (NSRCG 1) only");
+  if (m.mp.rescale_smat) throw Fail(MVMC_E_INPUT, "RescaleSmat is not supported");
+  std::vector<double> var(np);
+  bool nan = false;
+  for (int pi = 0; pi < np; ++pi) {
+    const int idx = pi + 1;
+    var[pi] = E->oo[size + idx] - E->oo[idx] * E->oo[idx];
+    nan = nan || std::isnan(var[pi]);
+  }
+  double vmax = -INFINITY;
+  if (nan) vmax = NAN;
+  else
+    for (double x : var) vmax = std::fmax(vmax, x);
+  const double thr = vmax * m.mp.dsr_opt_red_cut;
+  std::vector<int> map;
+  for (int pi = 0; pi < np; ++pi) {
+    if (m.opt_flags[2 * pi] != 1) continue;
+    if (var[pi] < thr) continue;
+    map.push_back(pi);
+  }
+  E->cg_x.clear();
+  if (map.empty()) { *info_out = 0; E->cg_info = 0; E->t_sr += now_s() - t0; return 0; }
+  const int na = (int)map.size();
+  std::vector<double> mean(na), diag(na), g(na);
+  const double dt = 2.0 * m.mp.dsr_opt_step_dt;
+  for (int si = 0; si < na; ++si) {
+    const int idx = map[si] + 1;
+    mean[si] = E->oo[idx];
+    diag[si] = var[map[si]];
+    g[si] = -dt * (E->ho[idx] - E->ho[0] * E->oo[idx]);
+  }
+  hipStream_t st = E->st;
+  HIPCHK(hipMemcpyAsync(E->d_map, map.data(), sizeof(int) * na, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(E->d_cmean, mean.data(), sizeof(double) * na, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(E->d_cdiag, diag.data(), sizeof(double) * na, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(E->d_cg, g.data(), sizeof(double) * na, hipMemcpyHostToDevice, st));
+  const size_t rows = (size_t)v.W * v.nsample;
+  if (E->pack_cap < rows * na) {
+    if (E->d_pack) (void)hipFree(E->d_pack);
+    E->d_pack = nullptr;
+    HIPCHK(hipMalloc(&E->d_pack, sizeof(double) * rows * na));
+    E->pack_cap = rows * na;
+  }
+  hipLaunchKernelGGL(k_cg_pack, dim3(nblk((long long)(rows * na))), dim3(256), 0, st, v, E->d_map, na, E->d_pack);
+  HIPCHK(hipGetLastError());
+  const int max_it = m.mp.nsr_opt_cg_max_iter > 0 ? m.mp.nsr_opt_cg_max_iter : na;
+  const double inv_w = 1.0 / E->wc;
+  long long status = cg_solve_dev(na, v.W, v.nsample, E->d_pack, E->d_cmean, E->d_cdiag, E->d_cg, inv_w,
+                                  m.mp.dsr_opt_sta_del, m.mp.dsr_opt_cg_tol, max_it, m.mp.use_diag_scale != 0,
+                                  E->d_cx, E->cg, st);
+  long long info = status;
+  if (status < 0) {
+    if (m.mp.nsrcg_fallback) {
+      long long st2 = cg_solve_dev(na, v.W, v.nsample, E->d_pack, E->d_cmean, E->d_cdiag, E->d_cg, inv_w,
+                                   m.mp.dsr_opt_sta_del, m.mp.dsr_opt_cg_tol, max_it, m.mp.use_diag_scale == 0,
+                                   E->d_cx, E->cg, st);
+      info = -((-status - 1) + 1);
+      if (st2 < 0 && m.mp.nsrcg_abort_on_fail) throw Fail(MVMC_E_NUMERIC, "SR-CG fallback failed");
+    } else {
+      if (m.mp.nsrcg_abort_on_fail) throw Fail(MVMC_E_NUMERIC, "SR-CG failed (NSRCGAbortOnFail)");
+      info = -((-status - 1) + 1);
+    }
+  }
+  std::vector<double> x(na);
+  HIPCHK(hipMemcpy(x.data(), E->d_cx, sizeof(double) * na, hipMemcpyDeviceToHost));
+  E->cg_x = x;
+  E->cg_info = info;
+  *info_out = info;
+  for (double xv : x)
+    if (!std::isfinite(xv)) {
+      if (m.mp.nsrcg_abort_on_fail) throw Fail(MVMC_E_NUMERIC, "SR-CG produced non-finite updates");
+      E->t_sr += now_s() - t0;
+      return 1;
+    }
+  for (int si = 0; si < na; ++si) {
+    const int pi = map[si];
+    if (pi < m.n_proj()) m.proj[pi] += x[si];
+    else m.slater[pi - m.n_proj()] += x[si];
+  }
+  mvmc::sync_modified_parameter(m, true);
+  E->t_sr += now_s() - t0;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mvmc_last_error(void) { return g_err.c_str(); }
+int mvmc_abi_version(void) { return 1; }
+
+int mvmc_pfinv_f64(const double* planes, int n, int count, double* pf, double* inv, int* status, void* stream) {
+  return guarded([&] {
+    require_gpu();
+    if (n < 2 || (n % 2) != 0 || count < 0) throw Fail(MVMC_E_INVALID_SHAPE, "n must be even and >= 2");
+    if (count == 0) return;
+    if (!planes || !pf || !inv || !status) throw Fail(MVMC_E_INVALID_SHAPE, "null buffer");
+    hipStream_t st = (hipStream_t)stream;
+    if ((const double*)inv != planes)
+      HIPCHK(hipMemcpyAsync(inv, planes, sizeof(double) * n * n * count, hipMemcpyDeviceToDevice, st));
+    const int np = n <= 32 ? 8 : (n <= 64 ? 4 : 1);
+    const int blocks = (count + np - 1) / np;
+    double* work = nullptr;
+    HIPCHK(hipMallocAsync((void**)&work, sizeof(double) * (size_t)blocks * np * n * n, st));
+    size_t lds = rnd16((size_t)np * n * 4) + rnd16((size_t)np * n * 8) + 3 * rnd16(np * 8) + 64;
+    HIPCHK(hipFuncSetAttribute((const void*)k_pfinv_batched, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_pfinv_batched, dim3(blocks), dim3(256), lds, st, inv, work, n, count, np, pf, status);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipFreeAsync(work, st));
+  });
+}
+
+int mvmc_pfinv_f64_inplace(double* planes, int n, int count, double* pf, int* status, void* stream) {
+  return mvmc_pfinv_f64(planes, n, count, pf, planes, status, stream);
+}
+
+int mvmc_ens_create(const char* namelist, const char* initial_def, int walkers, long long seed_override,
+                    int use_seed_override, mvmc_ens** out) {
+  if (!out) return MVMC_E_INVALID_SHAPE;
+  *out = nullptr;
+  mvmc_ens* E = new mvmc_ens();
+  int rc = guarded([&] { create_impl(E, namelist, initial_def, walkers, seed_override, use_seed_override); });
+  if (rc != MVMC_OK) {
+    delete E;
+    return rc;
+  }
+  *out = E;
+  return MVMC_OK;
+}
+
+int mvmc_ens_destroy(mvmc_ens* e) {
+  delete e;
+  return MVMC_OK;
+}
+
+int mvmc_ens_dims(mvmc_ens* e, int* d) {
+  const EnsView& v = e->v;
+  int vals[16] = {v.ns, v.ne, v.n, v.nqp, v.nproj, v.nslater, v.size, v.nsample, v.W, v.ex_path, v.interval,
+                  v.warmup, e->grid_samp, (int)e->lds_samp, (int)e->lds_meas, e->grid_meas};
+  std::memcpy(d, vals, sizeof(vals));
+  return MVMC_OK;
+}
+
+int mvmc_ens_get_params(mvmc_ens* e, double* proj, double* slater) {
+  std::copy(e->m.proj.begin(), e->m.proj.end(), proj);
+  std::copy(e->m.slater.begin(), e->m.slater.end(), slater);
+  return MVMC_OK;
+}
+
+int mvmc_ens_set_params(mvmc_ens* e, const double* proj, const double* slater) {
+  std::copy(proj, proj + e->m.n_proj(), e->m.proj.begin());
+  std::copy(slater, slater + e->m.n_slater, e->m.slater.begin());
+  return MVMC_OK;
+}
+
+int mvmc_ens_sample(mvmc_ens* e) { return guarded([&] { sample_impl(e); }); }
+int mvmc_ens_measure(mvmc_ens* e) { return guarded([&] { measure_impl(e); }); }
+int mvmc_ens_sr_cg(mvmc_ens* e, long long* info) {
+  long long dummy = 0;
+  return guarded([&] { sr_impl(e, info ? info : &dummy); });
+}
+
+int mvmc_ens_opt_step(mvmc_ens* e, double* row) {
+  return guarded([&] {
+    sample_impl(e);
+    measure_impl(e);
+    if (row) {
+      double var = 0.0;
+      if (std::fabs(e->etot) > 1.0e-14) var = (e->etot2 - e->etot * e->etot) / (e->etot * e->etot);
+      const double r6[6] = {e->etot, 0.0, e->etot2, var, e->sztot, e->sztot2};
+      std::memcpy(row, r6, sizeof(r6));
+    }
+    long long info = 0;
+    int rc = sr_impl(e, &info);
+    if (rc != 0) throw Fail(MVMC_E_NUMERIC, "SR step skipped (non-finite update)");
+  });
+}
+
+int mvmc_ens_get_energy(mvmc_ens* e, double* o) {
+  o[0] = e->wc; o[1] = e->etot; o[2] = e->etot2; o[3] = e->sztot; o[4] = e->sztot2;
+  return MVMC_OK;
+}
+
+int mvmc_ens_get_operands(mvmc_ens* e, double* oo, double* ho) {
+  if (oo) std::copy(e->oo.begin(), e->oo.end(), oo);
+  if (ho) std::copy(e->ho.begin(), e->ho.end(), ho);
+  return MVMC_OK;
+}
+
+int mvmc_ens_get_o_store(mvmc_ens* e, int w, double* o) {
+  return guarded([&] {
+    const EnsView& v = e->v;
+    if (w < 0 || w >= v.W) throw Fail(MVMC_E_INVALID_SHAPE, "walker out of range");
+    HIPCHK(hipMemcpy(o, v.o_store + (size_t)w * v.nsample * v.size, sizeof(double) * v.nsample * v.size,
+                     hipMemcpyDeviceToHost));
+  });
+}
+
+int mvmc_ens_get_walker(mvmc_ens* e, int w, int* saved, long long* counters, uint32_t* rng) {
+  return guarded([&] {
+    const EnsView& v = e->v;
+    if (w < 0 || w >= v.W) throw Fail(MVMC_E_INVALID_SHAPE, "walker out of range");
+    if (saved)
+      HIPCHK(hipMemcpy(saved, v.s_idx + (size_t)w * v.nsample * v.n, sizeof(int) * v.nsample * v.n,
+                       hipMemcpyDeviceToHost));
+    if (counters) HIPCHK(hipMemcpy(counters, v.counter + (size_t)w * 4, sizeof(long long) * 4, hipMemcpyDeviceToHost));
+    if (rng) HIPCHK(hipMemcpy(rng, v.rng + (size_t)w * 625, sizeof(uint32_t) * 625, hipMemcpyDeviceToHost));
+  });
+}
+
+int mvmc_ens_get_tables(mvmc_ens* e, int w, double* inv, double* pf) {
+  return guarded([&] {
+    const EnsView& v = e->v;
+    if (w < 0 || w >= v.W) throw Fail(MVMC_E_INVALID_SHAPE, "walker out of range");
+    const size_t nn = (size_t)v.n * v.n;
+    if (inv) HIPCHK(hipMemcpy(inv, v.inv + (size_t)w * v.nqp * nn, sizeof(double) * v.nqp * nn, hipMemcpyDeviceToHost));
+    if (pf) HIPCHK(hipMemcpy(pf, v.pf + (size_t)w * v.nqp, sizeof(double) * v.nqp, hipMemcpyDeviceToHost));
+  });
+}
+
+int mvmc_ens_get_config(mvmc_ens* e, int w, int* ei, int* ec, int* en, int* pc) {
+  return guarded([&] {
+    const EnsView& v = e->v;
+    if (w < 0 || w >= v.W) throw Fail(MVMC_E_INVALID_SHAPE, "walker out of range");
+    if (ei) HIPCHK(hipMemcpy(ei, v.ele_idx + (size_t)w * v.n, sizeof(int) * v.n, hipMemcpyDeviceToHost));
+    if (ec) HIPCHK(hipMemcpy(ec, v.ele_cfg + (size_t)w * v.n2, sizeof(int) * v.n2, hipMemcpyDeviceToHost));
+    if (en) HIPCHK(hipMemcpy(en, v.ele_num + (size_t)w * v.n2, sizeof(int) * v.n2, hipMemcpyDeviceToHost));
+    if (pc && v.nproj > 0)
+      HIPCHK(hipMemcpy(pc, v.proj_cnt + (size_t)w * v.nproj, sizeof(int) * v.nproj, hipMemcpyDeviceToHost));
+  });
+}
+
+int mvmc_ens_get_cg(mvmc_ens* e, double* x, int* n_active) {
+  if (x) std::copy(e->cg_x.begin(), e->cg_x.end(), x);
+  if (n_active) *n_active = (int)e->cg_x.size();
+  return (int)e->cg_info;
+}
+
+int mvmc_ens_timings(mvmc_ens* e, double* t) {
+  t[0] = e->t_sample; t[1] = e->t_measure; t[2] = e->t_sr; t[3] = e->proposals; t[4] = e->accepted;
+  t[5] = e->block_ms; t[6] = e->block_launches; t[7] = 0;
+  return MVMC_OK;
+}
+
+int mvmc_ens_status(mvmc_ens* e) {
+  int s = -1;
+  if (hipMemcpy(&s, e->v.status, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  return s;
+}
+
+int mvmc_sr_cg_solve(int na, int walkers, int nsample, const double* pack, const double* mean, const double* diag,
+                     const double* g, double inv_w, double shift, double tol, int max_it, int variant, double* x,
+                     long long* status, void* stream) {
+  static CgWork wk;
+  return guarded([&] {
+    require_gpu();
+    if (na < 1 || walkers < 1 || nsample < 0) throw Fail(MVMC_E_INVALID_SHAPE, "bad CG dimensions");
+    *status = cg_solve_dev(na, walkers, nsample, pack, mean, diag, g, inv_w, shift, tol, max_it, variant != 0, x, wk,
+                           (hipStream_t)stream);
+  });
+}
+
+}  // extern "C"

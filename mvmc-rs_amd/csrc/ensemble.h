@@ -1,0 +1,62 @@
+// Device view of a walker ensemble: model tables + per-walker state, all in
+// HBM.  Passed by value to every kernel.
+#pragma once
+#include <stdint.h>
+
+namespace mvmc_hip {
+
+struct EnsView {
+  // dimensions
+  int W;          // walkers
+  int n, ne, ns, n2;  // Nsize, Ne, Nsite, 2*Nsite
+  int nqp, nleg, nmp;
+  int nproj, ngutz, nslater, size;  // size = 1 + NPara
+  int nsample, warmup, interval, ex_path;
+  int ntrans, nintra, ninter, nhund, nexch;
+  int orb_injective;  // each orbital-index row is injective
+  int exp_cr;         // 1: correctly rounded exp/log (always on the device)
+  // model tables
+  const double* slater;   // [nqp][n2][n2]
+  const double* qpw;      // [nqp]
+  const double* proj;     // [nproj]
+  const int* gutz_idx;    // [ns]
+  const int* jast_idx;    // [ns*ns] symmetric, -1 on the diagonal
+  const int* loc_spn;     // [ns]
+  const int* trans;       // [ntrans][4] site1 spin1 site2 spin2
+  const double* trans_v;  // [ntrans]
+  const int* intra_site;  const double* intra_v;
+  const int* inter_site;  const double* inter_v;  // [ninter][2]
+  const int* hund_site;   const double* hund_v;
+  const int* exch_site;   const double* exch_v;
+  const double* dproj_g;  // [ns]
+  const double* dproj_j;  // [ns*ns]
+  const int* tsite;       // [nmp][ns]
+  const int* tsgn;        // [nmp][ns]
+  const int* orb_idx;     // [ns*ns]
+  const int* orb_sgn;     // [ns*ns]
+  const double* spgl_cs; const double* spgl_cc; const double* spgl_ss;  // [nleg]
+  // walker state
+  uint32_t* rng;      // [W][625]: 624 words + cursor
+  int* ele_idx;       // [W][n]
+  int* ele_cfg;       // [W][n2]
+  int* ele_num;       // [W][n2]
+  int* proj_cnt;      // [W][nproj]
+  double* inv;        // [W][nqp][n*n]
+  double* pf;         // [W][nqp]
+  double* log_ip;     // [W]
+  int* wstate;        // [W][8]: 0 burn flag, 1 n_accept window, 2 n_out, 3 status, 4..7 spare
+  long long* counter; // [W][4]
+  // saved samples
+  int* s_idx;  int* s_cfg;  int* s_num;  int* s_proj;  // [W][nsample][...]
+  // measurement outputs
+  double* e_loc;      // [W][nsample]
+  int* s_valid;       // [W][nsample]
+  double* sz;         // [W][nsample]
+  double* o_store;    // [W][nsample][size]
+  // per-slot scratch (persistent grids: slot = blockIdx.x)
+  double* work;       // [slots][nqp][n*n]
+  double* qorb;       // [slots][nqp][nslater]
+  int* status;        // [1] sticky
+};
+
+}  // namespace mvmc_hip

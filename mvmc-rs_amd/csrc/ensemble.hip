@@ -1,0 +1,1329 @@
+// Walker-ensemble kernels of the MI355X sampling + SR path.
+//
+// One workgroup advances one walker (an independent Markov chain with its
+// own SFMT19937 stream); grids are persistent (grid = resident capacity,
+// slot = blockIdx.x owns a scratch slot) and loop over walkers.  Inside a
+// walker every floating-point value is produced by the operation sequence of
+// the reference's C-order CPU path, so decisions, configurations and tables
+// reproduce that path bit for bit:
+//   candidates / Metropolis   crates/mvmc-core/src/sampling/{candidate.rs:137-385,metropolis.rs:42-83}
+//   projection counters       crates/mvmc-core/src/sampling/projection.rs:278-416
+//   Pfaffian ratios/updates   crates/mvmc-core/src/sampling/updates.rs:73-158,588-638,1184-1224,
+//                             395-441,779-830,1429-1622
+//   block loop / recompute    crates/mvmc-core/src/sampling/engine.rs:188-475
+//   driver (burn, save)       crates/mvmc-core/src/sampling/driver.rs:175-458
+//   local energy              crates/mvmc-core/src/observables.rs:483-552,1638-1768,558-880
+//   Slater derivative         crates/mvmc-core/src/slater_derivative.rs:75-99,490-548
+//   O store / reductions      crates/mvmc-core/src/observables.rs:202-366, run.rs:1863-1903
+// Serial sums stay serial (one lane); parallelism is across independent
+// elements, QP planes and walkers.
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "ensemble.h"
+#include "pfinv_dev.h"
+
+namespace mvmc_hip {
+
+// ---------------------------------------------------------------------------
+// LDS carving (the host computes the same total through lds_bytes_*)
+// ---------------------------------------------------------------------------
+struct Carver {
+  char* p;
+  template <class T>
+  __device__ T* take(size_t count) {
+    T* r = (T*)p;
+    size_t bytes = (count * sizeof(T) + 15) & ~(size_t)15;
+    p += bytes;
+    return r;
+  }
+};
+
+__host__ __device__ inline size_t rnd16(size_t b) { return (b + 15) & ~(size_t)15; }
+
+__host__ __device__ inline size_t pf_bytes(int n, int np) {
+  return rnd16((size_t)np * n * sizeof(int)) + rnd16((size_t)np * n * sizeof(double)) + 3 * rnd16(np * 8) ;
+}
+
+__host__ __device__ inline size_t union_doubles_sampler(const EnsView& v) {
+  const size_t nq = (size_t)v.nqp * v.n;
+  size_t hop = 3 * nq;
+  size_t ex = v.ex_path != 0 ? 5 * nq + 16 * (size_t)v.nqp : 0;
+  size_t m = hop > ex ? hop : ex;
+  size_t pfb = (pf_bytes(v.n, v.nqp) + 7) / 8;
+  return m > pfb ? m : pfb;
+}
+
+__host__ __device__ inline size_t lds_bytes_sampler(const EnsView& v, int nthreads) {
+  size_t b = 0;
+  b += rnd16(sizeof(SfmtLds));
+  b += rnd16((size_t)v.n * 4) + 2 * rnd16((size_t)v.n2 * 4);
+  b += rnd16(64 * 4);                      // misc ints
+  b += 2 * rnd16((size_t)v.nqp * 8) + rnd16(32 * 8);  // pf, pf_new, dmisc
+  b += rnd16((size_t)(v.nproj > 0 ? v.nproj : 1) * 4);  // delta
+  b += rnd16((size_t)(4 * v.ns + 16) * 8);  // compacted terms
+  b += rnd16((size_t)(nthreads + 64) * 4); // scan
+  b += rnd16((size_t)v.n * 4);             // rs
+  b += rnd16(union_doubles_sampler(v) * 8);
+  return b;
+}
+
+__host__ __device__ inline size_t lds_bytes_measure(const EnsView& v, int nthreads) {
+  size_t b = 0;
+  b += rnd16((size_t)v.n * 4) + 2 * rnd16((size_t)v.n2 * 4);
+  b += rnd16((size_t)(v.nproj > 0 ? v.nproj : 1) * 4);
+  b += rnd16((size_t)v.nqp * 8) + rnd16(32 * 8);
+  b += 2 * rnd16((size_t)v.n * 4);  // tri, sgi per QP plane handled in loop (n each)
+  b += rnd16((size_t)v.n * 4);      // rs
+  b += rnd16(pf_bytes(v.n, v.nqp));
+  b += rnd16((size_t)v.nqp * v.n * 4) * 2;  // tri/sgi for all planes
+  b += rnd16(64 * 4);
+  return b;
+}
+
+// ---------------------------------------------------------------------------
+// small block helpers
+// ---------------------------------------------------------------------------
+__device__ inline void set_status(int* st, int code) { atomicCAS(st, 0, code); }
+
+// Exclusive block scan of one int per thread; returns the total via *total.
+__device__ inline int block_exscan(int v, int* scan, int* total) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = (blockDim.x + 63) >> 6;
+  int x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63 || tid == (int)blockDim.x - 1) scan[wave] = x;
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int k = 0; k < nw; ++k) { int t = scan[k]; scan[k] = acc; acc += t; }
+    scan[nw] = acc;
+  }
+  __syncthreads();
+  const int ex = scan[wave] + x - v;
+  *total = scan[nw];
+  __syncthreads();
+  return ex;
+}
+
+// calc_m_all for one walker: assemble, LTL, Pfaffian, inverse, negate.
+// native = true skips the all-zero guard (C's separate real setup).
+// Returns 0 on success, 1 if any plane failed.
+__device__ int wg_calc_m_all(const EnsView& v, const int* ele_idx_l, int* rs, double* planes, double* work,
+                             double* pf_l, char* pfscratch, bool native) {
+  const int tid = threadIdx.x;
+  __shared__ int bad_site;
+  __shared__ double wave_max[16];
+  if (tid == 0) bad_site = 0;
+  __syncthreads();
+  for (int i = tid; i < v.n; i += blockDim.x) {
+    const int r = ele_idx_l[i];
+    rs[i] = r + (i / v.ne) * v.ns;
+    if (r < 0 || r >= v.ns) bad_site = 1;
+  }
+  __syncthreads();
+  if (bad_site) {
+    if (tid == 0) set_status(v.status, ST_BAD_INPUT);
+    __syncthreads();
+    return 1;
+  }
+  assemble_planes(planes, v.slater, rs, v.n, v.n2, v.nqp, 0);
+  Carver c{pfscratch};
+  PfinvShared sh;
+  sh.ld = v.n;
+  sh.piv = c.take<int>((size_t)v.nqp * v.n);
+  sh.colbuf = c.take<double>((size_t)v.nqp * v.n);
+  sh.kp = c.take<int>(v.nqp);
+  sh.info = c.take<int>(v.nqp);
+  sh.colmax = c.take<double>(v.nqp);
+  __syncthreads();
+  __shared__ int allzero_flag;
+  if (!native) {
+    // max|a|^2 < 1e-28 per plane (pfaffian.rs:884-890)
+    if (tid == 0) allzero_flag = 0;
+    __syncthreads();
+    const int nn = v.n * v.n;
+    for (int p = 0; p < v.nqp; ++p) {
+      double mx = 0.0;
+      for (int e = tid; e < nn; e += blockDim.x) {
+        double a = planes[(size_t)p * nn + e];
+        mx = fmax(mx, a * a);
+      }
+      for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+      if ((tid & 63) == 0) wave_max[tid >> 6] = mx;
+      __syncthreads();
+      if (tid == 0) {
+        double m = 0.0;
+        for (int k = 0; k < (int)(blockDim.x + 63) / 64; ++k) m = fmax(m, wave_max[k]);
+        if (m < 1.0e-28) allzero_flag = 1;
+      }
+      __syncthreads();
+    }
+    if (allzero_flag) return 1;
+  }
+  wg_pfinv(planes, work, v.n, v.nqp, sh, pf_l, -1.0);
+  int bad = 0;
+  for (int p = 0; p < v.nqp; ++p) bad |= (sh.info[p] != 0);
+  __syncthreads();
+  return bad;
+}
+
+__device__ inline double ip_of(const EnsView& v, const double* pf) {
+  double ip = 0.0;
+  for (int q = 0; q < v.nqp; ++q) ip += v.qpw[q] * pf[q];
+  return ip;
+}
+
+// ---------------------------------------------------------------------------
+// seeding: init_gen_rand(seed + w), then `skip` words (init_parameter draws)
+// ---------------------------------------------------------------------------
+__global__ void k_seed(EnsView v, long long base_seed, int skip) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= v.W) return;
+  uint32_t* st = v.rng + (size_t)w * 625;
+  uint32_t seed = (uint32_t)(base_seed + w);
+  st[0] = seed;
+  for (int i = 1; i < 624; ++i) st[i] = 1812433253u * (st[i - 1] ^ (st[i - 1] >> 30)) + (uint32_t)i;
+  const uint32_t parity[4] = {0x00000001u, 0x00000000u, 0x00000000u, 0x13c9e684u};
+  uint32_t inner = 0;
+  for (int i = 0; i < 4; ++i) inner ^= st[i] & parity[i];
+  for (int i = 16; i > 0; i >>= 1) inner ^= inner >> i;
+  if (!(inner & 1)) {
+    bool done = false;
+    for (int i = 0; i < 4 && !done; ++i) {
+      uint32_t work = 1;
+      for (int j = 0; j < 32; ++j) {
+        if (work & parity[i]) { st[i] ^= work; done = true; break; }
+        work <<= 1;
+      }
+    }
+  }
+  int idx = 624;
+  for (int k = 0; k < skip; ++k) {
+    if (idx >= 624) {
+      sfmt_refill_words(st);
+      idx = 0;
+    }
+    ++idx;
+  }
+  st[624] = (uint32_t)idx;
+  int* ws = v.wstate + (size_t)w * 8;
+  ws[0] = 0;
+  ws[1] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// sampler state load / store
+// ---------------------------------------------------------------------------
+struct SampLds {
+  SfmtLds* rng;
+  int *ele_idx, *ele_cfg, *ele_num, *misc, *delta, *scan, *rs;
+  double *pf, *pf_new, *dmisc, *terms, *uni;
+};
+
+__device__ inline SampLds carve_sampler(const EnsView& v, char* base) {
+  Carver c{base};
+  SampLds L;
+  L.rng = c.take<SfmtLds>(1);
+  L.ele_idx = c.take<int>(v.n);
+  L.ele_cfg = c.take<int>(v.n2);
+  L.ele_num = c.take<int>(v.n2);
+  L.misc = c.take<int>(64);
+  L.pf = c.take<double>(v.nqp);
+  L.pf_new = c.take<double>(v.nqp);
+  L.dmisc = c.take<double>(32);
+  L.delta = c.take<int>(v.nproj > 0 ? v.nproj : 1);
+  L.terms = c.take<double>(4 * v.ns + 16);
+  L.scan = c.take<int>(blockDim.x + 64);
+  L.rs = c.take<int>(v.n);
+  L.uni = c.take<double>(union_doubles_sampler(v));
+  return L;
+}
+
+__device__ inline void load_walker(const EnsView& v, int w, SampLds& L, bool with_rng) {
+  const int tid = threadIdx.x;
+  if (with_rng) {
+    const uint32_t* st = v.rng + (size_t)w * 625;
+    for (int i = tid; i < 624; i += blockDim.x) L.rng->w[i] = st[i];
+    if (tid == 0) L.rng->idx = (int)st[624];
+  }
+  for (int i = tid; i < v.n; i += blockDim.x) L.ele_idx[i] = v.ele_idx[(size_t)w * v.n + i];
+  for (int i = tid; i < v.n2; i += blockDim.x) {
+    L.ele_cfg[i] = v.ele_cfg[(size_t)w * v.n2 + i];
+    L.ele_num[i] = v.ele_num[(size_t)w * v.n2 + i];
+  }
+  for (int i = tid; i < v.nproj; i += blockDim.x) L.delta[i] = 0;
+  __syncthreads();
+}
+
+__device__ inline void store_walker(const EnsView& v, int w, SampLds& L) {
+  const int tid = threadIdx.x;
+  uint32_t* st = v.rng + (size_t)w * 625;
+  for (int i = tid; i < 624; i += blockDim.x) st[i] = L.rng->w[i];
+  if (tid == 0) st[624] = (uint32_t)L.rng->idx;
+  for (int i = tid; i < v.n; i += blockDim.x) v.ele_idx[(size_t)w * v.n + i] = L.ele_idx[i];
+  for (int i = tid; i < v.n2; i += blockDim.x) {
+    v.ele_cfg[(size_t)w * v.n2 + i] = L.ele_cfg[i];
+    v.ele_num[(size_t)w * v.n2 + i] = L.ele_num[i];
+  }
+}
+
+// make_initial_sample (initial.rs:77-141), lane 0.
+__device__ void initial_placement(const EnsView& v, SampLds& L) {
+  SfmtLds& R = *L.rng;
+  const int ns = v.ns, ne = v.ne;
+  for (int i = 0; i < v.n; ++i) L.ele_idx[i] = -1;
+  for (int i = 0; i < v.n2; ++i) L.ele_cfg[i] = -1;
+  for (int ri = 0; ri < ns; ++ri) {
+    if (v.loc_spn[ri] != 1) continue;
+    for (;;) {
+      int mi = (int)sfmt_mod(R, (uint32_t)ne);
+      double r = sfmt_real2(R);
+      int si = r < 0.5 ? 0 : 1;
+      if (L.ele_idx[mi + si * ne] == -1) {
+        L.ele_cfg[ri + si * ns] = mi;
+        L.ele_idx[mi + si * ne] = ri;
+        break;
+      }
+    }
+  }
+  for (int si = 0; si < 2; ++si)
+    for (int mi = 0; mi < ne; ++mi) {
+      if (L.ele_idx[mi + si * ne] != -1) continue;
+      for (;;) {
+        int ri = (int)sfmt_mod(R, (uint32_t)ns);
+        if (L.ele_cfg[ri + si * ns] == -1 && v.loc_spn[ri] != 1) {
+          L.ele_cfg[ri + si * ns] = mi;
+          L.ele_idx[mi + si * ne] = ri;
+          break;
+        }
+      }
+    }
+  for (int i = 0; i < v.n2; ++i) L.ele_num[i] = L.ele_cfg[i] < 0 ? 0 : 1;
+}
+
+// Projection counts from scratch (projection.rs:179-277), written to HBM.
+__device__ void make_proj_cnt(const EnsView& v, const int* ele_num, int* cnt) {
+  const int ns = v.ns;
+  for (int i = threadIdx.x; i < v.nproj; i += blockDim.x) cnt[i] = 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (v.ngutz > 0)
+      for (int ri = 0; ri < ns; ++ri) {
+        int idx = v.gutz_idx[ri];
+        if (idx >= 0 && idx < v.nproj) cnt[idx] += ele_num[ri] * ele_num[ri + ns];
+      }
+  }
+  __syncthreads();
+  if (v.nproj > v.ngutz) {
+    for (int ri = threadIdx.x; ri < ns; ri += blockDim.x) {
+      int xi = ele_num[ri] + ele_num[ri + ns] - 1;
+      if (xi == 0) continue;
+      for (int rj = ri + 1; rj < ns; ++rj) {
+        int idx = v.jast_idx[(size_t)ri * ns + rj];
+        if (idx < 0) continue;
+        int xj = ele_num[rj] + ele_num[rj + ns] - 1;
+        atomicAdd(&cnt[v.ngutz + idx], xi * xj);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// begin: burn restore or validated initial placement, initial tables, log|ip|
+// (driver.rs:239-330)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_begin(EnsView v) {
+  extern __shared__ __align__(16) char lds[];
+  SampLds L = carve_sampler(v, lds);
+  const int tid = threadIdx.x;
+  double* work = v.work + (size_t)blockIdx.x * v.nqp * v.n * v.n;
+  char* pfscr = (char*)L.uni;
+  for (int w = blockIdx.x; w < v.W; w += gridDim.x) {
+    int* ws = v.wstate + (size_t)w * 8;
+    double* planes = v.inv + (size_t)w * v.nqp * v.n * v.n;
+    int* cnt = v.proj_cnt + (size_t)w * v.nproj;
+    load_walker(v, w, L, true);
+    int burn = ws[0];
+    auto place = [&]() {
+      for (int attempt = 1; attempt <= 101; ++attempt) {
+        if (tid == 0) initial_placement(v, L);
+        __syncthreads();
+        int bad = wg_calc_m_all(v, L.ele_idx, L.rs, planes, work, L.pf, pfscr, true);
+        if (attempt > 100) { set_status(v.status, ST_INIT_RETRY); return; }
+        if (!bad) return;
+      }
+    };
+    if (!burn) {
+      place();
+      make_proj_cnt(v, L.ele_num, cnt);
+    } else {
+      wg_calc_m_all(v, L.ele_idx, L.rs, planes, work, L.pf, pfscr, true);
+    }
+    __shared__ double lip;
+    if (tid == 0) lip = ddm::cr_log(fabs(ip_of(v, L.pf)));
+    __syncthreads();
+    if (!(lip - lip == 0.0)) {
+      place();
+      make_proj_cnt(v, L.ele_num, cnt);
+      if (tid == 0) lip = ddm::cr_log(fabs(ip_of(v, L.pf)));
+      burn = 0;
+      __syncthreads();
+    }
+    for (int q = tid; q < v.nqp; q += blockDim.x) v.pf[(size_t)w * v.nqp + q] = L.pf[q];
+    if (tid == 0) {
+      v.log_ip[w] = lip;
+      ws[1] = 0;
+      ws[2] = burn ? v.nsample + 1 : v.warmup + v.nsample;
+      for (int k = 0; k < 4; ++k) v.counter[(size_t)w * 4 + k] = 0;
+    }
+    store_walker(v, w, L);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Projection-count deltas of a hop ri -> rj (post-hop occupations) into
+// LDS `delta` (integer adds commute, so lanes may add concurrently).
+// ---------------------------------------------------------------------------
+__device__ inline int jast_of(const EnsView& v, int a, int b) {
+  if (a == b) return -1;
+  return a < b ? v.jast_idx[(size_t)a * v.ns + b] : v.jast_idx[(size_t)b * v.ns + a];
+}
+
+__device__ void hop_deltas(const EnsView& v, SampLds& L, int ri, int rj) {
+  const int ns = v.ns;
+  const int* n0 = L.ele_num;
+  const int* n1 = L.ele_num + ns;
+  if (ri == rj) return;
+  if (threadIdx.x == 0 && v.ngutz > 0) {
+    int a = v.gutz_idx[ri], b = v.gutz_idx[rj];
+    if (a >= 0) atomicAdd(&L.delta[a], -(n0[ri] + n1[ri]));
+    if (b >= 0) atomicAdd(&L.delta[b], n0[rj] * n1[rj]);
+  }
+  if (v.nproj > v.ngutz) {
+    const int off = v.ngutz;
+    if (threadIdx.x == 0) {
+      int idx = jast_of(v, ri, rj);
+      if (idx >= 0) atomicAdd(&L.delta[off + idx], (n0[ri] + n1[ri]) - (n0[rj] + n1[rj]) + 1);
+    }
+    for (int rk = threadIdx.x; rk < ns; rk += blockDim.x) {
+      if (rk == ri || rk == rj) continue;
+      int x = n0[rk] + n1[rk] - 1;
+      if (x == 0) continue;
+      int a = jast_of(v, ri, rk);
+      if (a >= 0) atomicAdd(&L.delta[off + a], -x);
+      int b = jast_of(v, rj, rk);
+      if (b >= 0) atomicAdd(&L.delta[off + b], x);
+    }
+  }
+}
+
+// log_proj_ratio over the changed entries in ascending parameter order
+// (zero terms leave a serial sum unchanged); also applies the deltas to
+// `cnt` when `apply` and clears them.  Result in L.dmisc[0] (all lanes).
+__device__ double proj_ratio_ordered(const EnsView& v, SampLds& L) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int np = v.nproj;
+  const int chunk = (np + nt - 1) / nt;
+  const int p0 = tid * chunk, p1 = min(np, p0 + chunk);
+  int cntl = 0;
+  for (int p = p0; p < p1; ++p) cntl += (L.delta[p] != 0);
+  int total;
+  int off = block_exscan(cntl, L.scan, &total);
+  const int cap = 4 * v.ns + 16;
+  if (total > cap) {
+    if (tid == 0) set_status(v.status, ST_BAD_INPUT);
+    total = cap;
+  }
+  for (int p = p0; p < p1; ++p)
+    if (L.delta[p] != 0) {
+      if (off < cap) L.terms[off] = v.proj[p] * (double)L.delta[p];
+      ++off;
+    }
+  __syncthreads();
+  if (tid == 0) {
+    double z = 0.0;
+    for (int k = 0; k < total; ++k) z += L.terms[k];
+    L.dmisc[0] = z;
+  }
+  __syncthreads();
+  return L.dmisc[0];
+}
+
+__device__ inline void apply_clear_deltas(const EnsView& v, SampLds& L, int* cnt, bool apply) {
+  for (int p = threadIdx.x; p < v.nproj; p += blockDim.x) {
+    int d = L.delta[p];
+    if (d != 0) {
+      if (apply) cnt[p] += d;
+      L.delta[p] = 0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// block of proposals (engine.rs:188-475) and sample saving (driver.rs:419-443)
+// ---------------------------------------------------------------------------
+enum { M_TYPE = 0, M_REJ, M_MI, M_S, M_RI, M_RJ, M_MJ, M_T, M_ACC, M_NACC, M_NOUT, M_SKIP };
+
+__global__ void __launch_bounds__(256) k_block(EnsView v, int out_step, int n_in) {
+  extern __shared__ __align__(16) char lds[];
+  SampLds L = carve_sampler(v, lds);
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int n = v.n, ne = v.ne, ns = v.ns, n2 = v.n2, nqp = v.nqp, nn = n * n;
+  double* work = v.work + (size_t)blockIdx.x * nqp * nn;
+  for (int w = blockIdx.x; w < v.W; w += gridDim.x) {
+    int* ws = v.wstate + (size_t)w * 8;
+    if (tid == 0) {
+      L.misc[M_NOUT] = ws[2];
+      L.misc[M_NACC] = ws[1];
+      L.dmisc[1] = v.log_ip[w];
+    }
+    __syncthreads();
+    const int n_out = L.misc[M_NOUT];
+    if (out_step >= n_out) { __syncthreads(); continue; }
+    load_walker(v, w, L, true);
+    double* planes = v.inv + (size_t)w * nqp * nn;
+    int* cnt = v.proj_cnt + (size_t)w * v.nproj;
+    long long* ctr = v.counter + (size_t)w * 4;
+    for (int q = tid; q < nqp; q += nt) L.pf[q] = v.pf[(size_t)w * nqp + q];
+    __syncthreads();
+    for (int step = 0; step < n_in; ++step) {
+      // ---- candidate (lane 0) ----
+      if (tid == 0) {
+        SfmtLds& R = *L.rng;
+        int type;
+        switch (v.ex_path) {
+          case 0: type = 0; break;
+          case 1: type = sfmt_real2(R) < 0.5 ? 1 : 0; break;
+          case 2: type = 1; break;
+          default: type = 3;
+        }
+        int rej = 0, mi = 0, s = 0, ri = 0, rj = 0, mj = 0, t = 0;
+        if (type == 0) {
+          ctr[0]++;
+          const long long icnt_max = (long long)ns * ns;
+          long long icnt_e = 0;
+          for (;;) {
+            if (icnt_e == icnt_max + 1) {
+              bool any = false;
+              for (int k = 0; k < 2 * ne; ++k) {
+                int r = L.ele_idx[k] < 0 ? 0 : L.ele_idx[k];
+                any = any || !(r >= ns || v.loc_spn[r] == 1);
+              }
+              if (!any) { rej = 1; break; }
+            }
+            ++icnt_e;
+            mi = (int)sfmt_mod(R, (uint32_t)ne);
+            s = sfmt_real2(R) < 0.5 ? 0 : 1;
+            ri = L.ele_idx[mi + s * ne];
+            if (!(ri < 0 || ri >= ns || v.loc_spn[ri] == 1)) break;
+          }
+          if (!rej) {
+            long long icnt = 0;
+            for (;;) {
+              rj = (int)sfmt_mod(R, (uint32_t)ns);
+              if (icnt > icnt_max) { rej = 1; break; }
+              ++icnt;
+              if (L.ele_cfg[rj + s * ns] == -1 && v.loc_spn[rj] != 1) break;
+            }
+          }
+          if (!rej) {
+            // update_ele_config
+            L.ele_idx[mi + s * ne] = rj;
+            L.ele_cfg[ri + s * ns] = -1;
+            L.ele_cfg[rj + s * ns] = mi;
+            L.ele_num[ri + s * ns] = 0;
+            L.ele_num[rj + s * ns] = 1;
+          }
+        } else if (type == 1) {
+          ctr[2]++;
+          bool single = false;
+          for (int r = 0; r < ns; ++r)
+            if (L.ele_num[r] + L.ele_num[r + ns] == 1) { single = true; break; }
+          if (!single) rej = 1;
+          else {
+            for (;;) {
+              mi = (int)sfmt_mod(R, (uint32_t)ne);
+              s = sfmt_real2(R) < 0.5 ? 0 : 1;
+              ri = L.ele_idx[mi + s * ne];
+              if (L.ele_cfg[ri + (1 - s) * ns] == -1) break;
+            }
+            t = 1 - s;
+            for (;;) {
+              mj = (int)sfmt_mod(R, (uint32_t)ne);
+              rj = L.ele_idx[mj + t * ne];
+              if (L.ele_cfg[rj + s * ns] == -1) break;
+            }
+            // first hop (mi: ri -> rj, spin s)
+            L.ele_idx[mi + s * ne] = rj;
+            L.ele_cfg[ri + s * ns] = -1;
+            L.ele_cfg[rj + s * ns] = mi;
+            L.ele_num[ri + s * ns] = 0;
+            L.ele_num[rj + s * ns] = 1;
+          }
+        } else {
+          rej = 2;  // no update: still runs the recompute guard
+        }
+        L.misc[M_TYPE] = type;
+        L.misc[M_REJ] = rej;
+        L.misc[M_MI] = mi; L.misc[M_S] = s; L.misc[M_RI] = ri; L.misc[M_RJ] = rj;
+        L.misc[M_MJ] = mj; L.misc[M_T] = t;
+      }
+      __syncthreads();
+      const int rej = L.misc[M_REJ], type = L.misc[M_TYPE];
+      if (rej == 1) {
+        // `continue` in the reference skips the recompute guard too; the
+        // barrier keeps lane 0 from overwriting the candidate fields before
+        // every wave has read them.
+        __syncthreads();
+        continue;
+      }
+      if (rej == 0) {
+        const int mi = L.misc[M_MI], s = L.misc[M_S], ri = L.misc[M_RI], rj = L.misc[M_RJ];
+        const int mj = L.misc[M_MJ], t = L.misc[M_T];
+        hop_deltas(v, L, ri, rj);
+        __syncthreads();
+        if (type == 1) {
+          if (tid == 0) {  // second hop (mj: rj -> ri, spin t)
+            L.ele_idx[mj + t * ne] = ri;
+            L.ele_cfg[rj + t * ns] = -1;
+            L.ele_cfg[ri + t * ns] = mj;
+            L.ele_num[rj + t * ns] = 0;
+            L.ele_num[ri + t * ns] = 1;
+          }
+          __syncthreads();
+          hop_deltas(v, L, rj, ri);
+          __syncthreads();
+        }
+        const double lpd = proj_ratio_ordered(v, L);
+        // ---- Pfaffian ratios ----
+        double* stage = L.uni;            // [nqp][n]
+        double* slt = L.uni + nqp * n;    // [nqp][n]
+        double* vec1 = L.uni + 2 * nqp * n;
+        if (type == 0) {
+          const int msa = mi + s * ne;
+          const int rsa = L.ele_idx[msa] + s * ns;
+          for (int e = tid; e < nqp * n; e += nt) {
+            const int q = e / n, j = e % n;
+            stage[e] = planes[(size_t)q * nn + (size_t)msa * n + j];
+            const int rsj = L.ele_idx[j] + (j < ne ? 0 : ns);
+            slt[e] = v.slater[((size_t)q * n2 + rsa) * n2 + rsj];
+          }
+          __syncthreads();
+          for (int q = tid; q < nqp; q += nt) {
+            double ratio = 0.0;
+            const double* a = stage + q * n;
+            const double* b = slt + q * n;
+            for (int j = 0; j < n; ++j) ratio += a[j] * b[j];
+            L.pf_new[q] = -ratio * L.pf[q];
+          }
+        } else {
+          // two-electron ratio (updates.rs:395-441, 1539-1622)
+          const int msa = mi + s * ne, msb = mj + t * ne;
+          double* va = L.uni;                 // [nqp][n]
+          double* vb = L.uni + nqp * n;
+          double* ia = L.uni + 2 * nqp * n;   // inv row msa
+          double* ib = L.uni + 3 * nqp * n;   // inv row msb
+          double* tmp = L.uni + 4 * nqp * n;  // row dots of the bilinear form
+          const int rsa = L.ele_idx[msa] + s * ns, rsb = L.ele_idx[msb] + t * ns;
+          for (int e = tid; e < nqp * n; e += nt) {
+            const int q = e / n, i = e % n;
+            const int rsi = L.ele_idx[i] + (i < ne ? 0 : ns);
+            const double* S = v.slater + (size_t)q * n2 * n2;
+            va[e] = S[(size_t)rsa * n2 + rsi];
+            vb[e] = S[(size_t)rsb * n2 + rsi];
+            ia[e] = planes[(size_t)q * nn + (size_t)msa * n + i];
+            ib[e] = planes[(size_t)q * nn + (size_t)msb * n + i];
+          }
+          __syncthreads();
+          for (int e = tid; e < nqp * n; e += nt) {
+            const int q = e / n, i = e % n;
+            const double* row = planes + (size_t)q * nn + (size_t)i * n;
+            const double* a = va + q * n;
+            double acc = 0.0;
+            for (int j = 0; j < n; ++j) acc += row[j] * a[j];
+            tmp[e] = acc;
+          }
+          __syncthreads();
+          for (int q = tid; q < nqp; q += nt) {
+            const double* a = va + q * n;
+            const double* b = vb + q * n;
+            const double* ra = ia + q * n;
+            const double* rb = ib + q * n;
+            double pa = 0.0, pb = 0.0, qa = 0.0, qb = 0.0;
+            for (int i = 0; i < n; ++i) {
+              pa += ra[i] * a[i];
+              pb += rb[i] * a[i];
+              qa += ra[i] * b[i];
+              qb += rb[i] * b[i];
+            }
+            const double iab = ra[msb];
+            double bma = 0.0;
+            for (int i = 0; i < n; ++i) bma += b[i] * tmp[q * n + i];
+            const double ratio = iab * b[msa] + iab * bma + pa * qb - pb * qa;
+            L.pf_new[q] = ratio * L.pf[q];
+          }
+        }
+        __syncthreads();
+        // ---- Metropolis (lane 0) ----
+        if (tid == 0) {
+          const double lip_new = ddm::cr_log(fabs(ip_of(v, L.pf_new)));
+          const double ex = 2.0 * (((lpd + 0.0) + lip_new) - L.dmisc[1]);
+          double wgt = ddm::cr_exp(ex);
+          if (!(wgt - wgt == 0.0)) wgt = -1.0;
+          const double u = sfmt_real2(*L.rng);
+          const int acc = wgt > u;
+          L.misc[M_ACC] = acc;
+          if (acc) {
+            L.dmisc[1] = lip_new;
+            L.misc[M_NACC] += 1;
+            ctr[type == 0 ? 1 : 3]++;
+          } else {
+            if (type == 1) {
+              L.ele_idx[mj + t * ne] = rj;
+              L.ele_cfg[ri + t * ns] = -1;
+              L.ele_cfg[rj + t * ns] = mj;
+              L.ele_num[ri + t * ns] = 0;
+              L.ele_num[rj + t * ns] = 1;
+            }
+            L.ele_idx[mi + s * ne] = ri;
+            L.ele_cfg[ri + s * ns] = mi;
+            L.ele_cfg[rj + s * ns] = -1;
+            L.ele_num[ri + s * ns] = 1;
+            L.ele_num[rj + s * ns] = 0;
+          }
+        }
+        __syncthreads();
+        const int acc = L.misc[M_ACC];
+        apply_clear_deltas(v, L, cnt, acc != 0);
+        if (acc) {
+          if (type == 0) {
+            // update_one_real_scalar: vec1 = -(inv^T slt), serial in i
+            const int msa = mi + s * ne;
+            for (int e = tid; e < nqp * n; e += nt) {
+              const int q = e / n, j = e % n;
+              const double* pl = planes + (size_t)q * nn;
+              const double* b = slt + q * n;
+              double a = 0.0;
+              for (int i = 0; i < n; ++i) a += -pl[(size_t)i * n + j] * b[i];
+              vec1[e] = a;
+            }
+            __syncthreads();
+            // vec2 (in place of the staged old row msa)
+            for (int e = tid; e < nqp * n; e += nt) {
+              const int q = e / n;
+              const double inv_a = -1.0 / vec1[q * n + msa];
+              stage[e] = stage[e] * inv_a;
+            }
+            __syncthreads();
+            for (size_t e = tid; e < (size_t)nqp * nn; e += nt) {
+              const int q = (int)(e / nn), r = (int)(e % nn);
+              const int i = r / n, j = r % n;
+              const double* v1 = vec1 + q * n;
+              const double* v2 = stage + q * n;
+              double x = planes[e];
+              x += v1[i] * v2[j] - v1[j] * v2[i];
+              if (j == msa) x -= v2[i];
+              if (i == msa) x += v2[j];
+              planes[e] = x;
+            }
+            for (int q = tid; q < nqp; q += nt) L.pf[q] = L.pf_new[q];
+            __syncthreads();
+          } else {
+            // update_two_real (updates.rs:1429-1503); pf *= ratio recomputed here
+            const int msa = mi + s * ne, msb = mj + t * ne;
+            const int rsa = L.ele_idx[msa] + s * ns, rsb = L.ele_idx[msb] + t * ns;
+            const int rsa_old = ri + s * ns, rsb_old = rj + t * ns;
+            double* P = L.uni;
+            double* Q = L.uni + nqp * n;
+            double* Sv = L.uni + 2 * nqp * n;
+            double* T = L.uni + 3 * nqp * n;
+            double* cf = L.uni + 5 * nqp * n;  // [nqp][16]
+            for (int e = tid; e < nqp * n; e += nt) {
+              const int q = e / n, i = e % n;
+              const int rsi = L.ele_idx[i] + (i < ne ? 0 : ns);
+              const double* S = v.slater + (size_t)q * n2 * n2;
+              Sv[e] = (i == msb) ? S[(size_t)rsa_old * n2 + rsb_old] : S[(size_t)rsa * n2 + rsi];
+              T[e] = S[(size_t)rsb * n2 + rsi];
+            }
+            __syncthreads();
+            for (int e = tid; e < nqp * n; e += nt) {
+              const int q = e / n, i = e % n;
+              const double* row = planes + (size_t)q * nn + (size_t)i * n;
+              double p = 0.0, qq = 0.0;
+              for (int j = 0; j < n; ++j) {
+                const double x = row[j];
+                p += x * Sv[q * n + j];
+                qq += x * T[q * n + j];
+              }
+              P[e] = p;
+              Q[e] = qq;
+            }
+            __syncthreads();
+            for (int q = tid; q < nqp; q += nt) {
+              const double* Pq = P + q * n;
+              const double* Qq = Q + q * n;
+              const double* Tq = T + q * n;
+              double bma = 0.0;
+              for (int i = 0; i < n; ++i) bma += Tq[i] * Pq[i];
+              const double iab = planes[(size_t)q * nn + (size_t)msa * n + msb];
+              const double ratio = iab * Tq[msa] + iab * bma + Pq[msa] * Qq[msb] - Pq[msb] * Qq[msa];
+              L.pf[q] *= ratio;
+              const double a = -Pq[msa], b = Pq[msb], c = Qq[msa], d = -Qq[msb];
+              const double ee = -bma - Tq[msa];
+              const double f = iab;
+              const double det = a * d - b * c - ee * f;
+              double* k = cf + q * 16;
+              k[0] = a; k[1] = b; k[2] = c; k[3] = d; k[4] = ee; k[5] = f; k[6] = det; k[7] = 1.0 / det;
+            }
+            __syncthreads();
+            for (int e = tid; e < nqp * n; e += nt) {  // S, T rows of the old inverse, scaled
+              const int q = e / n, i = e % n;
+              const double idet = cf[q * 16 + 7];
+              const double sa = planes[(size_t)q * nn + (size_t)msa * n + i];
+              const double sb = planes[(size_t)q * nn + (size_t)msb * n + i];
+              Sv[e] = idet * sa;
+              T[e] = idet * sb;
+            }
+            __syncthreads();
+            for (size_t e = tid; e < (size_t)nqp * nn; e += nt) {
+              const int q = (int)(e / nn), r = (int)(e % nn);
+              const int i = r / n, j = r % n;
+              const double* k = cf + q * 16;
+              const double a = k[0], b = k[1], c = k[2], d = k[3], ee = k[4], f = k[5], det = k[6], idet = k[7];
+              const double pi = P[q * n + i], qi = Q[q * n + i], si = Sv[q * n + i], ti = T[q * n + i];
+              const double pj = P[q * n + j], qj = Q[q * n + j], sj = Sv[q * n + j], tj = T[q * n + j];
+              double x = planes[e];
+              x += a * (qi * tj - qj * ti) + b * (qi * sj - qj * si) + c * (pi * tj - pj * ti) +
+                   d * (pi * sj - pj * si) + ee * det * (si * tj - sj * ti) + f * idet * (pi * qj - qi * pj);
+              if (j == msa) x += -c * ti - d * si - f * idet * qi;
+              if (j == msb) x += -a * ti - b * si + f * idet * pi;
+              if (i == msa) x += c * tj + d * sj + f * idet * qj;
+              if (i == msb) x += a * tj + b * sj - f * idet * pj;
+              if (i == msa && j == msb) x += f * idet;
+              if (i == msb && j == msa) x -= f * idet;
+              planes[e] = x;
+            }
+            __syncthreads();
+          }
+        } else {
+          __syncthreads();
+        }
+      }
+      // ---- periodic recompute (engine.rs:454-473) ----
+      const int nacc_now = L.misc[M_NACC];
+      __syncthreads();
+      if (nacc_now > ns) {
+        int bad = wg_calc_m_all(v, L.ele_idx, L.rs, planes, work, L.pf_new, (char*)L.uni, false);
+        if (tid == 0) {
+          if (!bad) {
+            for (int q = 0; q < nqp; ++q) L.pf[q] = L.pf_new[q];
+            L.dmisc[1] = ddm::cr_log(fabs(ip_of(v, L.pf)));
+          }
+          L.misc[M_NACC] = 0;
+        }
+        __syncthreads();
+      }
+    }
+    // ---- end of block: persist state, save sample ----
+    store_walker(v, w, L);
+    for (int q = tid; q < nqp; q += nt) v.pf[(size_t)w * nqp + q] = L.pf[q];
+    if (tid == 0) {
+      v.log_ip[w] = L.dmisc[1];
+      ws[1] = L.misc[M_NACC];
+      if (out_step == n_out - 1) ws[0] = 1;  // save_burn + counter[9] = 1
+    }
+    if (out_step + v.nsample >= n_out) {
+      const int smp = out_step + v.nsample - n_out;
+      if (smp < v.nsample) {
+        const size_t ws_ = (size_t)w * v.nsample + smp;
+        for (int i = tid; i < n; i += nt) v.s_idx[ws_ * n + i] = L.ele_idx[i];
+        for (int i = tid; i < n2; i += nt) {
+          v.s_cfg[ws_ * n2 + i] = L.ele_cfg[i];
+          v.s_num[ws_ * n2 + i] = L.ele_num[i];
+        }
+        for (int i = tid; i < v.nproj; i += nt) v.s_proj[ws_ * v.nproj + i] = cnt[i];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// measurement of the saved samples (run.rs:2937-3616)
+// ---------------------------------------------------------------------------
+__device__ inline double julia_reciprocal_re(double c) {
+  const double cd = fabs(c);
+  if (1.0547686614863e-154 <= cd && cd <= 9.480751908109176e153) {
+    const double norm = fma(cd, cd, 0.0 * 0.0);
+    return c / norm;
+  }
+  return 1.0 / c;
+}
+
+// direct projection ratio of a hop source -> dest of `spin` (observables.rs:1638-1672)
+__device__ double direct_proj_ratio_moved(const EnsView& v, int source, int dest, int spin, const int* en) {
+  const int n = v.ns;
+  const int src_slot = source + spin * n, dst_slot = dest + spin * n;
+  auto occ = [&](int slot) { return slot == src_slot ? 0 : (slot == dst_slot ? 1 : en[slot]); };
+  const double* row_dest = v.dproj_j + (size_t)dest * n;
+  const double* row_source = v.dproj_j + (size_t)source * n;
+  double z = 0.0;
+  z -= v.dproj_g[source] * (double)(occ(source) + occ(n + source));
+  z += v.dproj_g[dest] * (double)(occ(dest) * occ(n + dest));
+  z += row_source[dest] * (double)((occ(source) + occ(n + source) - 1) - (occ(dest) + occ(n + dest) - 1) + 1);
+  for (int site = 0; site < n; ++site)
+    if (site != source && site != dest) z += (row_dest[site] - row_source[site]) * (double)(en[site] + en[n + site] - 1);
+  return ddm::cr_exp(z);
+}
+
+__global__ void __launch_bounds__(256) k_measure(EnsView v, double* gscratch, int gscratch_stride, int mode_exp) {
+  extern __shared__ __align__(16) char lds[];
+  Carver c{lds};
+  int* ei = c.take<int>(v.n);
+  int* ec = c.take<int>(v.n2);
+  int* en = c.take<int>(v.n2);
+  int* epc = c.take<int>(v.nproj > 0 ? v.nproj : 1);
+  double* pf = c.take<double>(v.nqp);
+  double* dm = c.take<double>(32);
+  int* rs = c.take<int>(v.n);
+  char* pfscr = (char*)c.take<char>(pf_bytes(v.n, v.nqp));
+  int* tri = c.take<int>((size_t)v.nqp * v.n);
+  int* sgi = c.take<int>((size_t)v.nqp * v.n);
+  int* flags = c.take<int>(64);
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int n = v.n, ne = v.ne, ns = v.ns, n2 = v.n2, nqp = v.nqp, nn = n * n;
+  double* work = v.work + (size_t)blockIdx.x * nqp * nn;
+  double* qorb = v.qorb + (size_t)blockIdx.x * nqp * v.nslater;
+  double* g1 = gscratch + (size_t)blockIdx.x * gscratch_stride;  // per term
+  for (int w = blockIdx.x; w < v.W; w += gridDim.x) {
+    double* planes = v.inv + (size_t)w * nqp * nn;
+    for (int s = 0; s < v.nsample; ++s) {
+      const size_t ws = (size_t)w * v.nsample + s;
+      double* o = v.o_store + ws * v.size;
+      for (int i = tid; i < v.size; i += nt) o[i] = 0.0;
+      for (int i = tid; i < n; i += nt) ei[i] = v.s_idx[ws * n + i];
+      for (int i = tid; i < n2; i += nt) { ec[i] = v.s_cfg[ws * n2 + i]; en[i] = v.s_num[ws * n2 + i]; }
+      for (int i = tid; i < v.nproj; i += nt) epc[i] = v.s_proj[ws * v.nproj + i];
+      if (tid == 0) {
+        int all0 = 1, allneg = 1;
+        for (int k = 0; k < n; ++k) { all0 &= (ei[k] == 0); allneg &= (ei[k] < 0); }
+        flags[0] = (all0 || allneg) ? 0 : 1;
+      }
+      __syncthreads();
+      int valid = flags[0];
+      if (valid) {
+        int bad = wg_calc_m_all(v, ei, rs, planes, work, pf, pfscr, false);
+        valid = !bad;
+      }
+      double ip = 0.0;
+      if (valid) {
+        if (tid == 0) dm[0] = ip_of(v, pf);
+        __syncthreads();
+        ip = dm[0];
+        if (fabs(ip) < 1.0e-100) valid = 0;
+      }
+      double e = 0.0;
+      if (valid) {
+        // diagonal part (observables.rs:483-552)
+        if (tid == 0) {
+          double d = 0.0;
+          for (int k = 0; k < v.nintra; ++k) { int r = v.intra_site[k]; d += v.intra_v[k] * (double)(en[r] * en[r + ns]); }
+          for (int k = 0; k < v.ninter; ++k) {
+            int a = v.inter_site[2 * k], b = v.inter_site[2 * k + 1];
+            d += v.inter_v[k] * (double)(en[a] + en[a + ns]) * (double)(en[b] + en[b + ns]);
+          }
+          for (int k = 0; k < v.nhund; ++k) {
+            int a = v.hund_site[2 * k], b = v.hund_site[2 * k + 1];
+            d -= v.hund_v[k] * ((double)(en[a] * en[b]) + (double)(en[a + ns] * en[b + ns]));
+          }
+          dm[1] = d;
+        }
+        // transfer Green ratios, one term per lane (calh1_real_fast)
+        if (v.ntrans > 0 && fabs(ip) > 0.0) {
+          for (int k = tid; k < v.ntrans; k += nt) {
+            const int ri = v.trans[4 * k], s1 = v.trans[4 * k + 1], rj = v.trans[4 * k + 2], s2 = v.trans[4 * k + 3];
+            const int dst = ri + s1 * ns, src = rj + s2 * ns;
+            double g;
+            if (s1 == s2 && ri == rj) g = (double)en[src];
+            else if (s1 != s2 || en[dst] == 1 || en[src] == 0 || ec[src] < 0) g = 0.0;
+            else {
+              const int mj = ec[src];
+              const int msa = mj + s2 * ne;
+              const double pr = direct_proj_ratio_moved(v, rj, ri, s2, en);
+              const int rsa = ri + s2 * ns;
+              double nip = 0.0;
+              for (int q = 0; q < nqp; ++q) {
+                const double* row = v.slater + ((size_t)q * n2 + rsa) * n2;
+                const double* iv = planes + (size_t)q * nn + (size_t)msa * n;
+                double ratio = 0.0;
+                for (int msj = 0; msj < n; ++msj) {
+                  const int site = (msj == msa) ? ri : ei[msj];
+                  ratio += iv[msj] * row[site + (msj < ne ? 0 : ns)];
+                }
+                const double npf = -ratio * pf[q];
+                nip += v.qpw[q] * npf;
+              }
+              g = pr * nip / ip;
+            }
+            g1[k] = g;
+          }
+          __syncthreads();
+          if (tid == 0) {
+            double te = 0.0;
+            for (int k = 0; k < v.ntrans; ++k) te -= v.trans_v[k] * g1[k];
+            dm[1] = dm[1] + te;
+          }
+        }
+        __syncthreads();
+        // exchange terms (green_func2 general case), one term per lane
+        if (v.nexch > 0 && fabs(ip) > 0.0) {
+          for (int k = tid; k < v.nexch; k += nt) {
+            const int a = v.exch_site[2 * k], b = v.exch_site[2 * k + 1];
+            double gsum[2] = {0.0, 0.0};
+            for (int o2 = 0; o2 < 2; ++o2) {
+              const int spin = o2 == 0 ? 0 : 1, spin_o = 1 - spin;
+              const int ri = a, rj = b, rk = b, rl = a;
+              double gv = 0.0;
+              if (ri != rj && en[ri + spin * ns] == 0 && en[rj + spin * ns] == 1 && en[rk + spin_o * ns] == 0 &&
+                  en[rl + spin_o * ns] == 1) {
+                const int mjx = ec[rj + spin * ns], mix = ec[rl + spin_o * ns];
+                if (mjx >= 0 && mix >= 0) {
+                  const int msj = mjx + spin * ne, msi = mix + spin_o * ne;
+                  // projection deltas of the two hops (rl->rk spin_o, then rj->ri spin)
+                  double lpd = 0.0;
+                  {
+                    // count changes via a small local map: evaluate log_proj_ratio over all p
+                    // in ascending order from the exact integer deltas.
+                    // new occupation after both hops
+                    auto num_after = [&](int slot) {
+                      if (slot == rl + spin_o * ns) return 0;
+                      if (slot == rk + spin_o * ns) return 1;
+                      if (slot == rj + spin * ns) return 0;
+                      if (slot == ri + spin * ns) return 1;
+                      return en[slot];
+                    };
+                    auto num_mid = [&](int slot) {
+                      if (slot == rl + spin_o * ns) return 0;
+                      if (slot == rk + spin_o * ns) return 1;
+                      return en[slot];
+                    };
+                    double z = 0.0;
+                    for (int p = 0; p < v.nproj; ++p) {
+                      int d = 0;
+                      // Gutzwiller / Jastrow deltas of hop 1 (rl -> rk) with mid occupations
+                      // and of hop 2 (rj -> ri) with final occupations
+                      for (int hop = 0; hop < 2; ++hop) {
+                        const int hri = hop == 0 ? rl : rj, hrj = hop == 0 ? rk : ri;
+                        auto nm = [&](int slot) { return hop == 0 ? num_mid(slot) : num_after(slot); };
+                        if (hri == hrj) continue;
+                        if (p < v.ngutz) {
+                          if (v.gutz_idx[hri] == p) d -= nm(hri) + nm(hri + ns);
+                          if (v.gutz_idx[hrj] == p) d += nm(hrj) * nm(hrj + ns);
+                        } else {
+                          const int jp = p - v.ngutz;
+                          if (jast_of(v, hri, hrj) == jp) d += (nm(hri) + nm(hri + ns)) - (nm(hrj) + nm(hrj + ns)) + 1;
+                          for (int rkk = 0; rkk < ns; ++rkk) {
+                            if (rkk == hri || rkk == hrj) continue;
+                            const int x = nm(rkk) + nm(rkk + ns) - 1;
+                            if (jast_of(v, hri, rkk) == jp) d -= x;
+                            if (jast_of(v, hrj, rkk) == jp) d += x;
+                          }
+                        }
+                      }
+                      if (d != 0) z += v.proj[p] * (double)d;
+                    }
+                    lpd = z;
+                  }
+                  const double proj_ratio = ddm::cr_exp(lpd);
+                  // two-electron ratio with mi (spin_o) moved to rk and mj (spin) to ri
+                  const int msa = msi, msb = msj;
+                  auto site_after = [&](int m) { return m == msi ? rk : (m == msj ? ri : ei[m]); };
+                  const int rsa = rk + spin_o * ns, rsb = ri + spin * ns;
+                  double nip = 0.0;
+                  for (int q = 0; q < nqp; ++q) {
+                    const double* S = v.slater + (size_t)q * n2 * n2;
+                    const double* inv = planes + (size_t)q * nn;
+                    double pa = 0, pb = 0, qa = 0, qb = 0, bma = 0;
+                    for (int i = 0; i < n; ++i) {
+                      const int rsi = site_after(i) + (i < ne ? 0 : ns);
+                      const double vai = S[(size_t)rsa * n2 + rsi], vbi = S[(size_t)rsb * n2 + rsi];
+                      const double xa = inv[(size_t)msa * n + i], xb = inv[(size_t)msb * n + i];
+                      pa += xa * vai; pb += xb * vai; qa += xa * vbi; qb += xb * vbi;
+                    }
+                    for (int i = 0; i < n; ++i) {
+                      double tmp = 0.0;
+                      for (int j = 0; j < n; ++j) {
+                        const int rsj = site_after(j) + (j < ne ? 0 : ns);
+                        tmp += inv[(size_t)i * n + j] * S[(size_t)rsa * n2 + rsj];
+                      }
+                      const int rsi = site_after(i) + (i < ne ? 0 : ns);
+                      bma += S[(size_t)rsb * n2 + rsi] * tmp;
+                    }
+                    const double vba = S[(size_t)rsb * n2 + (site_after(msa) + (msa < ne ? 0 : ns))];
+                    const double iab = inv[(size_t)msa * n + msb];
+                    const double ratio = iab * vba + iab * bma + pa * qb - pb * qa;
+                    const double npf = ratio * pf[q];
+                    nip += v.qpw[q] * npf;
+                  }
+                  gv = (proj_ratio * nip) / ip;
+                }
+              }
+              gsum[o2] = gv;
+            }
+            g1[k] = gsum[0] + gsum[1];
+          }
+          __syncthreads();
+          if (tid == 0) {
+            double d = dm[1];
+            for (int k = 0; k < v.nexch; ++k) d += v.exch_v[k] * g1[k];
+            dm[1] = d;
+          }
+        }
+        __syncthreads();
+        e = dm[1];
+        if (!(e + 0.0 - (e + 0.0) == 0.0)) valid = 0;
+      }
+      if (tid == 0) {
+        v.s_valid[ws] = valid;
+        v.e_loc[ws] = valid ? e : 0.0;
+        long long szs = 0;
+        for (int r = 0; r < ns; ++r) szs += en[r] - en[r + ns];
+        v.sz[ws] = (double)szs;
+      }
+      if (valid) {
+        // O = [1, projection counts, Slater derivative]
+        if (tid == 0) o[0] = 1.0;
+        for (int p = tid; p < v.nproj; p += nt) o[1 + p] = (double)epc[p];
+        if (v.nslater > 0) {
+          for (size_t k = tid; k < (size_t)nqp * v.nslater; k += nt) qorb[k] = 0.0;
+          for (int e2 = tid; e2 < nqp * n; e2 += nt) {
+            const int q = e2 / n, m = e2 % n;
+            const int tr = q / v.nleg;
+            tri[e2] = v.tsite[(size_t)tr * ns + ei[m]];
+            sgi[e2] = v.tsgn[(size_t)tr * ns + ei[m]];
+          }
+          __syncthreads();
+          auto contrib = [&](int q, int msi, int msj) {
+            const int sp = q % v.nleg;
+            const double p = pf[q];
+            const double cs = p * v.spgl_cs[sp], cc = p * v.spgl_cc[sp], ss = p * v.spgl_ss[sp];
+            const double first = msi < ne ? cs : ss;
+            const double second = msi < ne ? cc : cs;
+            const int ti = tri[q * n + msi], tj = tri[q * n + msj];
+            const int orb = v.orb_idx[(size_t)ti * ns + tj];
+            if (orb < 0 || orb >= v.nslater) return;
+            const int sg = sgi[q * n + msi] * sgi[q * n + msj] * v.orb_sgn[(size_t)ti * ns + tj];
+            const double x = planes[(size_t)q * nn + msj + (size_t)msi * n];
+            const double value = msj < ne ? x * first : -x * second;
+            qorb[(size_t)q * v.nslater + orb] += value * (double)sg;
+          };
+          if (v.orb_injective) {
+            // ordered scatter: (msi, spin block of msj) phases, disjoint targets within a phase
+            for (int msi = 0; msi < n; ++msi) {
+              for (int blk = 0; blk < 2; ++blk) {
+                for (int e2 = tid; e2 < nqp * ne; e2 += nt) {
+                  const int q = e2 / ne, msj = e2 % ne + blk * ne;
+                  contrib(q, msi, msj);
+                }
+                __syncthreads();
+              }
+            }
+          } else {
+            for (int q = tid; q < nqp; q += nt)
+              for (int msi = 0; msi < n; ++msi)
+                for (int msj = 0; msj < n; ++msj) contrib(q, msi, msj);
+            __syncthreads();
+          }
+          const double inv_ip = julia_reciprocal_re(ip);
+          for (int orb = tid; orb < v.nslater; orb += nt) {
+            double acc = 0.0;
+            for (int q = 0; q < nqp; ++q) acc += v.qpw[q] * qorb[(size_t)q * v.nslater + orb];
+            o[1 + v.nproj + orb] = acc * inv_ip;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// reductions over walkers in walker (= rank) order (run.rs:1863-1903,
+// multichain.rs:474-600): per walker the sum over its samples, then a left
+// fold over walkers.
+// ---------------------------------------------------------------------------
+__global__ void k_reduce_energy(EnsView v, double* out5) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double tot[5] = {0, 0, 0, 0, 0};
+  for (int w = 0; w < v.W; ++w) {
+    double a[5] = {0, 0, 0, 0, 0};
+    for (int s = 0; s < v.nsample; ++s) {
+      const size_t ws = (size_t)w * v.nsample + s;
+      if (!v.s_valid[ws]) continue;
+      const double e = v.e_loc[ws], sz = v.sz[ws], wt = 1.0;
+      a[0] += wt;
+      a[1] += wt * e;
+      a[2] += (wt * e) * e;
+      a[3] += wt * sz;
+      a[4] += wt * sz * sz;
+    }
+    for (int k = 0; k < 5; ++k) tot[k] = (w == 0) ? a[k] : tot[k] + a[k];
+  }
+  for (int k = 0; k < 5; ++k) out5[k] = tot[k];
+}
+
+// ho[i], mean[i], diag[i] (CG mode: finalize_oo_store_real diagonal_only)
+__global__ void k_reduce_ops(EnsView v, double* ho, double* mean, double* diag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.size) return;
+  double th = 0, tm = 0, td = 0;
+  for (int w = 0; w < v.W; ++w) {
+    double h = 0.0, m = 0.0, d = 0.0;
+    for (int s = 0; s < v.nsample; ++s) {
+      const size_t ws = (size_t)w * v.nsample + s;
+      const double o = v.o_store[ws * v.size + i];
+      if (v.s_valid[ws]) h += (1.0 * v.e_loc[ws]) * o;
+      m += o;
+      d += o * o;
+    }
+    if (w == 0) { th = h; tm = m; td = d; }
+    else { th += h; tm += m; td += d; }
+  }
+  ho[i] = th;
+  mean[i] = tm;
+  diag[i] = td;
+}
+
+// ---------------------------------------------------------------------------
+// SR-CG operator pieces (sr_cg.rs:952-1004).  Samples are packed
+// [W*nsample][na] in walker order.
+// ---------------------------------------------------------------------------
+__global__ void k_cg_pack(EnsView v, const int* map, int na, double* pack) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t total = (size_t)v.W * v.nsample * na;
+  if (e >= total) return;
+  const size_t ws = e / na;
+  const int si = (int)(e % na);
+  pack[e] = v.o_store[ws * v.size + map[si] + 1];
+}
+
+// y[ws] = sum_si pack[ws][si] x[si] (serial in si)
+__global__ void k_cg_y(const double* pack, int nrows, int na, const double* x, double* y) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const double* row = pack + (size_t)r * na;
+  double acc = 0.0;
+  for (int i = 0; i < na; ++i) acc += row[i] * x[i];
+  y[r] = acc;
+}
+
+// z[si] = fold_w (sum_s pack[w,s][si] y[w,s]); then the mean/shift correction
+__global__ void k_cg_z(const double* pack, int W, int nsample, int na, const double* y, const double* x,
+                       const double* mean, const double* diag, const double* coef, double inv_w, double shift,
+                       double* z) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= na) return;
+  double tot = 0.0;
+  for (int w = 0; w < W; ++w) {
+    double acc = 0.0;
+    for (int s = 0; s < nsample; ++s) {
+      const size_t r = (size_t)w * nsample + s;
+      acc += pack[r * na + i] * y[r];
+    }
+    tot = (w == 0) ? acc : tot + acc;
+  }
+  z[i] = inv_w * tot - coef[0] * mean[i] + shift * diag[i] * x[i];
+}
+
+// serial dot product (sr_cg.rs:492-499) by one lane
+__global__ void k_seq_dot(const double* a, const double* b, int n, double* out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s += a[i] * b[i];
+  *out = s;
+}
+
+// x += alpha d; r = r - alpha q   (or only x)
+__global__ void k_cg_axpy(int n, const double* alpha, const double* d, double* x, const double* q, double* r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a = *alpha;
+  x[i] += a * d[i];
+  if (r) r[i] -= a * q[i];
+}
+
+// r = g - z
+__global__ void k_cg_resid(int n, const double* g, const double* z, double* r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] = g[i] - z[i];
+}
+
+// d = r + beta d
+__global__ void k_cg_dir(int n, const double* beta, const double* r, double* d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = r[i] + *beta * d[i];
+}
+
+// scalar step: alpha = delta/dq, with the C 1.4.0 breakdown guards
+__global__ void k_cg_alpha(const double* delta, const double* dq, double* alpha, int* flag) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double CG_TINY = 1.0e-20;
+  const double q = *dq;
+  if (!(q - q == 0.0) || fabs(q) < CG_TINY) { *flag = 1; return; }
+  const double a = *delta / q;
+  if (!(a - a == 0.0)) { *flag = 1; return; }
+  *alpha = a;
+  *flag = 0;
+}
+
+// beta = dnew/delta; delta *= beta (plain CG), guards
+__global__ void k_cg_beta(double* delta, const double* dnew, double* beta, int* flag) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double CG_TINY = 1.0e-20;
+  const double dn = *dnew, dl = *delta;
+  if (!(dn - dn == 0.0) || fabs(dl) < CG_TINY) { *flag = 1; return; }
+  const double b = dn / dl;
+  if (!(b - b == 0.0)) { *flag = 1; return; }
+  *beta = b;
+  *delta = dl * b;
+  *flag = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Batched Pfaffian + inverse (BatchedEngine): planes [count][n][n]
+// column-major; one workgroup per plane batch of np planes.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_pfinv_batched(double* planes, double* work, int n, int count, int np,
+                                                       double* pf, int* status) {
+  extern __shared__ __align__(16) char lds[];
+  const int first = blockIdx.x * np;
+  const int cnt = min(np, count - first);
+  if (cnt <= 0) return;
+  Carver c{lds};
+  PfinvShared sh;
+  sh.ld = n;
+  sh.piv = c.take<int>((size_t)np * n);
+  sh.colbuf = c.take<double>((size_t)np * n);
+  sh.kp = c.take<int>(np);
+  sh.info = c.take<int>(np);
+  sh.colmax = c.take<double>(np);
+  double* pl = planes + (size_t)first * n * n;
+  double* wk = work + (size_t)blockIdx.x * np * n * n;
+  wg_pfinv(pl, wk, n, cnt, sh, pf + first, 1.0);
+  for (int p = threadIdx.x; p < cnt; p += blockDim.x) {
+    const int info = sh.info[p];
+    status[first + p] = info;
+    if (info != 0) pf[first + p] = 0.0;
+  }
+  __syncthreads();
+  for (int p = 0; p < cnt; ++p)
+    if (sh.info[p] != 0)
+      for (int e = threadIdx.x; e < n * n; e += blockDim.x) pl[(size_t)p * n * n + e] = 0.0;
+}
+
+}  // namespace mvmc_hip

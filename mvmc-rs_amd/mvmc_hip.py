@@ -1,0 +1,196 @@
+"""Python host mirror of libmvmc_hip.so (ctypes over include/mvmc_hip.h).
+
+Names follow the reference's backend seams:
+  pfaffian_inverse_batched  mvmc_gpu::pfaffian_inverse_batched / BatchedEngine::run_f64
+                            (crates/mvmc-gpu/src/lib.rs:104-226)
+  Ensemble.sample           SamplingEngine over W walkers (crates/mvmc-core/src/sampling/engine.rs:134-168)
+  Ensemble.measure          MeasurementStages::measure_sample batched over saved samples
+                            (crates/mvmc-core/src/stage_backend.rs:418-438)
+  Ensemble.sr_cg            SrStages::cg_step + parameter update (crates/mvmc-core/src/sr_backend.rs:128-292)
+  Ensemble.opt_step         one SR step of vmc_para_opt (crates/mvmc-core/src/run.rs:1294-1475)
+There is no CPU fallback: a missing library or GPU raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libmvmc_hip.so")
+
+_lib = None
+
+
+class BackendUnavailable(RuntimeError):
+    """BatchedError::BackendUnavailable (crates/mvmc-gpu/src/lib.rs:83-101)."""
+
+
+class InvalidShape(ValueError):
+    """BatchedError::InvalidShape."""
+
+
+class BackendError(RuntimeError):
+    """BatchedError::Backend / StageError::Failed."""
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BackendUnavailable(f"libmvmc_hip.so not built: {LIB_PATH}")
+        L = ctypes.CDLL(LIB_PATH)
+        L.mvmc_last_error.restype = ctypes.c_char_p
+        L.mvmc_pfinv_f64.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_void_p]
+        L.mvmc_ens_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_longlong,
+                                      ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        for name in ["mvmc_ens_destroy", "mvmc_ens_sample", "mvmc_ens_measure", "mvmc_ens_status"]:
+            getattr(L, name).argtypes = [ctypes.c_void_p]
+        L.mvmc_ens_sr_cg.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)]
+        L.mvmc_ens_opt_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.mvmc_sr_cg_solve.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 4 + \
+            [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+             ctypes.POINTER(ctypes.c_longlong), ctypes.c_void_p]
+        _lib = L
+    return _lib
+
+
+EXPORTED_SYMBOLS = [
+    "mvmc_last_error", "mvmc_abi_version", "mvmc_pfinv_f64", "mvmc_pfinv_f64_inplace", "mvmc_ens_create",
+    "mvmc_ens_destroy", "mvmc_ens_dims", "mvmc_ens_get_params", "mvmc_ens_set_params", "mvmc_ens_sample",
+    "mvmc_ens_measure", "mvmc_ens_sr_cg", "mvmc_ens_opt_step", "mvmc_ens_get_energy", "mvmc_ens_get_operands",
+    "mvmc_ens_get_o_store", "mvmc_ens_get_walker", "mvmc_ens_get_tables", "mvmc_ens_get_config", "mvmc_ens_get_cg",
+    "mvmc_ens_timings", "mvmc_ens_status", "mvmc_sr_cg_solve",
+]
+
+
+def _check(rc):
+    if rc == 0:
+        return
+    msg = lib().mvmc_last_error().decode(errors="replace")
+    if rc == -1:
+        raise InvalidShape(msg)
+    if rc == -2:
+        raise BackendUnavailable(msg)
+    raise BackendError(f"rc={rc}: {msg}")
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def pfaffian_inverse_batched(planes_dev, pf_dev, inv_dev, status_dev, n, count, stream=None):
+    """Device-pointer form (torch CUDA tensors or raw ints): planes [count][n][n]
+    column-major planes, outputs Pf, TRUE inverse, status."""
+    def addr(x):
+        return x if isinstance(x, int) else x.data_ptr()
+    _check(lib().mvmc_pfinv_f64(ctypes.c_void_p(addr(planes_dev)), n, count, ctypes.c_void_p(addr(pf_dev)),
+                                ctypes.c_void_p(addr(inv_dev)), ctypes.c_void_p(addr(status_dev)),
+                                ctypes.c_void_p(stream or 0)))
+
+
+class Ensemble:
+    """W walkers of one Expert-mode model resident on the GPU."""
+
+    def __init__(self, namelist, initial_def=None, walkers=1, seed=None):
+        h = ctypes.c_void_p()
+        rc = lib().mvmc_ens_create(namelist.encode(), initial_def.encode() if initial_def else None, int(walkers),
+                                   0 if seed is None else int(seed), 0 if seed is None else 1, ctypes.byref(h))
+        _check(rc)
+        self.h = h
+        d = np.zeros(16, dtype=np.int32)
+        lib().mvmc_ens_dims(self.h, _p(d))
+        (self.nsite, self.ne, self.n, self.nqp, self.nproj, self.nslater, self.size, self.nsample, self.walkers,
+         self.ex_path, self.interval, self.warmup, self.grid, self.lds_sampler, self.lds_measure,
+         self.grid_measure) = [int(x) for x in d]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mvmc_ens_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def sample(self):
+        _check(lib().mvmc_ens_sample(self.h))
+
+    def measure(self):
+        _check(lib().mvmc_ens_measure(self.h))
+
+    def sr_cg(self):
+        info = ctypes.c_longlong(0)
+        _check(lib().mvmc_ens_sr_cg(self.h, ctypes.byref(info)))
+        return info.value
+
+    def opt_step(self):
+        row = np.zeros(6)
+        _check(lib().mvmc_ens_opt_step(self.h, _p(row)))
+        return row
+
+    def params(self):
+        p = np.zeros(max(self.nproj, 1))
+        s = np.zeros(max(self.nslater, 1))
+        lib().mvmc_ens_get_params(self.h, _p(p), _p(s))
+        return p[:self.nproj].copy(), s[:self.nslater].copy()
+
+    def set_params(self, proj, slater):
+        p = np.ascontiguousarray(proj, dtype=np.float64)
+        s = np.ascontiguousarray(slater, dtype=np.float64)
+        lib().mvmc_ens_set_params(self.h, _p(p), _p(s))
+
+    def energy(self):
+        e = np.zeros(5)
+        lib().mvmc_ens_get_energy(self.h, _p(e))
+        return e
+
+    def operands(self):
+        oo = np.zeros(2 * self.size)
+        ho = np.zeros(self.size)
+        lib().mvmc_ens_get_operands(self.h, _p(oo), _p(ho))
+        return oo, ho
+
+    def o_store(self, walker=0):
+        o = np.zeros(self.nsample * self.size)
+        _check(lib().mvmc_ens_get_o_store(self.h, int(walker), _p(o)))
+        return o.reshape(self.nsample, self.size)
+
+    def walker(self, w):
+        idx = np.zeros(self.nsample * self.n, dtype=np.int32)
+        cnt = np.zeros(4, dtype=np.int64)
+        rng = np.zeros(625, dtype=np.uint32)
+        _check(lib().mvmc_ens_get_walker(self.h, int(w), _p(idx), _p(cnt), _p(rng)))
+        return {"ele_idx": idx.reshape(self.nsample, self.n), "counter": cnt, "rng": rng[:624].copy(),
+                "rng_idx": int(rng[624])}
+
+    def tables(self, w):
+        inv = np.zeros(self.nqp * self.n * self.n)
+        pf = np.zeros(self.nqp)
+        _check(lib().mvmc_ens_get_tables(self.h, int(w), _p(inv), _p(pf)))
+        return inv.reshape(self.nqp, self.n, self.n), pf
+
+    def cg(self):
+        x = np.zeros(self.size)
+        na = ctypes.c_int(0)
+        st = lib().mvmc_ens_get_cg(self.h, _p(x), ctypes.byref(na))
+        return st, x[:na.value].copy()
+
+    def timings(self):
+        t = np.zeros(8)
+        lib().mvmc_ens_timings(self.h, _p(t))
+        return {"sample_s": t[0], "measure_s": t[1], "sr_s": t[2], "proposals": t[3], "accepted": t[4],
+                "block_ms": t[5], "block_launches": t[6]}
+
+    def status(self):
+        return int(lib().mvmc_ens_status(self.h))
+
+
+def sr_cg_solve(pack_dev, mean_dev, diag_dev, g_dev, x_dev, na, walkers, nsample, inv_w, shift, tol, max_it,
+                variant=0, stream=None):
+    """SrStages::cg_step on device operands (torch CUDA tensors)."""
+    st = ctypes.c_longlong(0)
+    _check(lib().mvmc_sr_cg_solve(na, walkers, nsample, ctypes.c_void_p(pack_dev.data_ptr()),
+                                  ctypes.c_void_p(mean_dev.data_ptr()), ctypes.c_void_p(diag_dev.data_ptr()),
+                                  ctypes.c_void_p(g_dev.data_ptr()), inv_w, shift, tol, max_it, variant,
+                                  ctypes.c_void_p(x_dev.data_ptr()), ctypes.byref(st), ctypes.c_void_p(stream or 0)))
+    return st.value

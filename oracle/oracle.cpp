@@ -23,6 +23,7 @@
 #include <thread>
 #include <vector>
 
+#include "../mvmc-rs_amd/csrc/ddmath.h"
 #include "../mvmc-rs_amd/host/model.h"
 #include "pfapack.h"
 #include "sfmt.h"
@@ -31,6 +32,16 @@ using mvmc::Model;
 using std::vector;
 
 namespace oracle {
+
+// exp/log used by the Metropolis weight, the overlap logarithm and the
+// projection ratios.  Default: the host libm, as the reference (glibc).
+// Mode 1: the correctly rounded double-double routines that the device uses,
+// so that GPU-parity runs compare identical arithmetic; the two agree except
+// where glibc misrounds a near-midpoint case (measured in
+// tests/test_oracle_math.py).
+static int g_cr_math = 0;
+static inline double o_exp(double x) { return g_cr_math ? ddm::cr_exp(x) : std::exp(x); }
+static inline double o_log(double x) { return g_cr_math ? ddm::cr_log(x) : std::log(x); }
 
 // ---------------------------------------------------------------------------
 // Walker state (one independent chain = one ungrouped C rank,
@@ -390,11 +401,11 @@ static void make_initial_sample_validated(const Run& R, Walker& W) {
   }
 }
 
-static double log_ip_of(const Model& m, const double* pf) { return std::log(std::fabs(calc_ip(m, pf))); }
+static double log_ip_of(const Model& m, const double* pf) { return o_log(std::fabs(calc_ip(m, pf))); }
 
 static bool metropolis(Walker& W, double lpd, double lip_new, double lip_old) {
   const double ex = 2.0 * (((lpd + 0.0) + lip_new) - lip_old);
-  double w = std::exp(ex);
+  double w = o_exp(ex);
   if (!std::isfinite(w)) w = -1.0;
   const double u = W.rng.real2();
   return w > u;
@@ -593,7 +604,7 @@ static double direct_proj_ratio_moved(const Run& R, int source, int dest, int sp
   z += row_source[dest] * (double)(charge_moved(source) - charge_moved(dest) + 1);
   for (int site = 0; site < n; ++site)
     if (site != source && site != dest) z += (row_dest[site] - row_source[site]) * (double)charge(site);
-  return std::exp(z);
+  return o_exp(z);
 }
 
 // updates.rs:168-262 calculate_new_pf_m2_ip_real_flat
@@ -669,7 +680,7 @@ static double green_exchange(const Run& R, int ri, int rj, int spin, int spin_ot
   num[ri + spin * ns] = 1;
   update_proj_cnt(m, rj, ri, cnt.data(), num.data());
   const double lpd = log_proj_ratio(m, cnt.data(), proj_cnt);
-  const double proj_ratio = std::exp(lpd);
+  const double proj_ratio = o_exp(lpd);
   tmp.inv.assign(inv, inv + (size_t)m.nqp() * m.nsize() * m.nsize());
   tmp.pf.assign(pf, pf + m.nqp());
   vector<double> pf_new(m.nqp());
@@ -1097,6 +1108,10 @@ static int opt_step(Run& R, bool do_sr) {
 using namespace oracle;
 
 extern "C" {
+
+void oracle_set_math(int cr) { g_cr_math = cr; }
+double oracle_cr_exp(double x) { return ddm::cr_exp(x); }
+double oracle_cr_log(double x) { return ddm::cr_log(x); }
 
 void oracle_sfmt_words(uint32_t seed, int count, uint32_t* out) {
   Sfmt r;

@@ -1,0 +1,204 @@
+"""ORACLE (test infrastructure only): ctypes binding of oracle/_build/liboracle.so,
+the CPU FP64 restatement of the reference path.  Imported by tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker; the
+product (mvmc-rs_amd) never imports it."""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        L.oracle_run_create.restype = ctypes.c_void_p
+        L.oracle_run_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_longlong,
+                                        ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+        L.oracle_run_destroy.argtypes = [ctypes.c_void_p]
+        L.oracle_run_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.oracle_run_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+        L.oracle_run_sample.argtypes = [ctypes.c_void_p]
+        for name in ["oracle_dims", "oracle_get_params", "oracle_set_params", "oracle_get_energy",
+                     "oracle_get_energies", "oracle_get_operands", "oracle_get_o_last", "oracle_get_o_store",
+                     "oracle_get_walker", "oracle_get_walker_tables", "oracle_get_burn", "oracle_get_slater_table",
+                     "oracle_get_qp_weights", "oracle_get_times", "oracle_get_cg"]:
+            getattr(L, name).argtypes = None
+        L.oracle_cr_exp.restype = ctypes.c_double
+        L.oracle_cr_exp.argtypes = [ctypes.c_double]
+        L.oracle_cr_log.restype = ctypes.c_double
+        L.oracle_cr_log.argtypes = [ctypes.c_double]
+        L.oracle_two_hop_bilinear.restype = ctypes.c_double
+        L.oracle_cg.restype = ctypes.c_longlong
+        L.oracle_cg.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 4 + \
+            [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 3
+        L.oracle_cg_apply.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 5 + \
+            [ctypes.c_double, ctypes.c_double]
+        L.oracle_measure_config.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def set_math(cr):
+    """0: host libm exp/log (as the reference); 1: correctly rounded (device arithmetic)."""
+    lib().oracle_set_math(ctypes.c_int(1 if cr else 0))
+
+
+def sfmt_words(seed, count):
+    out = np.zeros(count, dtype=np.uint32)
+    lib().oracle_sfmt_words(ctypes.c_uint32(seed), ctypes.c_int(count), ptr(out))
+    return out
+
+
+def pfinv(planes):
+    """planes: (count, n, n) array of column-major planes stored as [p][c][r]."""
+    planes = np.ascontiguousarray(planes, dtype=np.float64)
+    count, n, _ = planes.shape
+    pf = np.zeros(count)
+    inv = np.zeros_like(planes)
+    status = np.zeros(count, dtype=np.int32)
+    lib().oracle_pfinv(ptr(planes), ctypes.c_int(n), ctypes.c_int(count), ptr(pf), ptr(inv), ptr(status))
+    return pf, inv, status
+
+
+def pfaffian(plane_colmajor, n):
+    a = np.ascontiguousarray(plane_colmajor, dtype=np.float64)
+    pf = ctypes.c_double(0.0)
+    info = lib().oracle_pfaffian(ptr(a), ctypes.c_int(n), ctypes.byref(pf))
+    return pf.value, info
+
+
+class Run:
+    """W-walker optimisation run of the oracle (crates/mvmc-core/src/multichain.rs:647)."""
+
+    def __init__(self, namelist, initial_def=None, walkers=1, seed=None, threads=1):
+        err = ctypes.create_string_buffer(1024)
+        h = lib().oracle_run_create(namelist.encode(), initial_def.encode() if initial_def else None,
+                                    walkers, 0 if seed is None else int(seed), 0 if seed is None else 1, err, 1024)
+        if not h:
+            raise RuntimeError("oracle_run_create: " + err.value.decode())
+        self.h = ctypes.c_void_p(h)
+        if threads > 1:
+            lib().oracle_run_set_threads(self.h, ctypes.c_int(threads))
+        d = np.zeros(10, dtype=np.int32)
+        lib().oracle_dims(self.h, ptr(d))
+        (self.nsite, self.ne, self.n, self.nqp, self.nproj, self.nslater, self.size, self.nsample,
+         self.walkers, _) = [int(x) for x in d]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_run_destroy(self.h)
+            self.h = None
+
+    def step(self, do_sr=True):
+        err = ctypes.create_string_buffer(1024)
+        rc = lib().oracle_run_step(self.h, ctypes.c_int(1 if do_sr else 0), err, 1024)
+        if rc == -1000:
+            raise RuntimeError("oracle step: " + err.value.decode())
+        return rc
+
+    def sample(self):
+        lib().oracle_run_sample(self.h)
+
+    def params(self):
+        p = np.zeros(max(self.nproj, 1))
+        s = np.zeros(max(self.nslater, 1))
+        lib().oracle_get_params(self.h, ptr(p), ptr(s))
+        return p[:self.nproj].copy(), s[:self.nslater].copy()
+
+    def set_params(self, proj, slater):
+        p = np.ascontiguousarray(proj, dtype=np.float64)
+        s = np.ascontiguousarray(slater, dtype=np.float64)
+        lib().oracle_set_params(self.h, ptr(p), ptr(s))
+
+    def energy(self):
+        e = np.zeros(5)
+        lib().oracle_get_energy(self.h, ptr(e))
+        return e
+
+    def energies(self, max_steps=100000):
+        out = np.zeros(6 * 4096)
+        k = lib().oracle_get_energies(self.h, ptr(out))
+        return out[:6 * k].reshape(k, 6)
+
+    def operands(self):
+        d = np.zeros(10, dtype=np.int32)
+        lib().oracle_dims(self.h, ptr(d))
+        oo = np.zeros(max(int(d[9]), 1))
+        ho = np.zeros(self.size)
+        lib().oracle_get_operands(self.h, ptr(oo), ptr(ho))
+        return oo[:int(d[9])], ho
+
+    def o_last(self, walker=0):
+        o = np.zeros(self.size)
+        lib().oracle_get_o_last(self.h, ctypes.c_int(walker), ptr(o))
+        return o
+
+    def o_store(self, walker=0):
+        o = np.zeros(self.size * self.nsample)
+        lib().oracle_get_o_store(self.h, ctypes.c_int(walker), ptr(o))
+        return o.reshape(self.nsample, self.size)
+
+    def walker(self, w):
+        idx = np.zeros(self.nsample * self.n, dtype=np.int32)
+        cnt = np.zeros(10, dtype=np.int32)
+        words = np.zeros(624, dtype=np.uint32)
+        ridx = ctypes.c_int(0)
+        acc = ctypes.c_longlong(0)
+        lib().oracle_get_walker(self.h, ctypes.c_int(w), ptr(idx), ptr(cnt), ptr(words), ctypes.byref(ridx),
+                                ctypes.byref(acc))
+        return {"ele_idx": idx.reshape(self.nsample, self.n), "counter": cnt, "rng": words, "rng_idx": ridx.value,
+                "accepted": acc.value}
+
+    def tables(self, w):
+        inv = np.zeros(self.nqp * self.n * self.n)
+        pf = np.zeros(self.nqp)
+        lib().oracle_get_walker_tables(self.h, ctypes.c_int(w), ptr(inv), ptr(pf))
+        return inv.reshape(self.nqp, self.n, self.n), pf
+
+    def times(self):
+        t = np.zeros(3)
+        lib().oracle_get_times(self.h, ptr(t))
+        return t
+
+    def cg(self):
+        x = np.zeros(self.size)
+        st = lib().oracle_get_cg(self.h, ptr(x))
+        return st, x
+
+
+def cg_solve(mean, diag, O, g, inv_w, shift, tol, max_it, diag_scale=False):
+    mean = np.ascontiguousarray(mean, dtype=np.float64)
+    diag = np.ascontiguousarray(diag, dtype=np.float64)
+    O = np.ascontiguousarray(O, dtype=np.float64)
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    n = mean.size
+    samples = O.size // n
+    x = np.zeros(n)
+    r = np.zeros(n)
+    d = np.zeros(n)
+    st = lib().oracle_cg(n, samples, ptr(mean), ptr(diag), ptr(O), ptr(g), inv_w, shift, tol, max_it,
+                         1 if diag_scale else 0, ptr(x), ptr(r), ptr(d))
+    return st, x, r, d
+
+
+def cg_apply(mean, diag, O, x, inv_w, shift):
+    mean = np.ascontiguousarray(mean, dtype=np.float64)
+    diag = np.ascontiguousarray(diag, dtype=np.float64)
+    O = np.ascontiguousarray(O, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = mean.size
+    z = np.zeros(n)
+    lib().oracle_cg_apply(n, O.size // n, ptr(mean), ptr(diag), ptr(O), ptr(x), ptr(z), inv_w, shift)
+    return z

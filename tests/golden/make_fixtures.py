@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Write the committed golden fixtures of tests/golden from the reference
+checkout's own fixture data (run once, in a container where the reference is
+mounted; the GPU box never runs this).
+
+Everything written here is data held by the reference's tests:
+  * SFMT words of the C library      crates/sfmt19937/tests/golden_vs_c.rs:40-160
+  * Hubbard chain step-1 operands    tests/fixtures/c_order_sr_operands/hubbard-cg-store0.txt
+    and its Expert inputs             tests/fixtures/c_order_sr_operands/inputs/hubbard
+    (modpara overrides as in tests/fixtures/c_order_sr_operands/PROVENANCE.md)
+  * Heisenberg multi-rank state      tests/fixtures/mpi_matrix_179/real-cg-r{2,4}w1-s7
+    and its Expert inputs             tests/fixtures/native_c_physcal_181/heisenberg_chain_real/inputs
+    (overrides from tests/fixtures/mpi_matrix_179/cells.txt)
+  * SR-CG iterates                   tests/fixtures/sr_cg/c_refresh/real{,_diag_scale}.txt
+  * PfaPack input matrices           tests/fixtures/pfapack/real_n*.orig.txt
+  * two-hop bilinear bits            tests/fixtures/pfaffian_cg/two_hop_bilinear{,_c}.txt
+"""
+import json
+import os
+import re
+import shutil
+import struct
+import sys
+
+import numpy as np
+
+REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+OUT = os.path.dirname(os.path.abspath(__file__))
+FIX = os.path.join(REF, "tests", "fixtures")
+
+
+def hexd(s):
+    return struct.unpack(">d", bytes.fromhex(s))[0]
+
+
+def sfmt():
+    src = open(os.path.join(REF, "crates/sfmt19937/tests/golden_vs_c.rs")).read()
+    first8 = {}
+    block = src[src.index("GOLDEN_FIRST8_U32"):src.index("GOLDEN_BOUNDARY_SEED")]
+    for seed, body in re.findall(r"\(\s*([0-9_]+|u32::MAX),\s*\[([^\]]*)\]", block):
+        key = 0xFFFFFFFF if seed == "u32::MAX" else int(seed.replace("_", ""))
+        first8[str(key)] = [int(x.replace("_", ""), 16) for x in re.findall(r"0x[0-9a-f_]+", body)]
+    bblock = src[src.index("const GOLDEN_BOUNDARY:"):src.index("GOLDEN_FIRST8_U64_SEED")]
+    boundary = [[int(i), int(v.replace("_", ""), 16)] for i, v in re.findall(r"\((\d+),\s*(0x[0-9a-f_]+)\)", bblock)]
+    u64block = src[src.index("const GOLDEN_FIRST8_U64:"):]
+    u64block = u64block[:u64block.index("];")]
+    u64 = [int(v.replace("_", ""), 16) for v in re.findall(r"0x[0-9a-f_]+", u64block)]
+    json.dump({"first8_u32": first8, "boundary_seed": 11272, "boundary": boundary,
+               "first8_u64_seed": 11272, "first8_u64": u64},
+              open(os.path.join(OUT, "sfmt_golden.json"), "w"), indent=1)
+
+
+def copy_inputs(src_dir, dst_dir, overrides):
+    os.makedirs(dst_dir, exist_ok=True)
+    for name in os.listdir(src_dir):
+        shutil.copy(os.path.join(src_dir, name), os.path.join(dst_dir, name))
+        os.chmod(os.path.join(dst_dir, name), 0o644)
+    path = os.path.join(dst_dir, "modpara.def")
+    lines = open(path).read().split("\n")
+    seen = set()
+    out = []
+    for line in lines:
+        t = line.split()
+        if t and t[0] in overrides:
+            out.append(f"{t[0]} {overrides[t[0]]}")
+            seen.add(t[0])
+        else:
+            out.append(line)
+    while out and out[-1] == "":
+        out.pop()
+    for k, v in overrides.items():
+        if k not in seen:
+            out.append(f"{k} {v}")
+    open(path, "w").write("\n".join(out) + "\n")
+
+
+def hubbard():
+    copy_inputs(os.path.join(FIX, "c_order_sr_operands/inputs/hubbard"), os.path.join(OUT, "inputs/hubbard_chain"),
+                {"RndSeed": 1, "NSROptItrStep": 1, "NSROptItrSmp": 1, "NSRCG": 1, "NStore": 0,
+                 "NSRCGAbortOnFail": 0})
+    ref = {}
+    for line in open(os.path.join(FIX, "c_order_sr_operands/hubbard-cg-store0.txt")):
+        if line.startswith("#") or not line.strip():
+            continue
+        t = line.split()
+        ref[t[0]] = t[1:]
+    json.dump(ref, open(os.path.join(OUT, "hubbard_cg_store0.json"), "w"), indent=0)
+
+
+def heisenberg():
+    over = {"NVMCCalMode": 0, "NSROptItrStep": 1, "NSROptItrSmp": 1, "NVMCSample": 7, "NSplitSize": 1,
+            "NStore": 0, "NSRCG": 1, "NSRCGAbortOnFail": 0}
+    copy_inputs(os.path.join(FIX, "native_c_physcal_181/heisenberg_chain_real/inputs"),
+                os.path.join(OUT, "inputs/heisenberg_chain"), over)
+    for cell in ["real-cg-r2w1-s7", "real-cg-r4w1-s7"]:
+        base = os.path.join(FIX, "mpi_matrix_179", cell)
+        ranks = []
+        r = 0
+        while os.path.exists(os.path.join(base, f"rank{r}.txt")):
+            rec = {"eleidx": {}, "counter": None, "rng": None, "rng_idx": None}
+            for line in open(os.path.join(base, f"rank{r}.txt")):
+                t = line.split()
+                if not t:
+                    continue
+                if t[0] == "COUNTER":
+                    rec["counter"] = [int(x) for x in t[3:]]
+                elif t[0] == "ELEIDX":
+                    rec["eleidx"][t[3].split("=")[1]] = [int(x) for x in t[4:]]
+                elif t[0] == "RNG":
+                    rec["rng_idx"] = int(t[3].split("=")[1])
+                    rec["rng"] = [int(x) for x in t[4:]]
+            ranks.append(rec)
+            r += 1
+        ops = {}
+        for line in open(os.path.join(base, "operands.txt")):
+            t = line.split()
+            if t:
+                ops[t[0]] = [float(x) for x in t[1:]]
+        json.dump({"ranks": ranks, "operands": ops}, open(os.path.join(OUT, f"mpi179_{cell.replace('-', '_')}.json"), "w"))
+
+
+def sr_cg():
+    for name in ["real.txt", "real_diag_scale.txt"]:
+        shutil.copy(os.path.join(FIX, "sr_cg/c_refresh", name), os.path.join(OUT, "sr_cg_" + name))
+        os.chmod(os.path.join(OUT, "sr_cg_" + name), 0o644)
+
+
+def pfapack():
+    arrays = {}
+    for name in sorted(os.listdir(os.path.join(FIX, "pfapack"))):
+        if not name.startswith("real_"):
+            continue
+        n = int(re.search(r"_n(\d+)_", name).group(1))
+        if n > 128:
+            continue
+        vals = []
+        for line in open(os.path.join(FIX, "pfapack", name)):
+            line = line.strip()
+            if not line or line.startswith("#") or " " in line:
+                continue
+            vals.append(float(line))
+        arrays[name.replace(".orig.txt", "")] = np.array(vals, dtype=np.float64).reshape(n, n, order="F")
+    np.savez_compressed(os.path.join(OUT, "pfapack_real_inputs.npz"), **arrays)
+
+
+def bilinear():
+    lines = [l for l in open(os.path.join(FIX, "pfaffian_cg/two_hop_bilinear.txt")) if l.strip() and not l.startswith("#")]
+    expected = [l.split() for l in open(os.path.join(FIX, "pfaffian_cg/two_hop_bilinear_c.txt")) if not l.startswith("#")]
+    sizes, values, bits = [], [], []
+    it = iter(lines)
+    for n_s, b in expected:
+        n = int(n_s)
+        assert int(next(it)) == n
+        vals = [struct.unpack(">d", bytes.fromhex(v))[0] for v in next(it).split()]
+        sizes.append(n)
+        values.append(np.array(vals[1:], dtype=np.float64))
+        bits.append(int(b, 16))
+    np.savez_compressed(os.path.join(OUT, "two_hop_bilinear.npz"), sizes=np.array(sizes),
+                        values=np.concatenate(values), bits=np.array(bits, dtype=np.uint64))
+
+
+if __name__ == "__main__":
+    sfmt()
+    hubbard()
+    heisenberg()
+    sr_cg()
+    pfapack()
+    bilinear()
+    print("fixtures written to", OUT)

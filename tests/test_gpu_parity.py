@@ -1,0 +1,204 @@
+"""GPU path (libmvmc_hip.so through its C ABI) against the oracle and the
+reference's golden records.  Bar: bit-exact configurations, SFMT states,
+counters, Pfaffians and inverses (the device reproduces the C operation
+order, -ffp-contract=off); floating-point operands bit-exact against the
+oracle run with the same correctly rounded exp/log, and within the
+reference's own tolerances (1e-14 + 1e-13 |b|) against the C records."""
+import json
+import os
+import struct
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+HUB = os.path.join(GOLDEN, "inputs", "hubbard_chain", "namelist.def")
+HEIS = os.path.join(GOLDEN, "inputs", "heisenberg_chain", "namelist.def")
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def M():
+    import mvmc_hip
+    return mvmc_hip
+
+
+def hexd(s):
+    return struct.unpack(">d", bytes.fromhex(s))[0]
+
+
+def _random_skew(rng, n):
+    a = rng.standard_normal((n, n))
+    return a - a.T
+
+
+def test_pfinv_bit_exact_vs_oracle(M):
+    import torch
+    rng = np.random.default_rng(3)
+    data = np.load(os.path.join(GOLDEN, "pfapack_real_inputs.npz"))
+    cases = [data[k] for k in data.files] + [_random_skew(rng, 224) for _ in range(3)] + \
+            [_random_skew(rng, 16) for _ in range(9)]
+    by_n = {}
+    for A in cases:
+        by_n.setdefault(A.shape[0], []).append(A)
+    for n, mats in sorted(by_n.items()):
+        log(f"pfinv n={n} count={len(mats)}")
+        planes = np.stack([m.T.copy() for m in mats])  # column-major planes
+        pf_o, inv_o, st_o = O.pfinv(planes)
+        dp = torch.tensor(planes, device="cuda")
+        pf = torch.zeros(len(mats), dtype=torch.float64, device="cuda")
+        inv = torch.zeros_like(dp)
+        st = torch.zeros(len(mats), dtype=torch.int32, device="cuda")
+        M.pfaffian_inverse_batched(dp, pf, inv, st, n, len(mats))
+        torch.cuda.synchronize()
+        assert np.array_equal(st.cpu().numpy(), st_o)
+        assert np.array_equal(pf.cpu().numpy(), pf_o), n
+        assert np.array_equal(inv.cpu().numpy(), inv_o), n
+
+
+def test_pfinv_zero_plane_status(M):
+    """Zero plane -> positive INFO, pf = 0, zero-filled inverse; a plane whose
+    LTL elimination meets a zero column (block diagonal) reports the same INFO
+    as the sequential algorithm; failed planes do not disturb their neighbours."""
+    import torch
+    rng = np.random.default_rng(9)
+    z = np.zeros((3, 4, 4))
+    z[1, 0, 1], z[1, 1, 0], z[1, 2, 3], z[1, 3, 2] = 1.0, -1.0, 2.0, -2.0
+    z[2] = _random_skew(rng, 4).T
+    pf_o, inv_o, st_o = O.pfinv(z)
+    dz = torch.tensor(z, device="cuda")
+    pf = torch.zeros(3, dtype=torch.float64, device="cuda")
+    inv = torch.zeros_like(dz)
+    st = torch.zeros(3, dtype=torch.int32, device="cuda")
+    M.pfaffian_inverse_batched(dz, pf, inv, st, 4, 3)
+    torch.cuda.synchronize()
+    s = st.cpu().numpy()
+    assert s[0] > 0 and s[2] == 0
+    assert np.array_equal(s, st_o)
+    assert np.array_equal(pf.cpu().numpy(), pf_o)
+    assert np.array_equal(inv.cpu().numpy(), inv_o)
+    assert pf_o[0] == 0.0 and not inv_o[0].any()
+
+
+def test_pfinv_rejects_odd_n(M):
+    import torch
+    p = torch.zeros((1, 3, 3), dtype=torch.float64, device="cuda")
+    with pytest.raises(M.InvalidShape):
+        M.pfaffian_inverse_batched(p, torch.zeros(1, dtype=torch.float64, device="cuda"), p,
+                                   torch.zeros(1, dtype=torch.int32, device="cuda"), 3, 1)
+
+
+def test_hubbard_step1_operands(M):
+    log("hubbard step-1 sample+measure")
+    O.set_math(1)
+    try:
+        ref = {k: np.array([hexd(x) for x in v]) for k, v in
+               json.load(open(os.path.join(GOLDEN, "hubbard_cg_store0.json"))).items()}
+        run = O.Run(HUB, walkers=1)
+        run.step(do_sr=False)
+        ens = M.Ensemble(HUB, walkers=1)
+        assert np.array_equal(ens.params()[1], run.params()[1])
+        ens.sample()
+        st_g = ens.walker(0)
+        st_o = run.walker(0)
+        assert np.array_equal(st_g["ele_idx"], st_o["ele_idx"])
+        assert np.array_equal(st_g["rng"], st_o["rng"]) and st_g["rng_idx"] == st_o["rng_idx"]
+        assert list(st_g["counter"]) == list(st_o["counter"][:4])
+        ens.measure()
+        e_g, e_o = ens.energy(), run.energy()
+        assert np.array_equal(e_g, e_o), (e_g, e_o)
+        oo_g, ho_g = ens.operands()
+        oo_o, ho_o = run.operands()
+        assert np.array_equal(ho_g, ho_o)
+        assert np.array_equal(oo_g, oo_o[:2 * run.size])
+        assert np.array_equal(ens.o_store(0), run.o_store(0))
+        size = run.size
+        assert e_g[1] == ref["energy"][0]
+        assert np.all(np.abs(ho_g - ref["ho"]) <= 1e-14 + 1e-13 * np.abs(ref["ho"]))
+        assert np.all(np.abs(oo_g - ref["oo"]) <= 1e-14 + 1e-13 * np.abs(ref["oo"]))
+        assert ens.status() == 0
+    finally:
+        O.set_math(0)
+
+
+@pytest.mark.parametrize("cell", ["real_cg_r2w1_s7", "real_cg_r4w1_s7"])
+def test_heisenberg_exchange_walkers_exact(M, cell):
+    log(f"heisenberg {cell}")
+    fx = json.load(open(os.path.join(GOLDEN, f"mpi179_{cell}.json")))
+    W = len(fx["ranks"])
+    ens = M.Ensemble(HEIS, walkers=W)
+    ens.sample()
+    tot = np.zeros(4, dtype=np.int64)
+    for w in range(W):
+        st = ens.walker(w)
+        rec = fx["ranks"][w]
+        for s, vals in rec["eleidx"].items():
+            assert list(st["ele_idx"][int(s)]) == vals, (w, s)
+        assert st["rng_idx"] == rec["rng_idx"]
+        assert list(st["rng"]) == rec["rng"]
+        tot += st["counter"]
+        if w > 0:
+            assert list(st["counter"]) == rec["counter"][:4]
+    assert list(tot) == fx["ranks"][0]["counter"][:4]
+    ens.measure()
+    ops = fx["operands"]
+    e = ens.energy()
+    assert abs(e[1] - ops["energy"][0]) <= 1e-13 + 1e-12 * abs(ops["energy"][0])
+    oo, ho = ens.operands()
+    assert np.all(np.abs(ho - ops["ho"][0::2]) <= 1e-13 + 1e-12 * np.abs(ops["ho"][0::2]))
+    assert np.all(np.abs(oo - ops["oo"][0::2]) <= 1e-13 + 1e-12 * np.abs(ops["oo"][0::2]))
+    # against the oracle with the device arithmetic: bit-exact
+    O.set_math(1)
+    try:
+        run = O.Run(HEIS, walkers=W)
+        run.step(do_sr=False)
+        assert np.array_equal(ens.energy(), run.energy())
+        oo_o, ho_o = run.operands()
+        assert np.array_equal(ho, ho_o)
+        assert np.array_equal(oo, oo_o[:2 * run.size])
+    finally:
+        O.set_math(0)
+
+
+def test_multistep_optimisation_matches_oracle(M):
+    """Five SR-CG steps on 3 walkers: every zvo_out row and every parameter bit-exact."""
+    log("multistep hubbard W=3")
+    O.set_math(1)
+    try:
+        run = O.Run(HUB, walkers=3)
+        ens = M.Ensemble(HUB, walkers=3)
+        for step in range(5):
+            rc = run.step(do_sr=True)
+            row = ens.opt_step()
+            ro = run.energies()[-1]
+            log(f"  step {step}: gpu {row[0]!r} oracle {ro[0]!r}")
+            assert np.array_equal(row, ro), step
+            pg, sg = ens.params()
+            po, so = run.params()
+            assert np.array_equal(pg, po) and np.array_equal(sg, so), step
+    finally:
+        O.set_math(0)
+
+
+@pytest.mark.parametrize("name,variant", [("sr_cg_real.txt", 0), ("sr_cg_real_diag_scale.txt", 1)])
+def test_sr_cg_solver_c_abi(M, name, variant):
+    import torch
+    from test_oracle_fixtures import _parse_cg
+    n, samples, mean, diag, Osamp, g, z_ref, limits = _parse_cg(os.path.join(GOLDEN, name))
+    dev = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+    pack, dm, dd, dg = dev(Osamp), dev(mean), dev(diag), dev(g)
+    for limit, status, x_ref, r_ref, d_ref in limits[:12]:
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        st = M.sr_cg_solve(pack, dm, dd, dg, x, n, 1, samples, 1.0 / samples, 1e-5, 0.0, limit, variant)
+        sto, xo, _, _ = O.cg_solve(mean, diag, Osamp, g, 1.0 / samples, 1e-5, 0.0, limit, bool(variant))
+        assert st == sto == status
+        assert np.array_equal(x.cpu().numpy(), xo), limit

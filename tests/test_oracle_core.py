@@ -1,0 +1,168 @@
+"""Pins of the CPU oracle's building blocks against the reference's golden
+vectors and against what the mathematics fixes."""
+import itertools
+import json
+import math
+import os
+import struct
+from decimal import Decimal, getcontext
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+# ---------------------------------------------------------------- SFMT ----
+def test_sfmt_first_words_match_c_library():
+    g = json.load(open(os.path.join(GOLDEN, "sfmt_golden.json")))
+    for seed, words in g["first8_u32"].items():
+        assert list(O.sfmt_words(int(seed), 8)) == words, seed
+
+
+def test_sfmt_refill_boundaries_match_c_library():
+    g = json.load(open(os.path.join(GOLDEN, "sfmt_golden.json")))
+    words = O.sfmt_words(g["boundary_seed"], 2000)
+    for idx, want in g["boundary"]:
+        assert int(words[idx]) == want, idx
+    w = O.sfmt_words(g["first8_u64_seed"], 16)
+    u64 = [int(w[2 * i]) | (int(w[2 * i + 1]) << 32) for i in range(8)]
+    assert u64 == g["first8_u64"]
+
+
+# ---------------------------------------------------------- Pfaffian ------
+def _random_skew(rng, n):
+    a = rng.standard_normal((n, n))
+    return a - a.T
+
+
+def _pf_bruteforce(A):
+    """Pfaffian by expansion along the first row (exact combinatorial definition)."""
+    n = A.shape[0]
+    if n == 0:
+        return 1.0
+    total = 0.0
+    for j in range(1, n):
+        idx = [k for k in range(n) if k not in (0, j)]
+        sub = A[np.ix_(idx, idx)]
+        total += (-1) ** (j + 1) * A[0, j] * _pf_bruteforce(sub)
+    return total
+
+
+def _col_major_planes(mats):
+    # plane stored column-major: element (r, c) at r + c*n -> array [c][r] = A.T
+    return np.stack([m.T.copy() for m in mats])
+
+
+def test_pfaffian_closed_forms():
+    a = np.array([[0.0, 1.7], [-1.7, 0.0]])
+    pf, inv, st = O.pfinv(_col_major_planes([a]))
+    assert st[0] == 0 and pf[0] == 1.7
+    rng = np.random.default_rng(4)
+    b = _random_skew(rng, 4)
+    want = b[0, 1] * b[2, 3] - b[0, 2] * b[1, 3] + b[0, 3] * b[1, 2]
+    pf, inv, st = O.pfinv(_col_major_planes([b]))
+    assert abs(pf[0] - want) <= 16 * np.finfo(float).eps * (abs(want) + 1)
+
+
+@pytest.mark.parametrize("n", [2, 4, 6, 8, 10])
+def test_pfaffian_matches_bruteforce(n):
+    rng = np.random.default_rng(100 + n)
+    for _ in range(4):
+        a = _random_skew(rng, n)
+        want = _pf_bruteforce(a)
+        pf, inv, st = O.pfinv(_col_major_planes([a]))
+        assert st[0] == 0
+        assert abs(pf[0] - want) <= 4 * n * np.finfo(float).eps * max(1.0, abs(want)) * 10
+
+
+def test_pfaffian_sign_flip_and_transpose():
+    rng = np.random.default_rng(7)
+    a = _random_skew(rng, 8)
+    pf1 = O.pfinv(_col_major_planes([a]))[0][0]
+    pf2 = O.pfinv(_col_major_planes([-a]))[0][0]  # Pf(-A) = (-1)^(n/2) Pf(A)
+    assert abs(pf1 - pf2) <= 1e-13 * abs(pf1)
+    p = np.arange(8)
+    p[[0, 1]] = p[[1, 0]]
+    pf3 = O.pfinv(_col_major_planes([a[np.ix_(p, p)]]))[0][0]  # one transposition flips the sign
+    assert abs(pf1 + pf3) <= 1e-13 * abs(pf1)
+
+
+def test_pfapack_fixture_inputs_identities():
+    """Reference inputs (tests/fixtures/pfapack/real_n*.orig.txt): Pf^2 = det (LU,
+    independent) and the inverse residual, the identities of
+    crates/mvmc-gpu/src/testkit.rs:167-189 and crates/pfapack/tests/fixed_matrix_identities.rs."""
+    data = np.load(os.path.join(GOLDEN, "pfapack_real_inputs.npz"))
+    for name in data.files:
+        A = data[name]
+        n = A.shape[0]
+        pf, inv, st = O.pfinv(_col_major_planes([A]))
+        assert st[0] == 0, name
+        sign, logdet = np.linalg.slogdet(A)
+        assert sign > 0
+        assert abs(2 * math.log(abs(pf[0])) - logdet) <= n * 1e-13 + 1e-12 * abs(logdet), name
+        X = inv[0].T  # back to row-major
+        res = np.linalg.norm(A @ X - np.eye(n)) / (np.linalg.norm(A) * np.linalg.norm(X) * n)
+        assert res <= 64 * np.finfo(float).eps, (name, res)
+        # inverse of a skew matrix is skew to rounding
+        assert np.linalg.norm(X + X.T) <= 1e-12 * np.linalg.norm(X)
+
+
+def test_pfinv_status_codes():
+    z = np.zeros((4, 4))
+    pf, inv, st = O.pfinv(_col_major_planes([z]))
+    assert st[0] > 0 and pf[0] == 0.0 and not inv.any()
+
+
+def test_two_hop_bilinear_matches_c_bits():
+    d = np.load(os.path.join(GOLDEN, "two_hop_bilinear.npz"))
+    sizes, values, bits = d["sizes"], d["values"], d["bits"]
+    off = 0
+    import ctypes
+    L = O.lib()
+    L.oracle_two_hop_bilinear.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    assert len(sizes) >= 100
+    for n, b in zip(sizes, bits):
+        n = int(n)
+        block = values[off:off + n * n + 2 * n]
+        off += n * n + 2 * n
+        m = np.ascontiguousarray(block[:n * n])
+        a = np.ascontiguousarray(block[n * n:n * n + n])
+        bb = np.ascontiguousarray(block[n * n + n:])
+        got = L.oracle_two_hop_bilinear(O.ptr(m), n, O.ptr(a), O.ptr(bb))
+        assert struct.unpack("<Q", struct.pack("<d", got))[0] == int(b)
+
+
+# ---------------------------------------------------------- exp / log -----
+def test_correctly_rounded_exp_log_against_high_precision():
+    getcontext().prec = 60
+    rng = np.random.default_rng(11)
+    L = O.lib()
+    for x in np.concatenate([rng.uniform(-60, 8, 300), rng.uniform(-1e-3, 1e-3, 50)]):
+        x = float(x)
+        got = L.oracle_cr_exp(x)
+        exact = Decimal(x).exp()
+        lo, hi = np.nextafter(got, -np.inf), np.nextafter(got, np.inf)
+        assert abs(Decimal(got) - exact) <= abs(Decimal(lo) - exact) and abs(Decimal(got) - exact) <= abs(Decimal(hi) - exact)
+    for x in np.exp(rng.uniform(-300, 300, 300)):
+        x = float(x)
+        got = L.oracle_cr_log(x)
+        exact = Decimal(x).ln()
+        lo, hi = np.nextafter(got, -np.inf), np.nextafter(got, np.inf)
+        assert abs(Decimal(got) - exact) <= abs(Decimal(lo) - exact) and abs(Decimal(got) - exact) <= abs(Decimal(hi) - exact)
+
+
+def test_correctly_rounded_math_agrees_with_libm_nearly_always():
+    """glibc exp/log are not correctly rounded in every case; the device uses
+    the correctly rounded values.  Bound the disagreement rate (1 ulp) that
+    drives the decision-flip probability stated in DESIGN.md."""
+    rng = np.random.default_rng(5)
+    L = O.lib()
+    xs = rng.uniform(-40, 5, 200000)
+    mism = sum(1 for x in xs if L.oracle_cr_exp(float(x)) != math.exp(float(x)))
+    assert mism / xs.size < 2e-3
+    ys = np.exp(rng.uniform(-200, 200, 200000))
+    mism_l = sum(1 for y in ys if L.oracle_cr_log(float(y)) != math.log(float(y)))
+    assert mism_l / ys.size < 1e-4
