@@ -7,7 +7,9 @@ A step is one SR step in the steady state: every walker restores its burn
 configuration, runs (NVMCSample + 1) x NVMCInterval x Nsite proposals,
 saves NVMCSample samples, the samples are measured and reduced, and the
 SR-CG solve updates the parameters.  --warmup untimed steps come first (the
-first one thermalises with NVMCWarmUp outer steps).
+first one thermalises with NVMCWarmUp x NVMCInterval x Nsite extra
+proposals; --vmc-warmup sets NVMCWarmUp, default 2, which only changes the
+length of that untimed step).
 
 value  = saved samples / second of the whole timed SR steps (all walkers).
 roofline: dominant kernel = the sampling kernel k_block; algorithmic bytes
@@ -57,10 +59,16 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="square16")
+    ap.add_argument("--vmc-warmup", type=int, default=2,
+                    help="NVMCWarmUp of the generated workload: thermalisation sweeps of the first (untimed) "
+                         "SR step only; steady-state steps do not depend on it")
     ap.add_argument("--walkers", type=int, default=0, help="override (not a valid headline number)")
     ap.add_argument("--cpu-walkers", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="before the run, copy a buffer of this many MiB once (device-to-device), the known-size "
+                         "kernel that scales FETCH_SIZE / WRITE_SIZE counters in a profiler run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); walkers shard across GPUs "
@@ -69,8 +77,17 @@ def main():
 
     import lattice
     import mvmc_hip
+    if args.copy_probe_mib > 0:
+        import torch
+        nel = args.copy_probe_mib * (1 << 20) // 8
+        src = torch.ones(nel, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written")
+        del src, dst
     wl_dir = os.path.join(tempfile.gettempdir(), f"mvmc_bench_{args.workload}")
-    info = lattice.write_workload(args.workload, wl_dir, sr_steps=args.steps + args.warmup)
+    info = lattice.write_workload(args.workload, wl_dir, sr_steps=args.steps + args.warmup, warmup=args.vmc_warmup)
     walkers = args.walkers or info["walkers"]
     log(f"workload {args.workload}: Nsite={info['nsite']} Nsize={info['nsize']} NQP={info['nqp']} "
         f"NPara={info['npara']} walkers={walkers}")

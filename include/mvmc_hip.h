@@ -95,6 +95,14 @@ int mvmc_ens_get_cg(mvmc_ens* e, double* solution, int* n_active);
  * proposals, accepted, hop-kernel device ms of the last k_block launch set,
  * k_block launches, reserved. */
 int mvmc_ens_timings(mvmc_ens* e, double* timings8);
+/* Device phase clocks of the sampling kernel, summed over walkers (ticks of
+ * the 100 MHz constant clock): 0 candidate, 1 projection deltas + ratio,
+ * 2 inverse/Slater row staging, 3 Pfaffian ratio dots, 4 Metropolis,
+ * 5 accepted-hop inverse pass, 6 periodic rebuild, 7 exchange path,
+ * 8 block end.  Only libraries built with MVMC_PHASE_CLOCKS collect them;
+ * others return MVMC_E_UNAVAILABLE.  No reference counterpart (profiling aid,
+ * like the reference's timer sections in crates/mvmc-core/src/timer.rs). */
+int mvmc_ens_phase_clocks(mvmc_ens* e, long long* clocks16);
 /* Sticky device status word (0 ok; 1 init retries exhausted, 2 bad input,
  * 3 rebuild failure, 4 non-finite, 5 CG failure). */
 int mvmc_ens_status(mvmc_ens* e);

@@ -27,16 +27,19 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build_library():
-    os.makedirs(BUILD, exist_ok=True)
+def build_library(defines=(), out=None):
+    """defines: extra -D macros (MVMC_PHASE_CLOCKS adds the sampling kernel's
+    phase clocks); out: output path (default mvmc-rs_amd/_build/libmvmc_hip.so)."""
+    out = out or LIB
+    os.makedirs(os.path.dirname(out), exist_ok=True)
     srcs = [os.path.join(HERE, "csrc", "capi.hip"), os.path.join(HERE, "host", "model.cpp")]
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-Wno-unused-result", "-o", tmp] + srcs
+           "-Wno-unused-result"] + [f"-D{d}" for d in defines] + ["-o", tmp] + srcs
     _run(cmd)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 def build_oracle():

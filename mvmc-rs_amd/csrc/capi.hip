@@ -63,29 +63,34 @@ void require_gpu() {
 
 // Device scratch of the SR-CG solver.
 struct CgWork {
-  double *x = nullptr, *r = nullptr, *d = nullptr, *q = nullptr, *y = nullptr, *g = nullptr;
-  double* sc = nullptr;  // scalars: 0 delta, 1 dq, 2 alpha, 3 dnew, 4 beta, 5 coef
-  int* flag = nullptr;   // 0 alpha fail, 1 beta fail
+  double *x = nullptr, *r = nullptr, *d = nullptr, *q = nullptr, *g = nullptr;
+  double *cpart = nullptr, *spart = nullptr;  // chunk / superchunk partials of S x
+  double* sc = nullptr;  // scalars: 0 delta, 2 alpha, 4 beta, 5 coef
+  int* ctl = nullptr;    // 0 done, 1..2 status (int64)
   int cap_n = 0, cap_rows = 0;
   void ensure(int na, int rows) {
     if (na <= cap_n && rows <= cap_rows) return;
     release();
     cap_n = std::max(na, 1);
     cap_rows = std::max(rows, 1);
+    const size_t nch = (size_t)(cap_rows + mvmc_hip::CG_RCH - 1) / mvmc_hip::CG_RCH;
+    const size_t nsup = (nch + mvmc_hip::CG_SCH - 1) / mvmc_hip::CG_SCH;
     HIPCHK(hipMalloc(&x, sizeof(double) * cap_n));
     HIPCHK(hipMalloc(&r, sizeof(double) * cap_n));
     HIPCHK(hipMalloc(&d, sizeof(double) * cap_n));
     HIPCHK(hipMalloc(&q, sizeof(double) * cap_n));
     HIPCHK(hipMalloc(&g, sizeof(double) * cap_n));
-    HIPCHK(hipMalloc(&y, sizeof(double) * cap_rows));
+    HIPCHK(hipMalloc(&cpart, sizeof(double) * cap_n * nch));
+    HIPCHK(hipMalloc(&spart, sizeof(double) * cap_n * nsup));
     HIPCHK(hipMalloc(&sc, sizeof(double) * 8));
-    HIPCHK(hipMalloc(&flag, sizeof(int) * 4));
+    HIPCHK(hipMalloc(&ctl, sizeof(int) * 4));
   }
   void release() {
-    for (void* p : {(void*)x, (void*)r, (void*)d, (void*)q, (void*)y, (void*)g, (void*)sc, (void*)flag})
+    for (void* p : {(void*)x, (void*)r, (void*)d, (void*)q, (void*)g, (void*)cpart, (void*)spart, (void*)sc,
+                    (void*)ctl})
       if (p) (void)hipFree(p);
-    x = r = d = q = y = g = sc = nullptr;
-    flag = nullptr;
+    x = r = d = q = g = cpart = spart = sc = nullptr;
+    ctl = nullptr;
     cap_n = cap_rows = 0;
   }
 };
@@ -93,55 +98,6 @@ struct CgWork {
 }  // namespace
 
 namespace mvmc_hip {
-
-__global__ void k_cg_precond(int n, double pfac, const double* diag, const double* r, double* out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double EPS = 1.0e-12;
-  double dq = pfac * diag[i];
-  if (!(dq - dq == 0.0) || fabs(dq) < EPS) dq = dq < 0.0 ? -EPS : EPS;
-  out[i] = r[i] / dq;
-}
-
-// alpha step guarded by the device flag; x += alpha d; r -= alpha q unless
-// this is a refresh iteration (then r is recomputed by the caller).
-__global__ void k_cg_update_x(int n, const double* sc, const int* flag, const double* d, double* x, const double* q,
-                              double* r, int refresh) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || flag[0]) return;
-  const double a = sc[2];
-  x[i] += a * d[i];
-  if (!refresh) r[i] -= a * q[i];
-}
-
-__global__ void k_cg_alpha2(double* sc, int* flag) {
-  if (threadIdx.x || blockIdx.x) return;
-  const double q = sc[1];
-  flag[0] = 0;
-  if (!(q - q == 0.0) || fabs(q) < 1.0e-20) { flag[0] = 1; return; }
-  const double a = sc[0] / q;
-  if (!(a - a == 0.0)) { flag[0] = 1; return; }
-  sc[2] = a;
-}
-
-// beta = dnew/delta; plain: delta *= beta; diag-scale: delta = dnew.
-__global__ void k_cg_beta2(double* sc, int* flag, int diag_scale) {
-  if (threadIdx.x || blockIdx.x) return;
-  flag[1] = 0;
-  if (flag[0]) return;
-  const double dn = sc[3], dl = sc[0];
-  if (!(dn - dn == 0.0) || fabs(dl) < 1.0e-20) { flag[1] = 1; return; }
-  const double b = dn / dl;
-  if (!(b - b == 0.0)) { flag[1] = 1; return; }
-  sc[4] = b;
-  sc[0] = diag_scale ? dn : dl * b;
-}
-
-__global__ void k_cg_dir2(int n, const double* sc, const int* flag, const double* src, double* d) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || flag[0] || flag[1]) return;
-  d[i] = src[i] + sc[4] * d[i];
-}
 
 __global__ void k_fill(double* p, int n, double v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -154,81 +110,81 @@ namespace {
 
 inline int nblk(long long n, int b = 256) { return (int)((n + b - 1) / b); }
 
-// z = S x with the sampled operator (sr_cg.rs:952-1004)
-void cg_apply_dev(int na, int W, int NS, const double* pack, const double* mean, const double* diag,
-                  const double* xv, double* z, double inv_w, double shift, CgWork& wk, hipStream_t st) {
-  const int rows = W * NS;
-  hipLaunchKernelGGL(k_cg_y, dim3(nblk(rows)), dim3(256), 0, st, pack, rows, na, xv, wk.y);
-  hipLaunchKernelGGL(k_seq_dot, dim3(1), dim3(64), 0, st, mean, xv, na, wk.sc + 5);
-  hipLaunchKernelGGL(k_cg_z, dim3(nblk(na)), dim3(256), 0, st, pack, W, NS, na, wk.y, xv, mean, diag, wk.sc + 5,
-                     inv_w, shift, z);
+// out = S x (or g - S x when g is given) with the sampled operator
+// (sr_cg.rs:952-1004) in the blocked reduction order.
+void cg_apply_dev(int na, int rows, const double* pack, const double* mean, const double* diag, const double* xv,
+                  const double* g, double* out, double inv_w, double shift, CgWork& wk, hipStream_t st) {
+  const int nch = (rows + CG_RCH - 1) / CG_RCH;
+  const int nsup = (nch + CG_SCH - 1) / CG_SCH;
+  const int cols = (na + 255) / 256;
+  const dim3 grid(nch + 1), blk(256);
+  double* coef = wk.sc + 5;
+  if (cols <= 4)
+    hipLaunchKernelGGL(k_cg_chunk<4>, grid, blk, 0, st, pack, rows, na, xv, mean, wk.cpart, coef, wk.ctl);
+  else if (cols <= 8)
+    hipLaunchKernelGGL(k_cg_chunk<8>, grid, blk, 0, st, pack, rows, na, xv, mean, wk.cpart, coef, wk.ctl);
+  else if (cols <= 16)
+    hipLaunchKernelGGL(k_cg_chunk<16>, grid, blk, 0, st, pack, rows, na, xv, mean, wk.cpart, coef, wk.ctl);
+  else if (cols <= CG_CMAX)
+    hipLaunchKernelGGL(k_cg_chunk<CG_CMAX>, grid, blk, 0, st, pack, rows, na, xv, mean, wk.cpart, coef, wk.ctl);
+  else
+    hipLaunchKernelGGL(k_cg_chunk_wide, grid, blk, 0, st, pack, rows, na, xv, mean, wk.cpart, coef, wk.ctl);
+  if (nsup > 0)
+    hipLaunchKernelGGL(k_cg_fold1, dim3(nblk(na), nsup), blk, 0, st, wk.cpart, nch, na, wk.spart, wk.ctl);
+  hipLaunchKernelGGL(k_cg_fold2, dim3(nblk(na)), blk, 0, st, wk.spart, nsup, na, xv, mean, diag, coef, inv_w, shift,
+                     g, out, wk.ctl);
 }
 
 // C 1.4.0 StochasticOptCG_Main / PreconCG_DiagScale (sr_cg.rs:700-880).
+// Iterations run on the device; a done flag set by the head / guard kernels
+// turns the rest of a batch into no-ops, and the host reads the flag once per
+// batch of CG_BATCH iterations.
+constexpr int CG_BATCH = 16;
+
 long long cg_solve_dev(int na, int W, int NS, const double* pack, const double* mean, const double* diag,
                        const double* g, double inv_w, double shift, double tol, int max_it, bool diag_scale,
                        double* x_out, CgWork& wk, hipStream_t st) {
-  wk.ensure(na, W * NS);
+  const int rows = W * NS;
+  wk.ensure(na, rows);
   const double threshold = tol * tol * (double)na * (double)na;
   const double pfac = 1.0 + shift;
+  const dim3 blk(256);
+  HIPCHK(hipMemsetAsync(wk.ctl, 0, sizeof(int) * 4, st));
   HIPCHK(hipMemcpyAsync(wk.g, g, sizeof(double) * na, hipMemcpyDeviceToDevice, st));
-  hipLaunchKernelGGL(k_fill, dim3(nblk(na)), dim3(256), 0, st, wk.x, na, 0.0);
+  hipLaunchKernelGGL(k_fill, dim3(nblk(na)), blk, 0, st, wk.x, na, 0.0);
   HIPCHK(hipMemcpyAsync(wk.r, g, sizeof(double) * na, hipMemcpyDeviceToDevice, st));
-  if (diag_scale) {
-    cg_apply_dev(na, W, NS, pack, mean, diag, wk.x, wk.q, inv_w, shift, wk, st);
-    hipLaunchKernelGGL(k_cg_resid, dim3(nblk(na)), dim3(256), 0, st, na, wk.g, wk.q, wk.r);
-    hipLaunchKernelGGL(k_cg_precond, dim3(nblk(na)), dim3(256), 0, st, na, pfac, diag, wk.r, wk.d);
-    hipLaunchKernelGGL(k_seq_dot, dim3(1), dim3(64), 0, st, wk.r, wk.d, na, wk.sc + 0);
-  } else {
+  if (diag_scale)
+    cg_apply_dev(na, rows, pack, mean, diag, wk.x, wk.g, wk.r, inv_w, shift, wk, st);
+  else
     HIPCHK(hipMemcpyAsync(wk.d, g, sizeof(double) * na, hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(k_seq_dot, dim3(1), dim3(64), 0, st, wk.r, wk.r, na, wk.sc + 0);
+  hipLaunchKernelGGL(k_cg_init, dim3(1), blk, 0, st, na, wk.r, wk.d, diag, pfac, diag_scale ? 1 : 0, wk.sc, wk.ctl);
+  long long status = 0;
+  for (int iter0 = 0;; iter0 += CG_BATCH) {
+    for (int iter = iter0; iter < iter0 + CG_BATCH && iter <= max_it; ++iter) {
+      hipLaunchKernelGGL(k_cg_head, dim3(1), dim3(64), 0, st, iter, max_it, threshold, wk.sc, wk.ctl);
+      if (iter == max_it) break;
+      cg_apply_dev(na, rows, pack, mean, diag, wk.d, nullptr, wk.q, inv_w, shift, wk, st);
+      hipLaunchKernelGGL(k_cg_alpha, dim3(1), blk, 0, st, iter, na, wk.d, wk.q, wk.sc, wk.ctl);
+      const int refresh = ((iter + 1) % 20 == 0);
+      hipLaunchKernelGGL(k_cg_update, dim3(nblk(na)), blk, 0, st, na, wk.sc, wk.ctl, wk.d, wk.x, wk.q, wk.r,
+                         refresh);
+      if (refresh) cg_apply_dev(na, rows, pack, mean, diag, wk.x, wk.g, wk.r, inv_w, shift, wk, st);
+      hipLaunchKernelGGL(k_cg_beta, dim3(1), blk, 0, st, iter, na, wk.r, wk.q, diag, pfac, diag_scale ? 1 : 0, wk.sc,
+                         wk.ctl);
+      hipLaunchKernelGGL(k_cg_dir, dim3(nblk(na)), blk, 0, st, na, wk.sc, wk.ctl, diag_scale ? wk.q : wk.r, wk.d);
+    }
+    int ctl[4];
+    HIPCHK(hipMemcpyAsync(ctl, wk.ctl, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (ctl[0]) {
+      status = (long long)(((unsigned long long)(unsigned)ctl[2] << 32) | (unsigned)ctl[1]);
+      break;
+    }
+    if (iter0 + CG_BATCH > max_it) throw Fail(MVMC_E_BACKEND, "SR-CG control flag not set at the iteration limit");
   }
-  double delta;
-  HIPCHK(hipMemcpyAsync(&delta, wk.sc, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(x_out, wk.x, sizeof(double) * na, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipStreamSynchronize(st));
-  auto finish = [&](long long status) {
-    HIPCHK(hipMemcpyAsync(x_out, wk.x, sizeof(double) * na, hipMemcpyDeviceToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));
-    return status;
-  };
-  if (!std::isfinite(delta)) return finish(-1);
-  int iter = 0;
-  while (iter < max_it) {
-    const long long failed = -(long long)(iter + 1);
-    if (!std::isfinite(delta)) return finish(failed);
-    if (std::fabs(delta) < threshold) break;
-    cg_apply_dev(na, W, NS, pack, mean, diag, wk.d, wk.q, inv_w, shift, wk, st);
-    hipLaunchKernelGGL(k_seq_dot, dim3(1), dim3(64), 0, st, wk.d, wk.q, na, wk.sc + 1);
-    hipLaunchKernelGGL(k_cg_alpha2, dim3(1), dim3(64), 0, st, wk.sc, wk.flag);
-    const int refresh = ((iter + 1) % 20 == 0);
-    hipLaunchKernelGGL(k_cg_update_x, dim3(nblk(na)), dim3(256), 0, st, na, wk.sc, wk.flag, wk.d, wk.x, wk.q, wk.r,
-                       refresh);
-    if (refresh) {
-      // r = g - S x (the product needs no guard: x is unchanged on failure and
-      // the result is discarded by the failure return below)
-      cg_apply_dev(na, W, NS, pack, mean, diag, wk.x, wk.q, inv_w, shift, wk, st);
-      hipLaunchKernelGGL(k_cg_resid, dim3(nblk(na)), dim3(256), 0, st, na, wk.g, wk.q, wk.r);
-    }
-    if (diag_scale) {
-      hipLaunchKernelGGL(k_cg_precond, dim3(nblk(na)), dim3(256), 0, st, na, pfac, diag, wk.r, wk.q);
-      hipLaunchKernelGGL(k_seq_dot, dim3(1), dim3(64), 0, st, wk.r, wk.q, na, wk.sc + 3);
-    } else {
-      hipLaunchKernelGGL(k_seq_dot, dim3(1), dim3(64), 0, st, wk.r, wk.r, na, wk.sc + 3);
-    }
-    hipLaunchKernelGGL(k_cg_beta2, dim3(1), dim3(64), 0, st, wk.sc, wk.flag, diag_scale ? 1 : 0);
-    hipLaunchKernelGGL(k_cg_dir2, dim3(nblk(na)), dim3(256), 0, st, na, wk.sc, wk.flag, diag_scale ? wk.q : wk.r,
-                       wk.d);
-    int flags[2];
-    double sc0;
-    HIPCHK(hipMemcpyAsync(flags, wk.flag, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&sc0, wk.sc, sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    if (flags[0] || flags[1]) return finish(failed);
-    delta = sc0;
-    ++iter;
-  }
-  if (iter == max_it && std::fabs(delta) >= threshold) return finish(-(long long)(max_it + 1));
-  return finish(iter);
+  return status;
 }
 
 template <class T>
@@ -473,8 +429,15 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
   E->grid_samp = (int)std::min<long long>(walkers, (long long)occ_s * ncu);
   E->grid_meas = (int)std::min<long long>(walkers, (long long)occ_m * ncu);
   E->slots = std::max(E->grid_samp, E->grid_meas);
-  v.work = dalloc<double>(A, (size_t)E->slots * v.nqp * v.n * v.n);
+  v.work = dalloc<double>(A, (size_t)E->slots * pfinv_work_doubles(v.n, v.nqp));
   v.qorb = dalloc<double>(A, (size_t)E->slots * v.nqp * std::max(v.nslater, 1));
+  v.pend = dalloc<double>(A, (size_t)E->slots * 2 * v.nqp * v.n);
+#ifdef MVMC_PHASE_CLOCKS
+  v.clk = dalloc<long long>(A, (size_t)E->slots * 16);
+  HIPCHK(hipMemset(v.clk, 0, sizeof(long long) * E->slots * 16));
+#else
+  v.clk = nullptr;
+#endif
   E->gstride = std::max(1, std::max(v.ntrans, v.nexch));
   E->d_gscr = dalloc<double>(A, (size_t)E->slots * E->gstride);
   E->d_out5 = dalloc<double>(A, 8);
@@ -670,8 +633,8 @@ int mvmc_pfinv_f64(const double* planes, int n, int count, double* pf, double* i
     const int np = n <= 32 ? 8 : (n <= 64 ? 4 : 1);
     const int blocks = (count + np - 1) / np;
     double* work = nullptr;
-    HIPCHK(hipMallocAsync((void**)&work, sizeof(double) * (size_t)blocks * np * n * n, st));
-    size_t lds = rnd16((size_t)np * n * 4) + rnd16((size_t)np * n * 8) + 3 * rnd16(np * 8) + 64;
+    HIPCHK(hipMallocAsync((void**)&work, sizeof(double) * (size_t)blocks * pfinv_work_doubles(n, np), st));
+    size_t lds = pf_bytes(n, np) + 64;
     HIPCHK(hipFuncSetAttribute((const void*)k_pfinv_batched, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_pfinv_batched, dim3(blocks), dim3(256), lds, st, inv, work, n, count, np, pf, status);
     HIPCHK(hipGetLastError());
@@ -803,6 +766,17 @@ int mvmc_ens_get_cg(mvmc_ens* e, double* x, int* n_active) {
   if (x) std::copy(e->cg_x.begin(), e->cg_x.end(), x);
   if (n_active) *n_active = (int)e->cg_x.size();
   return (int)e->cg_info;
+}
+
+int mvmc_ens_phase_clocks(mvmc_ens* e, long long* out) {
+  return guarded([&] {
+    if (!e->v.clk) throw Fail(MVMC_E_UNAVAILABLE, "library built without MVMC_PHASE_CLOCKS");
+    std::vector<long long> h((size_t)e->slots * 16);
+    HIPCHK(hipMemcpy(h.data(), e->v.clk, sizeof(long long) * h.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 16; ++k) out[k] = 0;
+    for (int sl = 0; sl < e->slots; ++sl)
+      for (int k = 0; k < 16; ++k) out[k] += h[(size_t)sl * 16 + k];
+  });
 }
 
 int mvmc_ens_timings(mvmc_ens* e, double* t) {

@@ -56,7 +56,9 @@ struct EnsView {
   // per-slot scratch (persistent grids: slot = blockIdx.x)
   double* work;       // [slots][nqp][n*n]
   double* qorb;       // [slots][nqp][nslater]
+  double* pend;       // [slots][2][nqp][n]: vec1 / vec2 of the not yet applied hop update
   int* status;        // [1] sticky
+  long long* clk;     // [slots][16] phase clocks (MVMC_PHASE_CLOCKS builds), else null
 };
 
 }  // namespace mvmc_hip

@@ -41,10 +41,6 @@ struct Carver {
 
 __host__ __device__ inline size_t rnd16(size_t b) { return (b + 15) & ~(size_t)15; }
 
-__host__ __device__ inline size_t pf_bytes(int n, int np) {
-  return rnd16((size_t)np * n * sizeof(int)) + rnd16((size_t)np * n * sizeof(double)) + 3 * rnd16(np * 8) ;
-}
-
 __host__ __device__ inline size_t union_doubles_sampler(const EnsView& v) {
   const size_t nq = (size_t)v.nqp * v.n;
   size_t hop = 3 * nq;
@@ -132,12 +128,7 @@ __device__ int wg_calc_m_all(const EnsView& v, const int* ele_idx_l, int* rs, do
   assemble_planes(planes, v.slater, rs, v.n, v.n2, v.nqp, 0);
   Carver c{pfscratch};
   PfinvShared sh;
-  sh.ld = v.n;
-  sh.piv = c.take<int>((size_t)v.nqp * v.n);
-  sh.colbuf = c.take<double>((size_t)v.nqp * v.n);
-  sh.kp = c.take<int>(v.nqp);
-  sh.info = c.take<int>(v.nqp);
-  sh.colmax = c.take<double>(v.nqp);
+  carve_pfinv(c, sh, v.n, v.nqp);
   __syncthreads();
   __shared__ int allzero_flag;
   if (!native) {
@@ -336,11 +327,11 @@ __device__ void make_proj_cnt(const EnsView& v, const int* ele_num, int* cnt) {
 // begin: burn restore or validated initial placement, initial tables, log|ip|
 // (driver.rs:239-330)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_begin(EnsView v) {
+__global__ void __launch_bounds__(256, 3) k_begin(EnsView v) {
   extern __shared__ __align__(16) char lds[];
   SampLds L = carve_sampler(v, lds);
   const int tid = threadIdx.x;
-  double* work = v.work + (size_t)blockIdx.x * v.nqp * v.n * v.n;
+  double* work = v.work + (size_t)blockIdx.x * pfinv_work_doubles(v.n, v.nqp);
   char* pfscr = (char*)L.uni;
   for (int w = blockIdx.x; w < v.W; w += gridDim.x) {
     int* ws = v.wstate + (size_t)w * 8;
@@ -348,30 +339,30 @@ __global__ void __launch_bounds__(256) k_begin(EnsView v) {
     int* cnt = v.proj_cnt + (size_t)w * v.nproj;
     load_walker(v, w, L, true);
     int burn = ws[0];
-    auto place = [&]() {
-      for (int attempt = 1; attempt <= 101; ++attempt) {
+    // phase 0: fresh placement (up to 100 retries while calc_m_all fails),
+    // 1: recompute from the burn configuration, 2: one fresh placement
+    // after a non-finite log|<x|psi>|.  A single calc_m_all call site.
+    int phase = burn ? 1 : 0, attempt = 0;
+    __shared__ double lip;
+    for (;;) {
+      const bool placing = phase != 1;
+      if (placing) {
+        ++attempt;
         if (tid == 0) initial_placement(v, L);
         __syncthreads();
-        int bad = wg_calc_m_all(v, L.ele_idx, L.rs, planes, work, L.pf, pfscr, true);
-        if (attempt > 100) { set_status(v.status, ST_INIT_RETRY); return; }
-        if (!bad) return;
       }
-    };
-    if (!burn) {
-      place();
-      make_proj_cnt(v, L.ele_num, cnt);
-    } else {
-      wg_calc_m_all(v, L.ele_idx, L.rs, planes, work, L.pf, pfscr, true);
-    }
-    __shared__ double lip;
-    if (tid == 0) lip = ddm::cr_log(fabs(ip_of(v, L.pf)));
-    __syncthreads();
-    if (!(lip - lip == 0.0)) {
-      place();
-      make_proj_cnt(v, L.ele_num, cnt);
+      const int bad = wg_calc_m_all(v, L.ele_idx, L.rs, planes, work, L.pf, pfscr, true);
+      if (placing) {
+        if (attempt <= 100 && bad) continue;
+        if (attempt > 100) set_status(v.status, ST_INIT_RETRY);
+        make_proj_cnt(v, L.ele_num, cnt);
+      }
       if (tid == 0) lip = ddm::cr_log(fabs(ip_of(v, L.pf)));
-      burn = 0;
       __syncthreads();
+      if (phase == 2 || lip - lip == 0.0) break;
+      phase = 2;
+      attempt = 0;
+      burn = 0;
     }
     for (int q = tid; q < v.nqp; q += blockDim.x) v.pf[(size_t)w * v.nqp + q] = L.pf[q];
     if (tid == 0) {
@@ -391,7 +382,9 @@ __global__ void __launch_bounds__(256) k_begin(EnsView v) {
 // ---------------------------------------------------------------------------
 __device__ inline int jast_of(const EnsView& v, int a, int b) {
   if (a == b) return -1;
-  return a < b ? v.jast_idx[(size_t)a * v.ns + b] : v.jast_idx[(size_t)b * v.ns + a];
+  // the table is stored in full (the loader checks its symmetry), so row a
+  // is contiguous and lanes over b read it coalesced
+  return v.jast_idx[(size_t)a * v.ns + b];
 }
 
 __device__ void hop_deltas(const EnsView& v, SampLds& L, int ri, int rj) {
@@ -467,16 +460,161 @@ __device__ inline void apply_clear_deltas(const EnsView& v, SampLds& L, int* cnt
 // ---------------------------------------------------------------------------
 // block of proposals (engine.rs:188-475) and sample saving (driver.rs:419-443)
 // ---------------------------------------------------------------------------
+constexpr int CG_RCH = 32, CG_SCH = 16, CG_CMAX = 32;  // SR-CG reduction blocking
+constexpr int VEC1_COLS = 4;  // columns per lane held in registers by stream_planes
+
+// One not yet applied hop update (updates.rs:1184-1224: vec1, vec2 = scaled
+// old row `pm`) on element (i, j), with the operation order of the in-place
+// update so the delayed element equals the eagerly updated one bit for bit.
+__device__ inline double apply_pend(double x, double v1i, double v2i, double v1j, double v2j, int i, int j,
+                                    int pm) {
+  x += v1i * v2j - v1j * v2i;
+  if (j == pm) x -= v2i;
+  if (i == pm) x += v2j;
+  return x;
+}
+
+// Raw buffer access to one walker's planes: a 32-bit per-lane byte offset
+// plus a wave-uniform row offset in a scalar register, so a tile of loads
+// costs one address register instead of one 64-bit pointer per load.
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+constexpr int BUF_WORD3 = 0x00020000;  // raw untyped buffer (gfx9 family)
+__device__ inline double2 buf_load2(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+__device__ inline void buf_store2(double2 x, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, x), r, voff, soff, 0);
+}
+
+// One streaming pass over a walker's planes.  A lane owns VEC1_COLS
+// contiguous columns of one plane (q = tid / LP; columns
+// LP*VEC1_COLS*g + VEC1_COLS*(tid % LP) + c), read and written as 16-byte
+// pairs (n is even), and issues the loads of STREAM_ROWS rows before using
+// any of them: each lane keeps STREAM_ROWS * VEC1_COLS loads in flight, which
+// is what bounds a pass over HBM at this occupancy.
+//   pm >= 0 : apply the pending update (pv1, pv2, row pm) and write back;
+//   ACC     : out[q][j] = sum_i -X[i][j] * b[q][i], serial in i (the
+//             update_one_real vec1 = -(X^T b)).
+constexpr int STREAM_ROWS = 8;
+template <bool ACC>
+__device__ void stream_planes(double* planes, int n, int nqp, const double* pv1, const double* pv2, int pm,
+                              const double* b, double* out) {
+  static_assert(VEC1_COLS % 2 == 0, "columns are handled in pairs");
+  constexpr int NP2 = VEC1_COLS / 2;
+  const int nt = blockDim.x, tid = threadIdx.x;
+  const int nn = n * n;
+  const int PG = nqp < nt ? nqp : nt;
+  const int LP = nt / PG;
+  const int qo = tid / LP, jl = tid % LP;
+  const int span = LP * VEC1_COLS;  // columns per group
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc(planes, 0, (int)((size_t)nqp * nn * sizeof(double)), BUF_WORD3);
+  const int rowb = n * (int)sizeof(double);
+  for (int q0 = 0; q0 < nqp; q0 += PG) {
+    const int q = q0 + qo;
+    const bool on = qo < PG && q < nqp;
+    const int qoff = (on ? q : 0) * nn * (int)sizeof(double);
+    const double* p1 = pv1 + (on ? q : 0) * n;
+    const double* p2 = pv2 + (on ? q : 0) * n;
+    const double* bq = b + (on ? q : 0) * n;
+    for (int g0 = 0; g0 < n; g0 += span) {
+      int jp[NP2];  // first column of each pair, -1 when outside the plane
+      double acc[VEC1_COLS], c1[VEC1_COLS], c2[VEC1_COLS];
+#pragma unroll
+      for (int m = 0; m < NP2; ++m) {
+        const int j = g0 + VEC1_COLS * jl + 2 * m;
+        jp[m] = (on && j < n) ? j : -1;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          acc[2 * m + h] = 0.0;
+          c1[2 * m + h] = (pm >= 0 && jp[m] >= 0) ? p1[j + h] : 0.0;
+          c2[2 * m + h] = (pm >= 0 && jp[m] >= 0) ? p2[j + h] : 0.0;
+        }
+      }
+      const int vbase = qoff + (g0 + VEC1_COLS * jl) * (int)sizeof(double);
+      for (int i0 = 0; i0 < n; i0 += STREAM_ROWS) {
+        double2 x[STREAM_ROWS][NP2];
+#pragma unroll
+        for (int r = 0; r < STREAM_ROWS; ++r)
+          if (i0 + r < n) {
+#pragma unroll
+            for (int m = 0; m < NP2; ++m)
+              if (jp[m] >= 0) x[r][m] = buf_load2(rs, vbase + 16 * m, (i0 + r) * rowb);
+          }
+#pragma unroll
+        for (int r = 0; r < STREAM_ROWS; ++r) {
+          const int i = i0 + r;
+          if (i < n) {
+            if (pm >= 0) {
+              const double a0 = p1[i], b0 = p2[i];
+#pragma unroll
+              for (int m = 0; m < NP2; ++m)
+                if (jp[m] >= 0) {
+                  x[r][m].x = apply_pend(x[r][m].x, a0, b0, c1[2 * m], c2[2 * m], i, jp[m], pm);
+                  x[r][m].y = apply_pend(x[r][m].y, a0, b0, c1[2 * m + 1], c2[2 * m + 1], i, jp[m] + 1, pm);
+                  buf_store2(x[r][m], rs, vbase + 16 * m, i * rowb);
+                }
+            }
+            if (ACC) {
+              const double s0 = bq[i];
+#pragma unroll
+              for (int m = 0; m < NP2; ++m)
+                if (jp[m] >= 0) {
+                  acc[2 * m] += -x[r][m].x * s0;
+                  acc[2 * m + 1] += -x[r][m].y * s0;
+                }
+            }
+          }
+        }
+      }
+      if (ACC) {
+#pragma unroll
+        for (int m = 0; m < NP2; ++m)
+          if (jp[m] >= 0) {
+            out[q * n + jp[m]] = acc[2 * m];
+            out[q * n + jp[m] + 1] = acc[2 * m + 1];
+          }
+      }
+    }
+  }
+}
+
+// Optional phase clocks (MVMC_PHASE_CLOCKS builds): lane 0 charges the time
+// since the previous mark to phase k of its slot, after the barrier that
+// ends the phase.
+#ifdef MVMC_PHASE_CLOCKS
+#define PHASE_INIT                   \
+  long long clk_t0 = wall_clock64(); \
+  long long* clk = v.clk + (size_t)blockIdx.x * 16;
+#define PHASE(k)                           \
+  do {                                     \
+    if (threadIdx.x == 0) {                \
+      const long long t_ = wall_clock64(); \
+      clk[k] += t_ - clk_t0;               \
+      clk_t0 = t_;                         \
+    }                                      \
+  } while (0)
+#else
+#define PHASE_INIT
+#define PHASE(k) \
+  do {           \
+  } while (0)
+#endif
+
 enum { M_TYPE = 0, M_REJ, M_MI, M_S, M_RI, M_RJ, M_MJ, M_T, M_ACC, M_NACC, M_NOUT, M_SKIP };
 
-__global__ void __launch_bounds__(256) k_block(EnsView v, int out_step, int n_in) {
+__global__ void __launch_bounds__(256, 2) k_block(EnsView v, int out_step, int n_in) {
   extern __shared__ __align__(16) char lds[];
   SampLds L = carve_sampler(v, lds);
   const int tid = threadIdx.x, nt = blockDim.x;
   const int n = v.n, ne = v.ne, ns = v.ns, n2 = v.n2, nqp = v.nqp, nn = n * n;
-  double* work = v.work + (size_t)blockIdx.x * nqp * nn;
+  double* work = v.work + (size_t)blockIdx.x * pfinv_work_doubles(n, nqp);
+  double* pv1 = v.pend + (size_t)blockIdx.x * 2 * nqp * n;  // pending hop update: vec1
+  double* pv2 = pv1 + (size_t)nqp * n;                      //                     vec2
+  PHASE_INIT
   for (int w = blockIdx.x; w < v.W; w += gridDim.x) {
     int* ws = v.wstate + (size_t)w * 8;
+    int pm = -1;  // row of the pending update (-1: none); uniform over the workgroup
     if (tid == 0) {
       L.misc[M_NOUT] = ws[2];
       L.misc[M_NACC] = ws[1];
@@ -574,6 +712,7 @@ __global__ void __launch_bounds__(256) k_block(EnsView v, int out_step, int n_in
         L.misc[M_MJ] = mj; L.misc[M_T] = t;
       }
       __syncthreads();
+      PHASE(0);
       const int rej = L.misc[M_REJ], type = L.misc[M_TYPE];
       if (rej == 1) {
         // `continue` in the reference skips the recompute guard too; the
@@ -600,6 +739,7 @@ __global__ void __launch_bounds__(256) k_block(EnsView v, int out_step, int n_in
           __syncthreads();
         }
         const double lpd = proj_ratio_ordered(v, L);
+        PHASE(1);
         // ---- Pfaffian ratios ----
         double* stage = L.uni;            // [nqp][n]
         double* slt = L.uni + nqp * n;    // [nqp][n]
@@ -607,13 +747,36 @@ __global__ void __launch_bounds__(256) k_block(EnsView v, int out_step, int n_in
         if (type == 0) {
           const int msa = mi + s * ne;
           const int rsa = L.ele_idx[msa] + s * ns;
-          for (int e = tid; e < nqp * n; e += nt) {
-            const int q = e / n, j = e % n;
-            stage[e] = planes[(size_t)q * nn + (size_t)msa * n + j];
-            const int rsj = L.ele_idx[j] + (j < ne ? 0 : ns);
-            slt[e] = v.slater[((size_t)q * n2 + rsa) * n2 + rsj];
+          // row msa of the (pending-updated) inverse and the Slater row,
+          // loads of STAGE_U elements per lane issued together
+          constexpr int STAGE_U = 4;
+          for (int e0 = 0; e0 < nqp * n; e0 += nt * STAGE_U) {
+            double xv[STAGE_U], sv[STAGE_U], a1[STAGE_U], a2[STAGE_U];
+#pragma unroll
+            for (int u = 0; u < STAGE_U; ++u) {
+              const int e = e0 + tid + u * nt;
+              if (e < nqp * n) {
+                const int q = e / n, j = e % n;
+                xv[u] = planes[(size_t)q * nn + (size_t)msa * n + j];
+                const int rsj = L.ele_idx[j] + (j < ne ? 0 : ns);
+                sv[u] = v.slater[((size_t)q * n2 + rsa) * n2 + rsj];
+                if (pm >= 0) { a1[u] = pv1[e]; a2[u] = pv2[e]; }
+              }
+            }
+#pragma unroll
+            for (int u = 0; u < STAGE_U; ++u) {
+              const int e = e0 + tid + u * nt;
+              if (e < nqp * n) {
+                const int q = e / n, j = e % n;
+                double x = xv[u];
+                if (pm >= 0) x = apply_pend(x, pv1[q * n + msa], pv2[q * n + msa], a1[u], a2[u], msa, j, pm);
+                stage[e] = x;
+                slt[e] = sv[u];
+              }
+            }
           }
           __syncthreads();
+          PHASE(2);
           for (int q = tid; q < nqp; q += nt) {
             double ratio = 0.0;
             const double* a = stage + q * n;
@@ -622,7 +785,13 @@ __global__ void __launch_bounds__(256) k_block(EnsView v, int out_step, int n_in
             L.pf_new[q] = -ratio * L.pf[q];
           }
         } else {
-          // two-electron ratio (updates.rs:395-441, 1539-1622)
+          // two-electron ratio (updates.rs:395-441, 1539-1622) on the
+          // fully updated inverse
+          if (pm >= 0) {
+            stream_planes<false>(planes, n, nqp, pv1, pv2, pm, nullptr, nullptr);
+            pm = -1;
+            __syncthreads();
+          }
           const int msa = mi + s * ne, msb = mj + t * ne;
           double* va = L.uni;                 // [nqp][n]
           double* vb = L.uni + nqp * n;
@@ -669,6 +838,7 @@ __global__ void __launch_bounds__(256) k_block(EnsView v, int out_step, int n_in
           }
         }
         __syncthreads();
+        PHASE(type == 0 ? 3 : 7);
         // ---- Metropolis (lane 0) ----
         if (tid == 0) {
           const double lip_new = ddm::cr_log(fabs(ip_of(v, L.pf_new)));
@@ -698,41 +868,27 @@ __global__ void __launch_bounds__(256) k_block(EnsView v, int out_step, int n_in
           }
         }
         __syncthreads();
+        PHASE(4);
         const int acc = L.misc[M_ACC];
         apply_clear_deltas(v, L, cnt, acc != 0);
         if (acc) {
           if (type == 0) {
-            // update_one_real_scalar: vec1 = -(inv^T slt), serial in i
+            // update_one_real_scalar (updates.rs:1184-1224), delayed by one
+            // accepted hop: one pass applies the pending update and forms
+            // vec1 = -(X^T slt) of this one; this update becomes pending.
             const int msa = mi + s * ne;
-            for (int e = tid; e < nqp * n; e += nt) {
-              const int q = e / n, j = e % n;
-              const double* pl = planes + (size_t)q * nn;
-              const double* b = slt + q * n;
-              double a = 0.0;
-              for (int i = 0; i < n; ++i) a += -pl[(size_t)i * n + j] * b[i];
-              vec1[e] = a;
-            }
+            stream_planes<true>(planes, n, nqp, pv1, pv2, pm, slt, vec1);
             __syncthreads();
-            // vec2 (in place of the staged old row msa)
             for (int e = tid; e < nqp * n; e += nt) {
               const int q = e / n;
               const double inv_a = -1.0 / vec1[q * n + msa];
-              stage[e] = stage[e] * inv_a;
+              pv2[e] = stage[e] * inv_a;
+              pv1[e] = vec1[e];
             }
-            __syncthreads();
-            for (size_t e = tid; e < (size_t)nqp * nn; e += nt) {
-              const int q = (int)(e / nn), r = (int)(e % nn);
-              const int i = r / n, j = r % n;
-              const double* v1 = vec1 + q * n;
-              const double* v2 = stage + q * n;
-              double x = planes[e];
-              x += v1[i] * v2[j] - v1[j] * v2[i];
-              if (j == msa) x -= v2[i];
-              if (i == msa) x += v2[j];
-              planes[e] = x;
-            }
+            pm = msa;
             for (int q = tid; q < nqp; q += nt) L.pf[q] = L.pf_new[q];
             __syncthreads();
+            PHASE(5);
           } else {
             // update_two_real (updates.rs:1429-1503); pf *= ratio recomputed here
             const int msa = mi + s * ne, msb = mj + t * ne;
@@ -818,6 +974,7 @@ __global__ void __launch_bounds__(256) k_block(EnsView v, int out_step, int n_in
       const int nacc_now = L.misc[M_NACC];
       __syncthreads();
       if (nacc_now > ns) {
+        pm = -1;  // the planes are rebuilt from the configuration
         int bad = wg_calc_m_all(v, L.ele_idx, L.rs, planes, work, L.pf_new, (char*)L.uni, false);
         if (tid == 0) {
           if (!bad) {
@@ -827,9 +984,14 @@ __global__ void __launch_bounds__(256) k_block(EnsView v, int out_step, int n_in
           L.misc[M_NACC] = 0;
         }
         __syncthreads();
+        PHASE(6);
       }
     }
-    // ---- end of block: persist state, save sample ----
+    // ---- end of block: apply the pending update, persist state, save sample ----
+    if (pm >= 0) {
+      stream_planes<false>(planes, n, nqp, pv1, pv2, pm, nullptr, nullptr);
+      pm = -1;
+    }
     store_walker(v, w, L);
     for (int q = tid; q < nqp; q += nt) v.pf[(size_t)w * nqp + q] = L.pf[q];
     if (tid == 0) {
@@ -850,6 +1012,7 @@ __global__ void __launch_bounds__(256) k_block(EnsView v, int out_step, int n_in
       }
     }
     __syncthreads();
+    PHASE(8);
   }
 }
 
@@ -881,7 +1044,7 @@ __device__ double direct_proj_ratio_moved(const EnsView& v, int source, int dest
   return ddm::cr_exp(z);
 }
 
-__global__ void __launch_bounds__(256) k_measure(EnsView v, double* gscratch, int gscratch_stride, int mode_exp) {
+__global__ void __launch_bounds__(256, 3) k_measure(EnsView v, double* gscratch, int gscratch_stride, int mode_exp) {
   extern __shared__ __align__(16) char lds[];
   Carver c{lds};
   int* ei = c.take<int>(v.n);
@@ -897,7 +1060,7 @@ __global__ void __launch_bounds__(256) k_measure(EnsView v, double* gscratch, in
   int* flags = c.take<int>(64);
   const int tid = threadIdx.x, nt = blockDim.x;
   const int n = v.n, ne = v.ne, ns = v.ns, n2 = v.n2, nqp = v.nqp, nn = n * n;
-  double* work = v.work + (size_t)blockIdx.x * nqp * nn;
+  double* work = v.work + (size_t)blockIdx.x * pfinv_work_doubles(n, nqp);
   double* qorb = v.qorb + (size_t)blockIdx.x * nqp * v.nslater;
   double* g1 = gscratch + (size_t)blockIdx.x * gscratch_stride;  // per term
   for (int w = blockIdx.x; w < v.W; w += gridDim.x) {
@@ -1200,8 +1363,14 @@ __global__ void k_reduce_ops(EnsView v, double* ho, double* mean, double* diag) 
 }
 
 // ---------------------------------------------------------------------------
-// SR-CG operator pieces (sr_cg.rs:952-1004).  Samples are packed
-// [W*nsample][na] in walker order.
+// SR-CG (sr_cg.rs:700-880, 952-1004).  Samples are packed [rows][na],
+// rows = W*nsample in walker order.  Reduction order (the reference's own
+// products go through BLAS dgemv, whose order is the BLAS build's):
+//   bdot: 256 strided lane partials (ascending index), then the pairwise
+//         tree p_l += p_{l+h}, h = 128, 64, ..., 1;
+//   S x:  y_r = bdot(O_r, x); rows cut into chunks of CG_RCH, per chunk
+//         c_i = sum_r O_ri y_r (ascending r); chunks folded left within
+//         superchunks of CG_SCH chunks, superchunks folded left.
 // ---------------------------------------------------------------------------
 __global__ void k_cg_pack(EnsView v, const int* map, int na, double* pack) {
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1212,86 +1381,239 @@ __global__ void k_cg_pack(EnsView v, const int* map, int na, double* pack) {
   pack[e] = v.o_store[ws * v.size + map[si] + 1];
 }
 
-// y[ws] = sum_si pack[ws][si] x[si] (serial in si)
-__global__ void k_cg_y(const double* pack, int nrows, int na, const double* x, double* y) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrows) return;
-  const double* row = pack + (size_t)r * na;
-  double acc = 0.0;
-  for (int i = 0; i < na; ++i) acc += row[i] * x[i];
-  y[r] = acc;
+enum { CTL_DONE = 0, CTL_STATUS_LO = 1, CTL_STATUS_HI = 2 };
+
+__device__ inline void cg_finish(int* ctl, long long status) {
+  ctl[CTL_STATUS_LO] = (int)(status & 0xffffffffLL);
+  ctl[CTL_STATUS_HI] = (int)(status >> 32);
+  ctl[CTL_DONE] = 1;
 }
 
-// z[si] = fold_w (sum_s pack[w,s][si] y[w,s]); then the mean/shift correction
-__global__ void k_cg_z(const double* pack, int W, int nsample, int na, const double* y, const double* x,
-                       const double* mean, const double* diag, const double* coef, double inv_w, double shift,
-                       double* z) {
+// Pairwise tree over the 256 lane partials of a 256-thread block; result
+// returned to every lane.  `red` is a 256-double LDS array.
+__device__ double block_tree256(double p, double* red) {
+  const int tid = threadIdx.x;
+  red[tid] = p;
+  __syncthreads();
+  if (tid < 128) red[tid] += red[tid + 128];
+  __syncthreads();
+  if (tid < 64) {
+    double v = red[tid] + red[tid + 64];
+    for (int h = 32; h >= 1; h >>= 1) v += __shfl_down(v, h, 64);
+    if (tid == 0) red[0] = v;
+  }
+  __syncthreads();
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ inline double block_bdot(const double* a, const double* b, int n, double* red) {
+  double p = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) p += a[i] * b[i];
+  return block_tree256(p, red);
+}
+
+// Chunk partials cpart[b][i] = sum_{r in chunk b} O_ri y_r with y_r formed
+// from the same registers; block nch computes coef = bdot(mean, x).
+// C = columns per lane (na <= 256*C).
+template <int C>
+__global__ void __launch_bounds__(256) k_cg_chunk(const double* pack, int rows, int na, const double* x,
+                                                  const double* mean, double* cpart, double* coef, const int* ctl) {
+  if (ctl[CTL_DONE]) return;
+  __shared__ double red[256];
+  const int tid = threadIdx.x;
+  const int nch = (rows + CG_RCH - 1) / CG_RCH;
+  const int b = blockIdx.x;
+  if (b == nch) {
+    const double c = block_bdot(mean, x, na, red);
+    if (tid == 0) *coef = c;
+    return;
+  }
+  double xv[C], acc[C];
+#pragma unroll
+  for (int k = 0; k < C; ++k) {
+    const int i = tid + 256 * k;
+    xv[k] = i < na ? x[i] : 0.0;
+    acc[k] = 0.0;
+  }
+  const int r1 = min(rows, (b + 1) * CG_RCH);
+  for (int r = b * CG_RCH; r < r1; ++r) {
+    const double* row = pack + (size_t)r * na;
+    double o[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+      const int i = tid + 256 * k;
+      o[k] = i < na ? row[i] : 0.0;
+    }
+    double p = 0.0;
+#pragma unroll
+    for (int k = 0; k < C; ++k)
+      if (tid + 256 * k < na) p += o[k] * xv[k];
+    const double y = block_tree256(p, red);
+#pragma unroll
+    for (int k = 0; k < C; ++k)
+      if (tid + 256 * k < na) acc[k] += o[k] * y;
+  }
+#pragma unroll
+  for (int k = 0; k < C; ++k) {
+    const int i = tid + 256 * k;
+    if (i < na) cpart[(size_t)b * na + i] = acc[k];
+  }
+}
+
+// Same order for na > 256*CG_CMAX: the chunk accumulators live in cpart.
+__global__ void __launch_bounds__(256) k_cg_chunk_wide(const double* pack, int rows, int na, const double* x,
+                                                       const double* mean, double* cpart, double* coef,
+                                                       const int* ctl) {
+  if (ctl[CTL_DONE]) return;
+  __shared__ double red[256];
+  const int tid = threadIdx.x;
+  const int nch = (rows + CG_RCH - 1) / CG_RCH;
+  const int b = blockIdx.x;
+  if (b == nch) {
+    const double c = block_bdot(mean, x, na, red);
+    if (tid == 0) *coef = c;
+    return;
+  }
+  double* cp = cpart + (size_t)b * na;
+  const int r0 = b * CG_RCH, r1 = min(rows, (b + 1) * CG_RCH);
+  for (int r = r0; r < r1; ++r) {
+    const double* row = pack + (size_t)r * na;
+    const double y = block_bdot(row, x, na, red);
+    for (int i = tid; i < na; i += 256) {
+      double a = (r == r0) ? 0.0 : cp[i];
+      a += row[i] * y;
+      cp[i] = a;
+    }
+  }
+}
+
+// Superchunk folds: spart[s][i] = c_{s*SCH} + ... (left fold)
+__global__ void k_cg_fold1(const double* cpart, int nch, int na, double* spart, const int* ctl) {
+  if (ctl[CTL_DONE]) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.y;
+  if (i >= na) return;
+  const int b0 = s * CG_SCH, b1 = min(nch, b0 + CG_SCH);
+  double a = cpart[(size_t)b0 * na + i];
+  for (int b = b0 + 1; b < b1; ++b) a += cpart[(size_t)b * na + i];
+  spart[(size_t)s * na + i] = a;
+}
+
+// z = inv_w * fold_s(spart) - coef mean + shift diag x; resid: out = g - z.
+__global__ void k_cg_fold2(const double* spart, int nsup, int na, const double* x, const double* mean,
+                           const double* diag, const double* coef, double inv_w, double shift, const double* g,
+                           double* out, const int* ctl) {
+  if (ctl[CTL_DONE]) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= na) return;
-  double tot = 0.0;
-  for (int w = 0; w < W; ++w) {
-    double acc = 0.0;
-    for (int s = 0; s < nsample; ++s) {
-      const size_t r = (size_t)w * nsample + s;
-      acc += pack[r * na + i] * y[r];
+  double tot = nsup > 0 ? spart[i] : 0.0;
+  for (int s = 1; s < nsup; ++s) tot += spart[(size_t)s * na + i];
+  const double z = inv_w * tot - coef[0] * mean[i] + shift * diag[i] * x[i];
+  out[i] = g ? g[i] - z : z;
+}
+
+__device__ inline double cg_precond(double pfac, double dg, double r) {
+  const double EPS = 1.0e-12;
+  double dq = pfac * dg;
+  if (!(dq - dq == 0.0) || fabs(dq) < EPS) dq = dq < 0.0 ? -EPS : EPS;
+  return r / dq;
+}
+
+// sc: 0 delta, 2 alpha, 4 beta, 5 coef.  ctl: done flag + 64-bit status.
+// Start of the initial residual: diag-scale d = precond(r), delta = r.d;
+// plain: delta = r.r.  Non-finite delta ends the solve with status -1.
+__global__ void __launch_bounds__(256) k_cg_init(int na, const double* r, double* d, const double* diag, double pfac,
+                                                 int diag_scale, double* sc, int* ctl) {
+  __shared__ double red[256];
+  double p = 0.0;
+  for (int i = threadIdx.x; i < na; i += 256) {
+    if (diag_scale) {
+      const double q = cg_precond(pfac, diag[i], r[i]);
+      d[i] = q;
+      p += r[i] * q;
+    } else {
+      p += r[i] * r[i];
     }
-    tot = (w == 0) ? acc : tot + acc;
   }
-  z[i] = inv_w * tot - coef[0] * mean[i] + shift * diag[i] * x[i];
+  const double delta = block_tree256(p, red);
+  if (threadIdx.x == 0) {
+    sc[0] = delta;
+    ctl[CTL_DONE] = 0;
+    if (!(delta - delta == 0.0)) cg_finish(ctl, -1);
+  }
 }
 
-// serial dot product (sr_cg.rs:492-499) by one lane
-__global__ void k_seq_dot(const double* a, const double* b, int n, double* out) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  double s = 0.0;
-  for (int i = 0; i < n; ++i) s += a[i] * b[i];
-  *out = s;
+// Head of iteration `iter` (the loop condition of the C solver).
+__global__ void k_cg_head(int iter, int max_it, double threshold, const double* sc, int* ctl) {
+  if (threadIdx.x || blockIdx.x || ctl[CTL_DONE]) return;
+  const double delta = sc[0];
+  if (iter >= max_it) {
+    cg_finish(ctl, fabs(delta) >= threshold ? -(long long)(max_it + 1) : (long long)iter);
+    return;
+  }
+  if (!(delta - delta == 0.0)) cg_finish(ctl, -(long long)(iter + 1));
+  else if (fabs(delta) < threshold) cg_finish(ctl, iter);
 }
 
-// x += alpha d; r = r - alpha q   (or only x)
-__global__ void k_cg_axpy(int n, const double* alpha, const double* d, double* x, const double* q, double* r) {
+// alpha = delta / (d.q) with the C guards
+__global__ void __launch_bounds__(256) k_cg_alpha(int iter, int na, const double* d, const double* q, double* sc,
+                                                  int* ctl) {
+  if (ctl[CTL_DONE]) return;
+  __shared__ double red[256];
+  const double dq = block_bdot(d, q, na, red);
+  if (threadIdx.x == 0) {
+    if (!(dq - dq == 0.0) || fabs(dq) < 1.0e-20) { cg_finish(ctl, -(long long)(iter + 1)); return; }
+    const double a = sc[0] / dq;
+    if (!(a - a == 0.0)) { cg_finish(ctl, -(long long)(iter + 1)); return; }
+    sc[2] = a;
+  }
+}
+
+// x += alpha d; r -= alpha q (not on refresh iterations)
+__global__ void k_cg_update(int na, const double* sc, const int* ctl, const double* d, double* x, const double* q,
+                            double* r, int refresh) {
+  if (ctl[CTL_DONE]) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double a = *alpha;
+  if (i >= na) return;
+  const double a = sc[2];
   x[i] += a * d[i];
-  if (r) r[i] -= a * q[i];
+  if (!refresh) r[i] -= a * q[i];
 }
 
-// r = g - z
-__global__ void k_cg_resid(int n, const double* g, const double* z, double* r) {
+// dnew (diag-scale: q = precond(r), dnew = r.q; plain: r.r), beta, delta
+__global__ void __launch_bounds__(256) k_cg_beta(int iter, int na, const double* r, double* q, const double* diag,
+                                                 double pfac, int diag_scale, double* sc, int* ctl) {
+  if (ctl[CTL_DONE]) return;
+  __shared__ double red[256];
+  double p = 0.0;
+  for (int i = threadIdx.x; i < na; i += 256) {
+    if (diag_scale) {
+      const double qq = cg_precond(pfac, diag[i], r[i]);
+      q[i] = qq;
+      p += r[i] * qq;
+    } else {
+      p += r[i] * r[i];
+    }
+  }
+  const double dn = block_tree256(p, red);
+  if (threadIdx.x == 0) {
+    const double dl = sc[0];
+    if (!(dn - dn == 0.0) || fabs(dl) < 1.0e-20) { cg_finish(ctl, -(long long)(iter + 1)); return; }
+    const double b = dn / dl;
+    if (!(b - b == 0.0)) { cg_finish(ctl, -(long long)(iter + 1)); return; }
+    sc[4] = b;
+    sc[0] = diag_scale ? dn : dl * b;
+  }
+}
+
+// d = src + beta d
+__global__ void k_cg_dir(int na, const double* sc, const int* ctl, const double* src, double* d) {
+  if (ctl[CTL_DONE]) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) r[i] = g[i] - z[i];
-}
-
-// d = r + beta d
-__global__ void k_cg_dir(int n, const double* beta, const double* r, double* d) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) d[i] = r[i] + *beta * d[i];
-}
-
-// scalar step: alpha = delta/dq, with the C 1.4.0 breakdown guards
-__global__ void k_cg_alpha(const double* delta, const double* dq, double* alpha, int* flag) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const double CG_TINY = 1.0e-20;
-  const double q = *dq;
-  if (!(q - q == 0.0) || fabs(q) < CG_TINY) { *flag = 1; return; }
-  const double a = *delta / q;
-  if (!(a - a == 0.0)) { *flag = 1; return; }
-  *alpha = a;
-  *flag = 0;
-}
-
-// beta = dnew/delta; delta *= beta (plain CG), guards
-__global__ void k_cg_beta(double* delta, const double* dnew, double* beta, int* flag) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const double CG_TINY = 1.0e-20;
-  const double dn = *dnew, dl = *delta;
-  if (!(dn - dn == 0.0) || fabs(dl) < CG_TINY) { *flag = 1; return; }
-  const double b = dn / dl;
-  if (!(b - b == 0.0)) { *flag = 1; return; }
-  *beta = b;
-  *delta = dl * b;
-  *flag = 0;
+  if (i >= na) return;
+  d[i] = src[i] + sc[4] * d[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -1306,14 +1628,9 @@ __global__ void __launch_bounds__(256) k_pfinv_batched(double* planes, double* w
   if (cnt <= 0) return;
   Carver c{lds};
   PfinvShared sh;
-  sh.ld = n;
-  sh.piv = c.take<int>((size_t)np * n);
-  sh.colbuf = c.take<double>((size_t)np * n);
-  sh.kp = c.take<int>(np);
-  sh.info = c.take<int>(np);
-  sh.colmax = c.take<double>(np);
+  carve_pfinv(c, sh, n, np);
   double* pl = planes + (size_t)first * n * n;
-  double* wk = work + (size_t)blockIdx.x * np * n * n;
+  double* wk = work + (size_t)blockIdx.x * pfinv_work_doubles(n, np);
   wg_pfinv(pl, wk, n, cnt, sh, pf + first, 1.0);
   for (int p = threadIdx.x; p < cnt; p += blockDim.x) {
     const int info = sh.info[p];

@@ -16,7 +16,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "libmvmc_hip.so")
+LIB_PATH = os.environ.get("MVMC_HIP_LIB") or os.path.join(HERE, "_build", "libmvmc_hip.so")
 
 _lib = None
 
@@ -60,7 +60,7 @@ EXPORTED_SYMBOLS = [
     "mvmc_ens_destroy", "mvmc_ens_dims", "mvmc_ens_get_params", "mvmc_ens_set_params", "mvmc_ens_sample",
     "mvmc_ens_measure", "mvmc_ens_sr_cg", "mvmc_ens_opt_step", "mvmc_ens_get_energy", "mvmc_ens_get_operands",
     "mvmc_ens_get_o_store", "mvmc_ens_get_walker", "mvmc_ens_get_tables", "mvmc_ens_get_config", "mvmc_ens_get_cg",
-    "mvmc_ens_timings", "mvmc_ens_status", "mvmc_sr_cg_solve",
+    "mvmc_ens_timings", "mvmc_ens_phase_clocks", "mvmc_ens_status", "mvmc_sr_cg_solve",
 ]
 
 
@@ -180,6 +180,17 @@ class Ensemble:
         lib().mvmc_ens_timings(self.h, _p(t))
         return {"sample_s": t[0], "measure_s": t[1], "sr_s": t[2], "proposals": t[3], "accepted": t[4],
                 "block_ms": t[5], "block_launches": t[6]}
+
+    PHASES = ("candidate", "projection", "staging", "ratio", "metropolis", "accept_pass", "rebuild", "exchange",
+              "block_end")
+
+    def phase_clocks(self):
+        """Per-phase device seconds of the sampling kernel (summed over walkers);
+        None unless the library was built with MVMC_PHASE_CLOCKS."""
+        c = np.zeros(16, dtype=np.int64)
+        if lib().mvmc_ens_phase_clocks(self.h, _p(c)) != 0:
+            return None
+        return {k: c[i] / 1e8 for i, k in enumerate(self.PHASES)}
 
     def status(self):
         return int(lib().mvmc_ens_status(self.h))

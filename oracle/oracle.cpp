@@ -826,44 +826,65 @@ static void measure_walker(const Run& R, Walker& W) {
 // ---------------------------------------------------------------------------
 struct CgOperator {
   int n = 0, samples = 0;
-  vector<double> mean, diag, O;  // O: [samples][n] (sample-major, column-major [n, samples])
-  vector<const double*> parts;   // per-walker sample blocks (rank order)
+  vector<double> mean, diag;
+  vector<const double*> parts;   // per-walker sample blocks [samples][n], walker (= rank) order
   vector<int> part_samples;
 };
 
-static double seq_dot(const double* p, const double* q, int n) {
-  double s = 0.0;
-  for (int i = 0; i < n; ++i) s += p[i] * q[i];
-  return s;
+// Reduction order of the SR-CG solver.  The reference evaluates S x with
+// BLAS dgemv (sr_backend.rs:520-597), whose summation order belongs to the
+// BLAS build, and reduces over ranks with MPI; this order is therefore a
+// choice, fixed here and reproduced by the device kernels:
+//   bdot(a, b): 256 strided partial sums p_l = sum_{i = l mod 256} a_i b_i
+//               (ascending i), then the pairwise tree p_l += p_{l+h},
+//               h = 128, 64, ..., 1; the result is p_0.
+//   S x:        y_r = bdot(O_r, x) per sample row r; rows are cut into chunks
+//               of CG_RCH consecutive rows (walker order); per chunk
+//               c_i = sum_r O_ri y_r (ascending r); chunks are folded left
+//               within superchunks of CG_SCH chunks, superchunk results folded
+//               left in order.
+constexpr int BD_LANES = 256;
+constexpr int CG_RCH = 32;
+constexpr int CG_SCH = 16;
+
+static double bdot(const double* a, const double* b, int n) {
+  double p[BD_LANES];
+  for (int l = 0; l < BD_LANES; ++l) {
+    double acc = 0.0;
+    for (int i = l; i < n; i += BD_LANES) acc += a[i] * b[i];
+    p[l] = acc;
+  }
+  for (int h = BD_LANES / 2; h >= 1; h >>= 1)
+    for (int l = 0; l < h; ++l) p[l] += p[l + h];
+  return p[0];
 }
 
-// z = Oᵀ-free product: per walker y = Oᵀx then z_w = O y, summed in walker
-// order; then z = inv_w z - (mean.x) mean + shift diag x (sr_cg.rs:952-1004).
+// z = inv_w (O^T O x) - (mean.x) mean + shift diag x (sr_cg.rs:952-1004).
 static void cg_apply(const CgOperator& op, const double* x, double* z, double inv_w, double shift) {
   const int n = op.n;
-  vector<double> zl(n), y;
-  for (int i = 0; i < n; ++i) z[i] = 0.0;
-  for (size_t part = 0; part < op.parts.size(); ++part) {
-    const double* O = op.parts[part];
-    const int S = op.part_samples[part];
-    y.assign(S, 0.0);
-    for (int s = 0; s < S; ++s) {
-      double acc = 0.0;
-      for (int i = 0; i < n; ++i) acc += O[(size_t)s * n + i] * x[i];
-      y[s] = acc;
+  vector<const double*> rows;
+  for (size_t part = 0; part < op.parts.size(); ++part)
+    for (int s = 0; s < op.part_samples[part]; ++s) rows.push_back(op.parts[part] + (size_t)s * n);
+  const int nrows = (int)rows.size();
+  const int nch = (nrows + CG_RCH - 1) / CG_RCH;
+  vector<double> tot(n, 0.0), sup(n, 0.0), c(n);
+  for (int b = 0; b < nch; ++b) {
+    std::fill(c.begin(), c.end(), 0.0);
+    for (int r = b * CG_RCH; r < std::min(nrows, (b + 1) * CG_RCH); ++r) {
+      const double y = bdot(rows[r], x, n);
+      for (int i = 0; i < n; ++i) c[i] += rows[r][i] * y;
     }
-    for (int i = 0; i < n; ++i) {
-      double acc = 0.0;
-      for (int s = 0; s < S; ++s) acc += O[(size_t)s * n + i] * y[s];
-      zl[i] = acc;
-    }
-    if (part == 0)
-      for (int i = 0; i < n; ++i) z[i] = zl[i];
+    if (b % CG_SCH == 0) sup = c;
     else
-      for (int i = 0; i < n; ++i) z[i] += zl[i];
+      for (int i = 0; i < n; ++i) sup[i] += c[i];
+    if (b % CG_SCH == CG_SCH - 1 || b == nch - 1) {
+      if (b < CG_SCH) tot = sup;
+      else
+        for (int i = 0; i < n; ++i) tot[i] += sup[i];
+    }
   }
-  const double coef = seq_dot(op.mean.data(), x, n);
-  for (int i = 0; i < n; ++i) z[i] = inv_w * z[i] - coef * op.mean[i] + shift * op.diag[i] * x[i];
+  const double coef = bdot(op.mean.data(), x, n);
+  for (int i = 0; i < n; ++i) z[i] = inv_w * tot[i] - coef * op.mean[i] + shift * op.diag[i] * x[i];
 }
 
 static long long cg_solve(const CgOperator& op, const double* g, double inv_w, double shift, double tol,
@@ -886,10 +907,10 @@ static long long cg_solve(const CgOperator& op, const double* g, double inv_w, d
     for (int i = 0; i < n; ++i) r[i] = g[i] - r[i];
     d.assign(n, 0.0);
     for (int i = 0; i < n; ++i) d[i] = precond(i, r[i]);
-    delta = seq_dot(r.data(), d.data(), n);
+    delta = bdot(r.data(), d.data(), n);
   } else {
     d.assign(g, g + n);
-    delta = seq_dot(r.data(), r.data(), n);
+    delta = bdot(r.data(), r.data(), n);
   }
   auto finish = [&](long long st) {
     if (r_out) std::copy(r.begin(), r.end(), r_out);
@@ -903,7 +924,7 @@ static long long cg_solve(const CgOperator& op, const double* g, double inv_w, d
     if (!std::isfinite(delta)) return finish(failed);
     if (std::fabs(delta) < threshold) break;
     cg_apply(op, d.data(), q.data(), inv_w, shift);
-    const double dq = seq_dot(d.data(), q.data(), n);
+    const double dq = bdot(d.data(), q.data(), n);
     if (!std::isfinite(dq) || std::fabs(dq) < CG_TINY) return finish(failed);
     const double alpha = delta / dq;
     if (!std::isfinite(alpha)) return finish(failed);
@@ -917,9 +938,9 @@ static long long cg_solve(const CgOperator& op, const double* g, double inv_w, d
     double dnew;
     if (diag_scale) {
       for (int i = 0; i < n; ++i) q[i] = precond(i, r[i]);
-      dnew = seq_dot(r.data(), q.data(), n);
+      dnew = bdot(r.data(), q.data(), n);
     } else {
-      dnew = seq_dot(r.data(), r.data(), n);
+      dnew = bdot(r.data(), r.data(), n);
     }
     if (!std::isfinite(dnew) || std::fabs(delta) < CG_TINY) return finish(failed);
     const double beta = dnew / delta;
@@ -1112,6 +1133,18 @@ extern "C" {
 void oracle_set_math(int cr) { g_cr_math = cr; }
 double oracle_cr_exp(double x) { return ddm::cr_exp(x); }
 double oracle_cr_log(double x) { return ddm::cr_log(x); }
+// Bulk evaluation: which = 0 exp (fast path + fallback), 1 exp accurate path
+// only, 2 log, 3 log accurate path only.
+void oracle_cr_eval(int which, const double* x, double* y, long long n) {
+  for (long long i = 0; i < n; ++i) {
+    switch (which) {
+      case 0: y[i] = ddm::cr_exp(x[i]); break;
+      case 1: y[i] = ddm::exp_accurate(x[i]); break;
+      case 2: y[i] = ddm::cr_log(x[i]); break;
+      default: y[i] = ddm::log_accurate(x[i]);
+    }
+  }
+}
 
 void oracle_sfmt_words(uint32_t seed, int count, uint32_t* out) {
   Sfmt r;

@@ -35,6 +35,8 @@ def lib():
         L.oracle_cr_exp.argtypes = [ctypes.c_double]
         L.oracle_cr_log.restype = ctypes.c_double
         L.oracle_cr_log.argtypes = [ctypes.c_double]
+        L.oracle_cr_eval.restype = None
+        L.oracle_cr_eval.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong]
         L.oracle_two_hop_bilinear.restype = ctypes.c_double
         L.oracle_cg.restype = ctypes.c_longlong
         L.oracle_cg.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 4 + \
@@ -202,3 +204,11 @@ def cg_apply(mean, diag, O, x, inv_w, shift):
     z = np.zeros(n)
     lib().oracle_cg_apply(n, O.size // n, ptr(mean), ptr(diag), ptr(O), ptr(x), ptr(z), inv_w, shift)
     return z
+
+
+def cr_eval(which, x):
+    """which: 0 exp, 1 exp (accurate path only), 2 log, 3 log (accurate path only)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    lib().oracle_cr_eval(which, x.ctypes.data, y.ctypes.data, x.size)
+    return y

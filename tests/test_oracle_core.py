@@ -154,10 +154,23 @@ def test_correctly_rounded_exp_log_against_high_precision():
         assert abs(Decimal(got) - exact) <= abs(Decimal(lo) - exact) and abs(Decimal(got) - exact) <= abs(Decimal(hi) - exact)
 
 
+def test_fast_paths_equal_accurate_paths():
+    """The table-driven fast paths return the accurate double-double results
+    on 2 million inputs (exp over the whole normal range and near 0, log over
+    ~600 binades and near 1)."""
+    rng = np.random.default_rng(17)
+    xe = np.concatenate([rng.uniform(-745.0, 709.7, 400000), rng.uniform(-2, 2, 300000),
+                         rng.uniform(-1e-3, 1e-3, 300000)])
+    assert np.array_equal(O.cr_eval(0, xe), O.cr_eval(1, xe))
+    xl = np.concatenate([np.exp(rng.uniform(-700, 700, 500000)), 1.0 + rng.uniform(-1e-4, 1e-4, 250000),
+                         rng.uniform(0.5, 2.0, 250000)])
+    assert np.array_equal(O.cr_eval(2, xl), O.cr_eval(3, xl))
+
+
 def test_correctly_rounded_math_agrees_with_libm_nearly_always():
     """glibc exp/log are not correctly rounded in every case; the device uses
-    the correctly rounded values.  Bound the disagreement rate (1 ulp) that
-    drives the decision-flip probability stated in DESIGN.md."""
+    the correctly rounded values.  Bound the disagreement rate (1 ulp), which
+    bounds how often a Metropolis decision can differ from a glibc build."""
     rng = np.random.default_rng(5)
     L = O.lib()
     xs = rng.uniform(-40, 5, 200000)
