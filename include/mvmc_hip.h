@@ -92,8 +92,8 @@ int mvmc_ens_get_tables(mvmc_ens* e, int walker, double* inv, double* pf);
 int mvmc_ens_get_config(mvmc_ens* e, int walker, int* ele_idx, int* ele_cfg, int* ele_num, int* proj_cnt);
 int mvmc_ens_get_cg(mvmc_ens* e, double* solution, int* n_active);
 /* timings[8]: sampling s, measurement s, SR s (wall, summed over calls),
- * proposals, accepted, hop-kernel device ms of the last k_block launch set,
- * k_block launches, reserved. */
+ * proposals, accepted, device ms of the last sampling call's k_block
+ * launches, their number, device ms of its periodic-rebuild launches. */
 int mvmc_ens_timings(mvmc_ens* e, double* timings8);
 /* Device phase clocks of the sampling kernel, summed over walkers (ticks of
  * the 100 MHz constant clock): 0 candidate, 1 projection deltas + ratio,

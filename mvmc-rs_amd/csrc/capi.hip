@@ -210,7 +210,9 @@ struct mvmc_ens {
   EnsView v{};
   std::vector<void*> allocs;
   hipStream_t st = nullptr;
-  int grid_samp = 0, grid_meas = 0, slots = 0;
+  int grid_samp = 0, grid_meas = 0, grid_rebuild = 0, slots = 0;
+  size_t lds_rebuild = 0;
+  double rebuild_ms = 0, rebuild_launches = 0;
   size_t lds_samp = 0, lds_meas = 0;
   double* d_slater = nullptr;
   double* d_proj = nullptr;
@@ -421,17 +423,35 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
   HIPCHK(hipFuncSetAttribute((const void*)k_block, hipFuncAttributeMaxDynamicSharedMemorySize, (int)E->lds_samp));
   HIPCHK(hipFuncSetAttribute((const void*)k_begin, hipFuncAttributeMaxDynamicSharedMemorySize, (int)E->lds_samp));
   HIPCHK(hipFuncSetAttribute((const void*)k_measure, hipFuncAttributeMaxDynamicSharedMemorySize, (int)E->lds_meas));
-  int occ_s = 0, occ_m = 0;
+  // rebuild kernels: as many planes as fit in LDS next to their scratch
+  // (two workgroups per CU), else the eliminations run in place in HBM
+  {
+    const size_t budget = 72 * 1024, base = lds_bytes_rebuild_base(v);
+    const size_t per = (size_t)v.n * v.n * sizeof(double);
+    // only when every plane fits: the planes then stay in lock step (one
+    // barrier per step for all of them); chunks would serialise the steps
+    v.rb_np = (budget > base && (budget - base) / per >= (size_t)v.nqp) ? v.nqp : 0;
+  }
+  E->lds_rebuild = lds_bytes_rebuild(v);
+  HIPCHK(hipFuncSetAttribute((const void*)k_rebuild, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)E->lds_rebuild));
+  HIPCHK(hipFuncSetAttribute((const void*)k_meas_rebuild, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)E->lds_rebuild));
+  int occ_s = 0, occ_m = 0, occ_r = 0;
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, k_block, 256, E->lds_samp));
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_m, k_measure, 256, E->lds_meas));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r, k_rebuild, 256, E->lds_rebuild));
   occ_s = std::max(1, occ_s);
   occ_m = std::max(1, occ_m);
+  occ_r = std::max(1, occ_r);
   E->grid_samp = (int)std::min<long long>(walkers, (long long)occ_s * ncu);
   E->grid_meas = (int)std::min<long long>(walkers, (long long)occ_m * ncu);
-  E->slots = std::max(E->grid_samp, E->grid_meas);
+  E->grid_rebuild = (int)std::min<long long>(walkers, (long long)occ_r * ncu);
+  E->slots = std::max(std::max(E->grid_samp, E->grid_meas), E->grid_rebuild);
   v.work = dalloc<double>(A, (size_t)E->slots * pfinv_work_doubles(v.n, v.nqp));
   v.qorb = dalloc<double>(A, (size_t)E->slots * v.nqp * std::max(v.nslater, 1));
   v.pend = dalloc<double>(A, (size_t)E->slots * 2 * v.nqp * v.n);
+  v.nflag = dalloc<int>(A, 1);
 #ifdef MVMC_PHASE_CLOCKS
   v.clk = dalloc<long long>(A, (size_t)E->slots * 16);
   HIPCHK(hipMemset(v.clk, 0, sizeof(long long) * E->slots * 16));
@@ -467,22 +487,48 @@ void sample_impl(mvmc_ens* E) {
   int n_out = 0;
   for (int w = 0; w < v.W; ++w) n_out = std::max(n_out, ws[(size_t)w * 8 + 2]);
   const int n_in = v.interval * std::max(v.ns, 1);
-  hipEvent_t e0, e1;
+  // Per outer step: k_block runs every walker until the step's end or its
+  // next periodic rebuild; k_rebuild rebuilds the walkers handed over; repeat
+  // until no walker is waiting.  Each k_block launch is timed with HIP
+  // events on the library's stream.
+  hipEvent_t e0, e1, r0, r1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  HIPCHK(hipEventRecord(e0, E->st));
+  HIPCHK(hipEventCreate(&r0));
+  HIPCHK(hipEventCreate(&r1));
+  double block_ms = 0, rebuild_ms = 0;
+  int launches = 0, rebuilds = 0;
   for (int out = 0; out < n_out; ++out) {
-    hipLaunchKernelGGL(k_block, dim3(E->grid_samp), dim3(256), E->lds_samp, E->st, v, out, n_in);
-    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_reset_resume, dim3(nblk(v.W)), dim3(256), 0, E->st, v);
+    for (;;) {
+      HIPCHK(hipMemsetAsync(v.nflag, 0, sizeof(int), E->st));
+      HIPCHK(hipEventRecord(e0, E->st));
+      hipLaunchKernelGGL(k_block, dim3(E->grid_samp), dim3(256), E->lds_samp, E->st, v, out, n_in);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipEventRecord(e1, E->st));
+      int nf = 0;
+      HIPCHK(hipMemcpyAsync(&nf, v.nflag, sizeof(int), hipMemcpyDeviceToHost, E->st));
+      HIPCHK(hipStreamSynchronize(E->st));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      block_ms += ms;
+      ++launches;
+      if (nf == 0) break;
+      HIPCHK(hipEventRecord(r0, E->st));
+      hipLaunchKernelGGL(k_rebuild, dim3(E->grid_rebuild), dim3(256), E->lds_rebuild, E->st, v);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipEventRecord(r1, E->st));
+      HIPCHK(hipEventSynchronize(r1));
+      HIPCHK(hipEventElapsedTime(&ms, r0, r1));
+      rebuild_ms += ms;
+      ++rebuilds;
+    }
   }
-  HIPCHK(hipEventRecord(e1, E->st));
-  HIPCHK(hipEventSynchronize(e1));
-  float ms = 0;
-  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  E->block_ms = ms;
-  E->block_launches = n_out;
+  for (hipEvent_t ev : {e0, e1, r0, r1}) (void)hipEventDestroy(ev);
+  E->block_ms = block_ms;
+  E->block_launches = launches;
+  E->rebuild_ms = rebuild_ms;
+  E->rebuild_launches = rebuilds;
   E->check_status();
   std::vector<long long> c((size_t)v.W * 4);
   HIPCHK(hipMemcpy(c.data(), v.counter, sizeof(long long) * c.size(), hipMemcpyDeviceToHost));
@@ -496,7 +542,11 @@ void sample_impl(mvmc_ens* E) {
 void measure_impl(mvmc_ens* E) {
   EnsView& v = E->v;
   const double t0 = now_s();
-  hipLaunchKernelGGL(k_measure, dim3(E->grid_meas), dim3(256), E->lds_meas, E->st, v, E->d_gscr, E->gstride, 1);
+  for (int s = 0; s < v.nsample; ++s) {
+    hipLaunchKernelGGL(k_meas_rebuild, dim3(E->grid_rebuild), dim3(256), E->lds_rebuild, E->st, v, s);
+    hipLaunchKernelGGL(k_measure, dim3(E->grid_meas), dim3(256), E->lds_meas, E->st, v, E->d_gscr, E->gstride, 1,
+                       s);
+  }
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_reduce_energy, dim3(1), dim3(64), 0, E->st, v, E->d_out5);
   hipLaunchKernelGGL(k_reduce_ops, dim3(nblk(v.size)), dim3(256), 0, E->st, v, E->d_ho, E->d_mean, E->d_diag);
@@ -781,7 +831,7 @@ int mvmc_ens_phase_clocks(mvmc_ens* e, long long* out) {
 
 int mvmc_ens_timings(mvmc_ens* e, double* t) {
   t[0] = e->t_sample; t[1] = e->t_measure; t[2] = e->t_sr; t[3] = e->proposals; t[4] = e->accepted;
-  t[5] = e->block_ms; t[6] = e->block_launches; t[7] = 0;
+  t[5] = e->block_ms; t[6] = e->block_launches; t[7] = e->rebuild_ms;
   return MVMC_OK;
 }
 

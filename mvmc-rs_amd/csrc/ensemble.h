@@ -15,6 +15,7 @@ struct EnsView {
   int ntrans, nintra, ninter, nhund, nexch;
   int orb_injective;  // each orbital-index row is injective
   int exp_cr;         // 1: correctly rounded exp/log (always on the device)
+  int rb_np;          // planes per LDS-resident chunk in the rebuild kernels (0: in place in HBM)
   // model tables
   const double* slater;   // [nqp][n2][n2]
   const double* qpw;      // [nqp]
@@ -44,7 +45,9 @@ struct EnsView {
   double* inv;        // [W][nqp][n*n]
   double* pf;         // [W][nqp]
   double* log_ip;     // [W]
-  int* wstate;        // [W][8]: 0 burn flag, 1 n_accept window, 2 n_out, 3 status, 4..7 spare
+  int* wstate;        // [W][8]: 0 burn flag, 1 n_accept window, 2 n_out, 3 status,
+                      //   4 next proposal of the outer step, 5 rebuild pending, 6 outer step done, 7 spare
+  int* nflag;         // [1] walkers handed to the rebuild kernel by the last k_block launch
   long long* counter; // [W][4]
   // saved samples
   int* s_idx;  int* s_cfg;  int* s_num;  int* s_proj;  // [W][nsample][...]

@@ -60,6 +60,7 @@ __host__ __device__ inline size_t lds_bytes_sampler(const EnsView& v, int nthrea
   b += rnd16((size_t)(4 * v.ns + 16) * 8);  // compacted terms
   b += rnd16((size_t)(nthreads + 64) * 4); // scan
   b += rnd16((size_t)v.n * 4);             // rs
+  b += 2 * rnd16((size_t)v.ns * 4) + rnd16((size_t)v.nqp * 8);  // loc_spn, gutz_idx, qpw copies
   b += rnd16(union_doubles_sampler(v) * 8);
   return b;
 }
@@ -107,8 +108,11 @@ __device__ inline int block_exscan(int v, int* scan, int* total) {
 // calc_m_all for one walker: assemble, LTL, Pfaffian, inverse, negate.
 // native = true skips the all-zero guard (C's separate real setup).
 // Returns 0 on success, 1 if any plane failed.
+// lds_pl: when non-null, room for v.rb_np planes in LDS; the eliminations
+// then run on LDS copies of the assembled planes, rb_np planes at a time
+// (each plane's operation sequence is unchanged: planes are independent).
 __device__ int wg_calc_m_all(const EnsView& v, const int* ele_idx_l, int* rs, double* planes, double* work,
-                             double* pf_l, char* pfscratch, bool native) {
+                             double* pf_l, char* pfscratch, bool native, double* lds_pl = nullptr) {
   const int tid = threadIdx.x;
   __shared__ int bad_site;
   __shared__ double wave_max[16];
@@ -154,6 +158,20 @@ __device__ int wg_calc_m_all(const EnsView& v, const int* ele_idx_l, int* rs, do
     }
     if (allzero_flag) return 1;
   }
+  if (lds_pl && v.rb_np > 0) {
+    const int nn = v.n * v.n;
+    int bad = 0;
+    for (int q0 = 0; q0 < v.nqp; q0 += v.rb_np) {
+      const int cnt = min(v.rb_np, v.nqp - q0);
+      for (int e = tid; e < cnt * nn; e += blockDim.x) lds_pl[e] = planes[(size_t)q0 * nn + e];
+      __syncthreads();
+      wg_pfinv(lds_pl, work, v.n, cnt, sh, pf_l + q0, -1.0);
+      for (int p = 0; p < cnt; ++p) bad |= (sh.info[p] != 0);
+      for (int e = tid; e < cnt * nn; e += blockDim.x) planes[(size_t)q0 * nn + e] = lds_pl[e];
+      __syncthreads();
+    }
+    return bad;
+  }
   wg_pfinv(planes, work, v.n, v.nqp, sh, pf_l, -1.0);
   int bad = 0;
   for (int p = 0; p < v.nqp; ++p) bad |= (sh.info[p] != 0);
@@ -164,6 +182,12 @@ __device__ int wg_calc_m_all(const EnsView& v, const int* ele_idx_l, int* rs, do
 __device__ inline double ip_of(const EnsView& v, const double* pf) {
   double ip = 0.0;
   for (int q = 0; q < v.nqp; ++q) ip += v.qpw[q] * pf[q];
+  return ip;
+}
+
+__device__ inline double ip_of_l(const double* qpw, const double* pf, int nqp) {
+  double ip = 0.0;
+  for (int q = 0; q < nqp; ++q) ip += qpw[q] * pf[q];
   return ip;
 }
 
@@ -212,6 +236,9 @@ struct SampLds {
   SfmtLds* rng;
   int *ele_idx, *ele_cfg, *ele_num, *misc, *delta, *scan, *rs;
   double *pf, *pf_new, *dmisc, *terms, *uni;
+  // copies of small model tables read on the serial lane-0 path
+  int *loc_spn, *gutz_idx;
+  double* qpw;
 };
 
 __device__ inline SampLds carve_sampler(const EnsView& v, char* base) {
@@ -229,8 +256,53 @@ __device__ inline SampLds carve_sampler(const EnsView& v, char* base) {
   L.terms = c.take<double>(4 * v.ns + 16);
   L.scan = c.take<int>(blockDim.x + 64);
   L.rs = c.take<int>(v.n);
+  L.loc_spn = c.take<int>(v.ns);
+  L.gutz_idx = c.take<int>(v.ns);
+  L.qpw = c.take<double>(v.nqp);
   L.uni = c.take<double>(union_doubles_sampler(v));
   return L;
+}
+
+// Fill the model-table copies of a freshly carved SampLds (all lanes; the
+// caller synchronises before use).
+__device__ inline void load_tables(const EnsView& v, SampLds& L) {
+  for (int i = threadIdx.x; i < v.ns; i += blockDim.x) {
+    L.loc_spn[i] = v.loc_spn[i];
+    L.gutz_idx[i] = v.gutz_idx[i];
+  }
+  for (int q = threadIdx.x; q < v.nqp; q += blockDim.x) L.qpw[q] = v.qpw[q];
+}
+
+// Serial sum_k a[k] * b[k] (k ascending) over LDS arrays, with the operands
+// of DOT_U terms read into registers ahead of their dependent adds.
+constexpr int DOT_U = 16;
+__device__ inline double lds_serial_dot(const double* a, const double* b, int n) {
+  double r = 0.0;
+  for (int k0 = 0; k0 < n; k0 += DOT_U) {
+    double av[DOT_U], bv[DOT_U];
+#pragma unroll
+    for (int u = 0; u < DOT_U; ++u)
+      if (k0 + u < n) { av[u] = a[k0 + u]; bv[u] = b[k0 + u]; }
+#pragma unroll
+    for (int u = 0; u < DOT_U; ++u)
+      if (k0 + u < n) r += av[u] * bv[u];
+  }
+  return r;
+}
+
+// Serial sum_k a[k] (k ascending) over an LDS array, same prefetching.
+__device__ inline double lds_serial_sum(const double* a, int n) {
+  double r = 0.0;
+  for (int k0 = 0; k0 < n; k0 += DOT_U) {
+    double av[DOT_U];
+#pragma unroll
+    for (int u = 0; u < DOT_U; ++u)
+      if (k0 + u < n) av[u] = a[k0 + u];
+#pragma unroll
+    for (int u = 0; u < DOT_U; ++u)
+      if (k0 + u < n) r += av[u];
+  }
+  return r;
 }
 
 __device__ inline void load_walker(const EnsView& v, int w, SampLds& L, bool with_rng) {
@@ -327,7 +399,7 @@ __device__ void make_proj_cnt(const EnsView& v, const int* ele_num, int* cnt) {
 // begin: burn restore or validated initial placement, initial tables, log|ip|
 // (driver.rs:239-330)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256, 3) k_begin(EnsView v) {
+__global__ void __launch_bounds__(256) k_begin(EnsView v) {
   extern __shared__ __align__(16) char lds[];
   SampLds L = carve_sampler(v, lds);
   const int tid = threadIdx.x;
@@ -393,7 +465,7 @@ __device__ void hop_deltas(const EnsView& v, SampLds& L, int ri, int rj) {
   const int* n1 = L.ele_num + ns;
   if (ri == rj) return;
   if (threadIdx.x == 0 && v.ngutz > 0) {
-    int a = v.gutz_idx[ri], b = v.gutz_idx[rj];
+    int a = L.gutz_idx[ri], b = L.gutz_idx[rj];
     if (a >= 0) atomicAdd(&L.delta[a], -(n0[ri] + n1[ri]));
     if (b >= 0) atomicAdd(&L.delta[b], n0[rj] * n1[rj]);
   }
@@ -438,11 +510,7 @@ __device__ double proj_ratio_ordered(const EnsView& v, SampLds& L) {
       ++off;
     }
   __syncthreads();
-  if (tid == 0) {
-    double z = 0.0;
-    for (int k = 0; k < total; ++k) z += L.terms[k];
-    L.dmisc[0] = z;
-  }
+  if (tid == 0) L.dmisc[0] = lds_serial_sum(L.terms, total);
   __syncthreads();
   return L.dmisc[0];
 }
@@ -479,11 +547,14 @@ __device__ inline double apply_pend(double x, double v1i, double v2i, double v1j
 // costs one address register instead of one 64-bit pointer per load.
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 constexpr int BUF_WORD3 = 0x00020000;  // raw untyped buffer (gfx9 family)
+// Cache policy of the stream (0: default; the non-temporal hint measured
+// slower: the walker's inverse is re-read at its next accepted hop).
+constexpr int BUF_NT = 0;
 __device__ inline double2 buf_load2(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, BUF_NT));
 }
 __device__ inline void buf_store2(double2 x, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
-  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, x), r, voff, soff, 0);
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, x), r, voff, soff, BUF_NT);
 }
 
 // One streaming pass over a walker's planes.  A lane owns VEC1_COLS
@@ -495,7 +566,7 @@ __device__ inline void buf_store2(double2 x, __amdgpu_buffer_rsrc_t r, int voff,
 //   pm >= 0 : apply the pending update (pv1, pv2, row pm) and write back;
 //   ACC     : out[q][j] = sum_i -X[i][j] * b[q][i], serial in i (the
 //             update_one_real vec1 = -(X^T b)).
-constexpr int STREAM_ROWS = 8;
+constexpr int STREAM_ROWS = 4;
 template <bool ACC>
 __device__ void stream_planes(double* planes, int n, int nqp, const double* pv1, const double* pv2, int pm,
                               const double* b, double* out) {
@@ -532,21 +603,37 @@ __device__ void stream_planes(double* planes, int n, int nqp, const double* pv1,
         }
       }
       const int vbase = qoff + (g0 + VEC1_COLS * jl) * (int)sizeof(double);
-      for (int i0 = 0; i0 < n; i0 += STREAM_ROWS) {
-        double2 x[STREAM_ROWS][NP2];
+      // Two tiles in flight: the loads of tile i0 + STREAM_ROWS (inverse
+      // rows, pending-vector and b rows) are issued before tile i0 is
+      // updated and stored (a load after a buffer store could not be moved
+      // above it by the compiler).
+      double2 x[STREAM_ROWS][NP2], xn[STREAM_ROWS][NP2];
+      double ra[STREAM_ROWS], rb[STREAM_ROWS], rsv[STREAM_ROWS];
+      double ran[STREAM_ROWS], rbn[STREAM_ROWS], rsvn[STREAM_ROWS];
+      auto load_tile = [&](int i0, double2 (&tx)[STREAM_ROWS][NP2], double (&ta)[STREAM_ROWS],
+                           double (&tb)[STREAM_ROWS], double (&ts)[STREAM_ROWS]) {
 #pragma unroll
         for (int r = 0; r < STREAM_ROWS; ++r)
           if (i0 + r < n) {
 #pragma unroll
             for (int m = 0; m < NP2; ++m)
-              if (jp[m] >= 0) x[r][m] = buf_load2(rs, vbase + 16 * m, (i0 + r) * rowb);
+              if (jp[m] >= 0) tx[r][m] = buf_load2(rs, vbase + 16 * m, (i0 + r) * rowb);
+            if (pm >= 0) {
+              ta[r] = p1[i0 + r];
+              tb[r] = p2[i0 + r];
+            }
+            if (ACC) ts[r] = bq[i0 + r];
           }
+      };
+      load_tile(0, x, ra, rb, rsv);
+      for (int i0 = 0; i0 < n; i0 += STREAM_ROWS) {
+        if (i0 + STREAM_ROWS < n) load_tile(i0 + STREAM_ROWS, xn, ran, rbn, rsvn);
 #pragma unroll
         for (int r = 0; r < STREAM_ROWS; ++r) {
           const int i = i0 + r;
           if (i < n) {
             if (pm >= 0) {
-              const double a0 = p1[i], b0 = p2[i];
+              const double a0 = ra[r], b0 = rb[r];
 #pragma unroll
               for (int m = 0; m < NP2; ++m)
                 if (jp[m] >= 0) {
@@ -556,7 +643,7 @@ __device__ void stream_planes(double* planes, int n, int nqp, const double* pv1,
                 }
             }
             if (ACC) {
-              const double s0 = bq[i];
+              const double s0 = rsv[r];
 #pragma unroll
               for (int m = 0; m < NP2; ++m)
                 if (jp[m] >= 0) {
@@ -565,6 +652,14 @@ __device__ void stream_planes(double* planes, int n, int nqp, const double* pv1,
                 }
             }
           }
+        }
+#pragma unroll
+        for (int r = 0; r < STREAM_ROWS; ++r) {
+#pragma unroll
+          for (int m = 0; m < NP2; ++m) x[r][m] = xn[r][m];
+          ra[r] = ran[r];
+          rb[r] = rbn[r];
+          rsv[r] = rsvn[r];
         }
       }
       if (ACC) {
@@ -601,11 +696,12 @@ __device__ void stream_planes(double* planes, int n, int nqp, const double* pv1,
   } while (0)
 #endif
 
-enum { M_TYPE = 0, M_REJ, M_MI, M_S, M_RI, M_RJ, M_MJ, M_T, M_ACC, M_NACC, M_NOUT, M_SKIP };
+enum { M_TYPE = 0, M_REJ, M_MI, M_S, M_RI, M_RJ, M_MJ, M_T, M_ACC, M_NACC, M_NOUT, M_SKIP, M_RES, M_DONE };
 
 __global__ void __launch_bounds__(256, 2) k_block(EnsView v, int out_step, int n_in) {
   extern __shared__ __align__(16) char lds[];
   SampLds L = carve_sampler(v, lds);
+  load_tables(v, L);
   const int tid = threadIdx.x, nt = blockDim.x;
   const int n = v.n, ne = v.ne, ns = v.ns, n2 = v.n2, nqp = v.nqp, nn = n * n;
   double* work = v.work + (size_t)blockIdx.x * pfinv_work_doubles(n, nqp);
@@ -618,18 +714,22 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v, int out_step, int n
     if (tid == 0) {
       L.misc[M_NOUT] = ws[2];
       L.misc[M_NACC] = ws[1];
+      L.misc[M_RES] = ws[4];
+      L.misc[M_DONE] = ws[6] | ws[5];
       L.dmisc[1] = v.log_ip[w];
     }
     __syncthreads();
     const int n_out = L.misc[M_NOUT];
-    if (out_step >= n_out) { __syncthreads(); continue; }
+    const int resume = L.misc[M_RES];
+    if (out_step >= n_out || L.misc[M_DONE]) { __syncthreads(); continue; }
     load_walker(v, w, L, true);
     double* planes = v.inv + (size_t)w * nqp * nn;
     int* cnt = v.proj_cnt + (size_t)w * v.nproj;
     long long* ctr = v.counter + (size_t)w * 4;
     for (int q = tid; q < nqp; q += nt) L.pf[q] = v.pf[(size_t)w * nqp + q];
     __syncthreads();
-    for (int step = 0; step < n_in; ++step) {
+    int stop_at = -1;  // proposal index to resume at after a rebuild
+    for (int step = resume; step < n_in; ++step) {
       // ---- candidate (lane 0) ----
       if (tid == 0) {
         SfmtLds& R = *L.rng;
@@ -650,7 +750,7 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v, int out_step, int n
               bool any = false;
               for (int k = 0; k < 2 * ne; ++k) {
                 int r = L.ele_idx[k] < 0 ? 0 : L.ele_idx[k];
-                any = any || !(r >= ns || v.loc_spn[r] == 1);
+                any = any || !(r >= ns || L.loc_spn[r] == 1);
               }
               if (!any) { rej = 1; break; }
             }
@@ -658,7 +758,7 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v, int out_step, int n
             mi = (int)sfmt_mod(R, (uint32_t)ne);
             s = sfmt_real2(R) < 0.5 ? 0 : 1;
             ri = L.ele_idx[mi + s * ne];
-            if (!(ri < 0 || ri >= ns || v.loc_spn[ri] == 1)) break;
+            if (!(ri < 0 || ri >= ns || L.loc_spn[ri] == 1)) break;
           }
           if (!rej) {
             long long icnt = 0;
@@ -666,7 +766,7 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v, int out_step, int n
               rj = (int)sfmt_mod(R, (uint32_t)ns);
               if (icnt > icnt_max) { rej = 1; break; }
               ++icnt;
-              if (L.ele_cfg[rj + s * ns] == -1 && v.loc_spn[rj] != 1) break;
+              if (L.ele_cfg[rj + s * ns] == -1 && L.loc_spn[rj] != 1) break;
             }
           }
           if (!rej) {
@@ -778,10 +878,7 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v, int out_step, int n
           __syncthreads();
           PHASE(2);
           for (int q = tid; q < nqp; q += nt) {
-            double ratio = 0.0;
-            const double* a = stage + q * n;
-            const double* b = slt + q * n;
-            for (int j = 0; j < n; ++j) ratio += a[j] * b[j];
+            const double ratio = lds_serial_dot(stage + q * n, slt + q * n, n);
             L.pf_new[q] = -ratio * L.pf[q];
           }
         } else {
@@ -841,7 +938,7 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v, int out_step, int n
         PHASE(type == 0 ? 3 : 7);
         // ---- Metropolis (lane 0) ----
         if (tid == 0) {
-          const double lip_new = ddm::cr_log(fabs(ip_of(v, L.pf_new)));
+          const double lip_new = ddm::cr_log(fabs(ip_of_l(L.qpw, L.pf_new, nqp)));
           const double ex = 2.0 * (((lpd + 0.0) + lip_new) - L.dmisc[1]);
           double wgt = ddm::cr_exp(ex);
           if (!(wgt - wgt == 0.0)) wgt = -1.0;
@@ -970,22 +1067,29 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v, int out_step, int n
           __syncthreads();
         }
       }
-      // ---- periodic recompute (engine.rs:454-473) ----
+      // ---- periodic recompute (engine.rs:454-473): handed to k_rebuild ----
       const int nacc_now = L.misc[M_NACC];
       __syncthreads();
       if (nacc_now > ns) {
-        pm = -1;  // the planes are rebuilt from the configuration
-        int bad = wg_calc_m_all(v, L.ele_idx, L.rs, planes, work, L.pf_new, (char*)L.uni, false);
-        if (tid == 0) {
-          if (!bad) {
-            for (int q = 0; q < nqp; ++q) L.pf[q] = L.pf_new[q];
-            L.dmisc[1] = ddm::cr_log(fabs(ip_of(v, L.pf)));
-          }
-          L.misc[M_NACC] = 0;
-        }
-        __syncthreads();
-        PHASE(6);
+        stop_at = step + 1;
+        break;
       }
+    }
+    if (stop_at >= 0) {
+      // the planes are rebuilt from the configuration, so the pending
+      // update is dropped; the walker resumes at proposal stop_at
+      store_walker(v, w, L);
+      for (int q = tid; q < nqp; q += nt) v.pf[(size_t)w * nqp + q] = L.pf[q];
+      if (tid == 0) {
+        v.log_ip[w] = L.dmisc[1];
+        ws[1] = L.misc[M_NACC];
+        ws[4] = stop_at;
+        ws[5] = 1;
+        atomicAdd(v.nflag, 1);
+      }
+      __syncthreads();
+      PHASE(6);
+      continue;
     }
     // ---- end of block: apply the pending update, persist state, save sample ----
     if (pm >= 0) {
@@ -997,6 +1101,8 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v, int out_step, int n
     if (tid == 0) {
       v.log_ip[w] = L.dmisc[1];
       ws[1] = L.misc[M_NACC];
+      ws[4] = n_in;
+      ws[6] = 1;
       if (out_step == n_out - 1) ws[0] = 1;  // save_burn + counter[9] = 1
     }
     if (out_step + v.nsample >= n_out) {
@@ -1014,6 +1120,60 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v, int out_step, int n
     __syncthreads();
     PHASE(8);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Periodic rebuild of the walkers k_block handed over (engine.rs:454-473):
+// calc_m_all from the configuration; on success the Pfaffians and
+// log|<x|psi>| are replaced; the accept window restarts either way.  A
+// kernel of its own so the elimination's register budget does not limit the
+// sampler's occupancy.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline size_t lds_bytes_rebuild_base(const EnsView& v) {
+  return 2 * rnd16((size_t)v.n * 4) + rnd16((size_t)v.nqp * 8) + rnd16(pf_bytes(v.n, v.nqp));
+}
+__host__ __device__ inline size_t lds_bytes_rebuild(const EnsView& v) {
+  return lds_bytes_rebuild_base(v) + rnd16((size_t)v.rb_np * v.n * v.n * 8);
+}
+
+__global__ void __launch_bounds__(256) k_rebuild(EnsView v) {
+  extern __shared__ __align__(16) char lds[];
+  Carver c{lds};
+  int* ei = c.take<int>(v.n);
+  int* rs = c.take<int>(v.n);
+  double* pfn = c.take<double>(v.nqp);
+  char* pfscr = (char*)c.take<char>(pf_bytes(v.n, v.nqp));
+  double* lds_pl = v.rb_np > 0 ? c.take<double>((size_t)v.rb_np * v.n * v.n) : nullptr;
+  const int tid = threadIdx.x;
+  double* work = v.work + (size_t)blockIdx.x * pfinv_work_doubles(v.n, v.nqp);
+  for (int w = blockIdx.x; w < v.W; w += gridDim.x) {
+    int* ws = v.wstate + (size_t)w * 8;
+    __shared__ int flag;
+    if (tid == 0) flag = ws[5];
+    __syncthreads();
+    if (!flag) continue;
+    for (int i = tid; i < v.n; i += blockDim.x) ei[i] = v.ele_idx[(size_t)w * v.n + i];
+    __syncthreads();
+    double* planes = v.inv + (size_t)w * v.nqp * v.n * v.n;
+    const int bad = wg_calc_m_all(v, ei, rs, planes, work, pfn, pfscr, false, lds_pl);
+    if (tid == 0) {
+      if (!bad) {
+        for (int q = 0; q < v.nqp; ++q) v.pf[(size_t)w * v.nqp + q] = pfn[q];
+        v.log_ip[w] = ddm::cr_log(fabs(ip_of(v, pfn)));
+      }
+      ws[1] = 0;
+      ws[5] = 0;
+    }
+    __syncthreads();
+  }
+}
+
+// Start of an outer step: every walker resumes at proposal 0.
+__global__ void k_reset_resume(EnsView v) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= v.W) return;
+  v.wstate[(size_t)w * 8 + 4] = 0;
+  v.wstate[(size_t)w * 8 + 6] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1044,7 +1204,46 @@ __device__ double direct_proj_ratio_moved(const EnsView& v, int source, int dest
   return ddm::cr_exp(z);
 }
 
-__global__ void __launch_bounds__(256, 3) k_measure(EnsView v, double* gscratch, int gscratch_stride, int mode_exp) {
+// Rebuild for measurement: the inverse and Pfaffians of saved sample s of
+// every walker into the walker's planes and pf; s_valid[w][s] = the sample
+// is not the all-zero / all-negative placeholder and calc_m_all succeeded.
+__global__ void __launch_bounds__(256) k_meas_rebuild(EnsView v, int s) {
+  extern __shared__ __align__(16) char lds[];
+  Carver c{lds};
+  int* ei = c.take<int>(v.n);
+  int* rs = c.take<int>(v.n);
+  double* pfn = c.take<double>(v.nqp);
+  char* pfscr = (char*)c.take<char>(pf_bytes(v.n, v.nqp));
+  double* lds_pl = v.rb_np > 0 ? c.take<double>((size_t)v.rb_np * v.n * v.n) : nullptr;
+  const int tid = threadIdx.x;
+  double* work = v.work + (size_t)blockIdx.x * pfinv_work_doubles(v.n, v.nqp);
+  __shared__ int flag;
+  for (int w = blockIdx.x; w < v.W; w += gridDim.x) {
+    const size_t ws = (size_t)w * v.nsample + s;
+    for (int i = tid; i < v.n; i += blockDim.x) ei[i] = v.s_idx[ws * v.n + i];
+    __syncthreads();
+    if (tid == 0) {
+      int all0 = 1, allneg = 1;
+      for (int k = 0; k < v.n; ++k) { all0 &= (ei[k] == 0); allneg &= (ei[k] < 0); }
+      flag = (all0 || allneg) ? 0 : 1;
+    }
+    __syncthreads();
+    int valid = flag;
+    if (valid) {
+      double* planes = v.inv + (size_t)w * v.nqp * v.n * v.n;
+      valid = !wg_calc_m_all(v, ei, rs, planes, work, pfn, pfscr, false, lds_pl);
+    }
+    if (tid == 0) {
+      v.s_valid[ws] = valid;
+      if (valid)
+        for (int q = 0; q < v.nqp; ++q) v.pf[(size_t)w * v.nqp + q] = pfn[q];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(256, 2) k_measure(EnsView v, double* gscratch, int gscratch_stride, int mode_exp,
+                                                   int s_only) {
   extern __shared__ __align__(16) char lds[];
   Carver c{lds};
   int* ei = c.take<int>(v.n);
@@ -1065,24 +1264,17 @@ __global__ void __launch_bounds__(256, 3) k_measure(EnsView v, double* gscratch,
   double* g1 = gscratch + (size_t)blockIdx.x * gscratch_stride;  // per term
   for (int w = blockIdx.x; w < v.W; w += gridDim.x) {
     double* planes = v.inv + (size_t)w * nqp * nn;
-    for (int s = 0; s < v.nsample; ++s) {
+    for (int s = s_only; s == s_only; ++s) {
       const size_t ws = (size_t)w * v.nsample + s;
       double* o = v.o_store + ws * v.size;
       for (int i = tid; i < v.size; i += nt) o[i] = 0.0;
       for (int i = tid; i < n; i += nt) ei[i] = v.s_idx[ws * n + i];
       for (int i = tid; i < n2; i += nt) { ec[i] = v.s_cfg[ws * n2 + i]; en[i] = v.s_num[ws * n2 + i]; }
       for (int i = tid; i < v.nproj; i += nt) epc[i] = v.s_proj[ws * v.nproj + i];
-      if (tid == 0) {
-        int all0 = 1, allneg = 1;
-        for (int k = 0; k < n; ++k) { all0 &= (ei[k] == 0); allneg &= (ei[k] < 0); }
-        flags[0] = (all0 || allneg) ? 0 : 1;
-      }
+      if (tid == 0) flags[0] = v.s_valid[ws];  // set by k_meas_rebuild
+      for (int q = tid; q < nqp; q += nt) pf[q] = v.pf[(size_t)w * nqp + q];
       __syncthreads();
       int valid = flags[0];
-      if (valid) {
-        int bad = wg_calc_m_all(v, ei, rs, planes, work, pf, pfscr, false);
-        valid = !bad;
-      }
       double ip = 0.0;
       if (valid) {
         if (tid == 0) dm[0] = ip_of(v, pf);

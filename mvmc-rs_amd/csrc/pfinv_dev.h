@@ -27,7 +27,9 @@ __host__ __device__ inline size_t pfinv_work_doubles(int n, int np) {
 
 // LDS scratch of wg_pfinv, carved from dynamic shared memory by the caller:
 // piv[np*n], colbuf[np*n], kp/info[np], colmax[np], ks[np*n] (panel step of
-// the last swap through each index), act[np*LTL_K] (step applied per plane).
+// the last swap through each index), act[np*LTL_K] (step applied per plane),
+// zmax[2*np] (extent of the zeroed diagonal; whether the last x column
+// awaits scaling).
 struct PfinvShared {
   int* piv;
   double* colbuf;
@@ -48,7 +50,7 @@ __host__ __device__ inline size_t pf_rnd16(size_t b) { return (b + 15) & ~(size_
 // Bytes of LDS that carve_pfinv takes.
 __host__ __device__ inline size_t pf_bytes(int n, int np) {
   return 2 * pf_rnd16((size_t)np * n * sizeof(int)) + pf_rnd16((size_t)np * n * sizeof(double)) +
-         3 * pf_rnd16((size_t)np * 8) + pf_rnd16((size_t)np * LTL_K * sizeof(int)) + pf_rnd16((size_t)np * sizeof(int));
+         3 * pf_rnd16((size_t)np * 8) + pf_rnd16((size_t)np * LTL_K * sizeof(int)) + pf_rnd16(2 * (size_t)np * sizeof(int));
 }
 
 template <class Carver>
@@ -61,7 +63,7 @@ __device__ inline void carve_pfinv(Carver& c, PfinvShared& sh, int n, int np) {
   sh.colmax = c.template take<double>(np);
   sh.ks = c.template take<int>((size_t)np * n);
   sh.act = c.template take<int>((size_t)np * LTL_K);
-  sh.zmax = c.template take<int>(np);
+  sh.zmax = c.template take<int>(2 * (size_t)np);
 }
 
 // Argmax of |x| over a column with C IDAMAX semantics: first index of the
@@ -85,11 +87,17 @@ __device__ inline void assemble_planes(double* planes, const double* slater, con
 }
 
 // Core routine.  planes: np planes (stride n*n).  work: pfinv_work_doubles
-// of scratch (panel vectors of the elimination, then M^T and a staged column
-// block).  On exit, for every plane with info == 0,
-// pf[p] holds the Pfaffian and the plane holds sign * A^{-1}
+// of scratch (panel vectors of the elimination, then M^T, a staged column
+// block and the permutation gather).  On exit, for every plane with
+// info == 0, pf[p] holds the Pfaffian and the plane holds sign * A^{-1}
 // (sign = -1 gives mVMC's invM).  info[p]: 0 ok, k > 0 zero pivot at 1-based
 // row k, -1 non-finite Pfaffian.  All threads of the block must call it.
+//
+// Every phase is written so that a lane issues the loads of all its items
+// (PF_U per batch) before its first store: the work is bound by memory
+// round trips, and a load after a store to the same buffer could not be
+// hoisted by the compiler.
+constexpr int PF_U = 8;
 __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShared& sh, double* pf, double sign) {
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int lane = tid & 63, wave = tid >> 6, nwave = (nthr + 63) >> 6;
@@ -99,30 +107,29 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
   __syncthreads();
 
   // ---------------- LTL^T (dsktf2) ----------------
-  // The rank-2 updates of the trailing block are delayed and applied per
-  // panel of `kw` steps.  Every element still receives its updates in step
-  // order, each as (c + x*t1) - y*t2 with the x, y, t1, t2 of that step, at
-  // the position it held at that step:
-  //  - before a swap, the elements it moves (columns kk0 and kp, row kp) and
-  //    the new y column are brought up to date, so nothing pending is moved;
-  //  - an element at (i, j) has received the steps before
-  //    max(ks[i], ks[j]) (the last panel step whose swap wrote row/column i
-  //    or j); the others stay pending with the step's x, y rows i, j, which
-  //    later swaps do not touch.
+  // Panels of `kw` steps.  The panel's own future y columns [m, kk0) take
+  // every step's rank-2 update as it happens (eager); the trailing block
+  // (columns j < m) takes them at the panel's end (delayed).  Every element
+  // still receives its updates in step order, each as (c + x*t1) - y*t2 with
+  // the x, y, t1, t2 of that step, at the position it held at that step:
+  //  - before a swap, the trailing elements it moves (column kp, row kp) are
+  //    brought up to date, so nothing pending is moved;
+  //  - a trailing element at (i, j) has received the panel steps before
+  //    max(ks[i], ks[j]) (ks[k]: the last panel step whose swap wrote row /
+  //    column k); the others stay pending with the step's x, y rows i, j,
+  //    which later swaps do not touch;
+  //  - the scaling of the finished x column by alpha is applied in the next
+  //    step, before its swap (a swap only exchanges rows that are both
+  //    scaled, so the order does not matter).
   // Panel vectors (x, y, t1 = alpha*y, t2 = alpha*x per step) live in `work`.
   const int kw = n >= 4 * LTL_K ? LTL_K : (n >= 4 ? n / 4 : 1);
   auto PX = [&](int p, int t) { return work + ((size_t)(p * 4 + 0) * kw + t) * n; };
   auto PY = [&](int p, int t) { return work + ((size_t)(p * 4 + 1) * kw + t) * n; };
   auto PT1 = [&](int p, int t) { return work + ((size_t)(p * 4 + 2) * kw + t) * n; };
   auto PT2 = [&](int p, int t) { return work + ((size_t)(p * 4 + 3) * kw + t) * n; };
-  // pending steps [s, t_end) of plane p on element (i, j), i < j
-  // (the loads of CU_G pending steps are issued before their serial chain)
-  constexpr int CU_G = 4;
-  auto catch_up = [&](int p, int i, int j, int t_end) {
-    const int s0 = max(sh.KS(p, i), sh.KS(p, j));
-    if (s0 >= t_end) return;
-    double* a = planes + (size_t)p * nn;
-    double c = a[i + (size_t)j * n];
+  // pending steps [s0, t_end) of plane p on c = element (i, j), i < j
+  auto pending = [&](int p, int i, int j, int s0, int t_end, double c) {
+    constexpr int CU_G = 4;  // pending steps whose loads are issued together
     for (int g = s0; g < t_end; g += CU_G) {
       double xv[CU_G], yv[CU_G], t1v[CU_G], t2v[CU_G];
 #pragma unroll
@@ -137,23 +144,39 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
       for (int u = 0; u < CU_G; ++u)
         if (g + u < t_end && sh.act[p * LTL_K + g + u]) c = (c + xv[u] * t1v[u]) - yv[u] * t2v[u];
     }
-    a[i + (size_t)j * n] = c;
+    return c;
   };
+  double* alph = sh.colbuf;  // alpha of the step whose x column awaits scaling, per plane (colbuf[p*n])
   for (int e = tid; e < np * n; e += nthr) sh.ks[e] = 0;
-  for (int p = tid; p < np; p += nthr) sh.zmax[p] = 0;
+  for (int p = tid; p < 2 * np; p += nthr) sh.zmax[p] = 0;
   __syncthreads();
-  int tp = 0;  // steps recorded in the open panel
+  int tp = 0;          // steps recorded in the open panel
+  int m_pan = 0;       // first eager column of the open panel
+  int scale_col = -1;  // x column of the previous step (awaiting scaling)
+  auto scale_on = [&](int p) { return sh.zmax[np + p] != 0; };
   for (int k0 = n - 1; k0 >= 1; --k0) {
     const int kk0 = k0 - 1;
-    // pivot search on the (up to date) column k0: one wave per plane
+    // (A) pivot search on the (up to date) column k0: one wave per plane
     for (int p = wave; p < np; p += nwave) {
       const double* col = planes + (size_t)p * nn + (size_t)k0 * n;
       double bv = -2.0;
       int bj = 0x7fffffff;
-      for (int j = lane; j <= kk0; j += 64) {
-        double kv;
-        piv_key(fabs(col[j]), j, kv);
-        if (kv > bv) { bv = kv; bj = j; }
+      for (int j0 = 0; j0 <= kk0; j0 += 64 * 4) {
+        double v4[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int j = j0 + lane + 64 * u;
+          if (j <= kk0) v4[u] = col[j];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int j = j0 + lane + 64 * u;
+          if (j <= kk0) {
+            double kv;
+            piv_key(fabs(v4[u]), j, kv);
+            if (kv > bv) { bv = kv; bj = j; }
+          }
+        }
       }
       for (int off = 32; off >= 1; off >>= 1) {
         double ov = __shfl_xor(bv, off, 64);
@@ -166,42 +189,80 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
       }
     }
     __syncthreads();
-    // bring the y column (kk0) and, when swapping, column kp and row kp up to
-    // date; items (plane, e) over all planes at once:
-    //   e < kk0: (e, kk0);  then f = e - kk0: f < kp: (f, kp), else (kp, f + 1)
-    if (tp > 0) {
-      const int per = 2 * kk0 - 1;
-      for (int it = tid; it < np * per; it += nthr) {
-        const int p = it / per, e = it % per;
-        const int kp = sh.kp[p];
-        const bool sw = sh.colmax[p] != 0.0 && kp != kk0;
-        if (e < kk0) catch_up(p, e, kk0, tp);
-        else if (sw) {
-          const int f = e - kk0;
-          if (f < kp) catch_up(p, f, kp, tp);
-          else catch_up(p, kp, f + 1, tp);
+    if (tp == 0) m_pan = max(0, kk0 - kw + 1);
+    const int m = m_pan;  // trailing columns j < m; eager columns [m, kk0)
+    // (B1) scale the previous x column (rows < k0)
+    if (scale_col >= 0) {
+      for (int b0 = 0; b0 < np * k0; b0 += nthr * PF_U) {
+        double vv[PF_U];
+#pragma unroll
+        for (int u = 0; u < PF_U; ++u) {
+          const int it = b0 + tid + nthr * u;
+          if (it < np * k0 && scale_on(it / k0)) vv[u] = planes[(size_t)(it / k0) * nn + it % k0 + (size_t)scale_col * n];
+        }
+#pragma unroll
+        for (int u = 0; u < PF_U; ++u) {
+          const int it = b0 + tid + nthr * u;
+          if (it < np * k0 && scale_on(it / k0))
+            planes[(size_t)(it / k0) * nn + it % k0 + (size_t)scale_col * n] = vv[u] * alph[(it / k0) * n];
         }
       }
-      __syncthreads();
     }
-    // swaps and sign flips (ltl.rs:131-158), items (plane, e), e < n - 1:
+    // (B2) bring the trailing elements a swap moves up to date: items
+    // (plane, f), f < m - 1: f < kp: (f, kp), else (kp, f + 1)  [kp < m]
+    if (tp > 0 && m >= 2) {
+      const int per = m - 1;
+      for (int it = tid; it < np * per; it += nthr) {
+        const int p = it / per, f = it % per;
+        const int kp = sh.kp[p];
+        if (sh.colmax[p] != 0.0 && kp != kk0 && kp < m) {
+          const int i = f < kp ? f : kp, j = f < kp ? kp : f + 1;
+          const int s0 = max(sh.KS(p, i), sh.KS(p, j));
+          if (s0 < tp) {
+            double* e = planes + (size_t)p * nn + i + (size_t)j * n;
+            *e = pending(p, i, j, s0, tp, *e);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // (C) swaps and sign flips (ltl.rs:131-158), items (plane, e), e < n - 1:
     //   e < kp: a[e][kk0] <-> a[e][kp];  e == kp: a[kp][kk0] = -a[kp][kk0];
     //   kp < e < kk0: a[e][kk0] <-> a[kp][e], both negated;
     //   e >= kk0: rows kk0 <-> kp of column e + 1 (finished columns)
-    for (int it = tid; it < np * (n - 1); it += nthr) {
-      const int p = it / (n - 1), e = it % (n - 1);
-      const int kp = sh.kp[p];
-      if (sh.colmax[p] == 0.0 || kp == kk0) continue;
-      double* a = planes + (size_t)p * nn;
-      if (e < kp) {
-        double t = a[e + (size_t)kk0 * n]; a[e + (size_t)kk0 * n] = a[e + (size_t)kp * n]; a[e + (size_t)kp * n] = t;
-      } else if (e == kp) {
-        a[kp + (size_t)kk0 * n] = -a[kp + (size_t)kk0 * n];
-      } else if (e < kk0) {
-        double t = a[e + (size_t)kk0 * n]; a[e + (size_t)kk0 * n] = -a[kp + (size_t)e * n]; a[kp + (size_t)e * n] = -t;
-      } else {
-        const int j = e + 1;
-        double t = a[kk0 + (size_t)j * n]; a[kk0 + (size_t)j * n] = a[kp + (size_t)j * n]; a[kp + (size_t)j * n] = t;
+    {
+      // the item count is made opaque so the item decompositions are not
+      // hoisted out of the step loop (they would stay live across phases)
+      int nit = np * (n - 1);
+      asm volatile("" : "+v"(nit));
+      nit = __builtin_amdgcn_readfirstlane(nit);
+      for (int b0 = 0; b0 < nit; b0 += nthr * 4) {
+        double v0[4], v1[4];
+        int o0[4], o1[4], kind[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int it = b0 + tid + nthr * u;
+          kind[u] = 0;
+          if (it < nit) {
+            const int p = it / (n - 1), e = it % (n - 1);
+            const int kp = sh.kp[p];
+            if (sh.colmax[p] != 0.0 && kp != kk0) {
+              const int base = p * nn;
+              if (e < kp) { kind[u] = 1; o0[u] = base + e + kk0 * n; o1[u] = base + e + kp * n; }
+              else if (e == kp) { kind[u] = 2; o0[u] = base + kp + kk0 * n; o1[u] = o0[u]; }
+              else if (e < kk0) { kind[u] = 3; o0[u] = base + e + kk0 * n; o1[u] = base + kp + e * n; }
+              else { kind[u] = 1; o0[u] = base + kk0 + (e + 1) * n; o1[u] = base + kp + (e + 1) * n; }
+              v0[u] = planes[o0[u]];
+              v1[u] = planes[o1[u]];
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (kind[u] == 1) { planes[o0[u]] = v1[u]; planes[o1[u]] = v0[u]; }
+          else if (kind[u] == 2) planes[o0[u]] = -v0[u];
+          else if (kind[u] == 3) { planes[o0[u]] = -v1[u]; planes[o1[u]] = -v0[u]; }
+        }
       }
     }
     if (tid < np) {
@@ -218,38 +279,63 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
       if (on && kk0 > sh.zmax[p]) sh.zmax[p] = kk0;
     }
     __syncthreads();
+    // (D) record the step (x, y unscaled, t1 = alpha*y[j], t2 = alpha*x[j])
+    // and apply it to the eager columns [m, kk0): a lane owns row i of plane
+    // p, loads x_i, y_i and its eager elements together; x_j, y_j are
+    // wave-shared loads.  t1 = alpha*y_j, t2 = alpha*x_j are formed by the
+    // same expressions as the recorded ones.
     if (kk0 >= 1) {
-      // record the step: x, y (unscaled), t1 = alpha*y[j], t2 = alpha*x[j]
-      for (int e = tid; e < np * kk0; e += nthr) {
-        const int p = e / kk0, i = e % kk0;
-        if (sh.colmax[p] == 0.0) continue;
-        const double* a = planes + (size_t)p * nn;
-        const double alpha = 1.0 / a[kk0 + (size_t)k0 * n];
-        const double x = a[i + (size_t)k0 * n], y = a[i + (size_t)kk0 * n];
-        PX(p, tp)[i] = x;
-        PY(p, tp)[i] = y;
-        PT1(p, tp)[i] = alpha * y;
-        PT2(p, tp)[i] = alpha * x;
+      for (int b0 = 0; b0 < np * kk0; b0 += nthr) {
+        const int it = b0 + tid;
+        if (it < np * kk0) {
+          const int p = it / kk0, i = it % kk0;
+          if (sh.colmax[p] != 0.0) {
+            double* a = planes + (size_t)p * nn;
+            const double xi = a[i + (size_t)k0 * n], yi = a[i + (size_t)kk0 * n];
+            const double den = a[kk0 + (size_t)k0 * n];
+            const double alpha = 1.0 / den;
+            PX(p, tp)[i] = xi;
+            PY(p, tp)[i] = yi;
+            PT1(p, tp)[i] = alpha * yi;
+            PT2(p, tp)[i] = alpha * xi;
+            if (i == 0) alph[p * n] = alpha;
+            // eager columns in chunks of EC (their loads issued together)
+            constexpr int EC = 4;
+            for (int c0 = m; c0 < kk0; c0 += EC) {
+              double cv[EC], xj[EC], yj[EC];
+#pragma unroll
+              for (int c = 0; c < EC; ++c) {
+                const int j = c0 + c;
+                if (j < kk0) {
+                  xj[c] = a[j + (size_t)k0 * n];
+                  yj[c] = a[j + (size_t)kk0 * n];
+                  if (i < j) cv[c] = a[i + (size_t)j * n];
+                }
+              }
+#pragma unroll
+              for (int c = 0; c < EC; ++c) {
+                const int j = c0 + c;
+                if (j < kk0 && i < j) a[i + (size_t)j * n] = (cv[c] + xi * (alpha * yj[c])) - yi * (alpha * xj[c]);
+              }
+            }
+          }
+        }
       }
       __syncthreads();
-      for (int e = tid; e < np * kk0; e += nthr) {
-        const int p = e / kk0, i = e % kk0;
-        if (sh.colmax[p] == 0.0) continue;
-        double* a = planes + (size_t)p * nn;
-        const double alpha = 1.0 / a[kk0 + (size_t)k0 * n];
-        a[i + (size_t)k0 * n] *= alpha;
-      }
+      scale_col = k0;
+    } else {
+      scale_col = -1;
     }
+    for (int p = tid; p < np; p += nthr) sh.zmax[np + p] = sh.colmax[p] != 0.0 && kk0 >= 1;  // scale_on
     ++tp;
-    // close the panel: every element of the trailing triangle (i < j < kk0)
-    // takes its pending steps; a lane owns row i of plane p and keeps the
-    // row's x, y of all panel steps in registers.
+    // (E) close the panel: every element of the trailing triangle (i < j < kk0)
+    // takes its pending steps; work item = (plane, 64-row chunk) per wave, so
+    // plane and column are wave-uniform; a lane keeps its row's x, y of all
+    // panel steps in registers.
     if (tp == kw || kk0 <= 1) {
       __syncthreads();
       const int m = kk0;  // trailing columns j < m, rows i < m - 1
       if (m >= 2) {
-        // work item = (plane, 64-row chunk) per wave, so plane and column
-        // are wave-uniform and the step scalars t1, t2 load once per wave
         const int chunks = (m - 1 + 63) / 64;
         for (int it = __builtin_amdgcn_readfirstlane(wave); it < np * chunks; it += nwave) {
           const int p = it / chunks, c = it % chunks;
@@ -266,8 +352,7 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
           }
           const int ksi = on ? sh.KS(p, i) : tp;
           double* a = planes + (size_t)p * nn;
-          // JU columns per iteration, all their loads issued first
-          constexpr int JU = 2;
+          constexpr int JU = 1;
           for (int j = c * 64 + 1; j < m; j += JU) {
             double t1v[JU][LTL_K], t2v[JU][LTL_K], cv[JU];
             int ksj[JU];
@@ -339,13 +424,29 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
   for (int p = 0; p < np; ++p) {
     if (sh.info[p] != 0) continue;
     double* a = planes + (size_t)p * nn;
+    // opaque per-plane copies of the sizes keep the item decompositions of
+    // the batched loops inside the plane loop (not hoisted and kept live)
+    int nno = nn, tjn = TJ * n;
+    asm volatile("" : "+v"(nno), "+v"(tjn));
+    nno = __builtin_amdgcn_readfirstlane(nno);
+    tjn = __builtin_amdgcn_readfirstlane(tjn);
     // trtri('U','U') on B(i,j) = A(i,j+1), 0 <= i < j < n-1:
     // Bn(i,j) = -(B(i,j) + sum_{k=i+1}^{j-1} Bn(i,k) B(k,j)), k ascending.
     for (int j0 = 1; j0 < n - 1; j0 += TJ) {
       const int j1 = min(n - 1, j0 + TJ);
-      for (int e = tid; e < TJ * n; e += nthr) {
-        const int jj = e / n, k = e % n;
-        stage[e] = (j0 + jj < j1 && k < j0 + jj) ? a[k + (size_t)(j0 + jj + 1) * n] : 0.0;
+      for (int b0 = 0; b0 < tjn; b0 += nthr * PF_U) {
+        double sv[PF_U];
+#pragma unroll
+        for (int u = 0; u < PF_U; ++u) {
+          const int e = b0 + tid + nthr * u;
+          const int jj = e / n, k = e % n;
+          sv[u] = (e < tjn && j0 + jj < j1 && k < j0 + jj) ? a[k + (size_t)(j0 + jj + 1) * n] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < PF_U; ++u) {
+          const int e = b0 + tid + nthr * u;
+          if (e < tjn) stage[e] = sv[u];
+        }
       }
       __syncthreads();
       for (int i0 = 0; i0 < j1 - 1; i0 += nthr) {
@@ -355,8 +456,8 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
 #pragma unroll
         for (int jj = 0; jj < TJ; ++jj) acc[jj] = (on && j0 + jj < j1 && i < j0 + jj) ? stage[jj * n + i] : 0.0;
         // k < j0: finished columns of earlier blocks (KU loads issued together)
-        const int kst = i0 + (tid & ~63) + 1;  // wave-uniform start
-        constexpr int KU = 4;
+        const int kst = __builtin_amdgcn_readfirstlane(i0 + (tid & ~63) + 1);  // wave-uniform start
+        constexpr int KU = 8;
         for (int k = kst; k < j0; k += KU) {
           double bn[KU];
 #pragma unroll
@@ -392,11 +493,21 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
     // Mt = M^T with M = I, M(ir, jr+1) = A(ir, jr+2) for ir <= jr < n-2;
     // vt(i) = -A(i, i+1)
     double* mt = work + (size_t)p * nn;
-    for (int e = tid; e < nn; e += nthr) {
-      const int col = e % n, row = e / n;  // Mt(col, row) = M(row, col) at col + row*n
-      double v = (row == col) ? 1.0 : 0.0;
-      if (col >= 1 && col <= n - 2 && row <= col - 1) v = a[row + (size_t)(col + 1) * n];
-      mt[e] = v;
+    for (int b0 = 0; b0 < nno; b0 += nthr * PF_U) {
+      double mv[PF_U];
+#pragma unroll
+      for (int u = 0; u < PF_U; ++u) {
+        const int e = b0 + tid + nthr * u;
+        const int col = e % n, row = e / n;  // Mt(col, row) = M(row, col) at col + row*n
+        double v = (row == col) ? 1.0 : 0.0;
+        if (e < nno && col >= 1 && col <= n - 2 && row <= col - 1) v = a[row + (size_t)(col + 1) * n];
+        mv[u] = v;
+      }
+#pragma unroll
+      for (int u = 0; u < PF_U; ++u) {
+        const int e = b0 + tid + nthr * u;
+        if (e < nno) mt[e] = mv[u];
+      }
     }
     for (int i = tid; i < n - 1; i += nthr) sh.C(p, i) = -a[i + (size_t)(i + 1) * n];
     __syncthreads();
@@ -445,9 +556,19 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
     const bool upper = n > 64;
     for (int j0 = 0; j0 < n; j0 += TJ) {
       const int j1 = min(n, j0 + TJ);
-      for (int e = tid; e < TJ * n; e += nthr) {
-        const int jj = e / n, k = e % n;
-        stage[e] = (j0 + jj < j1) ? a[k + (size_t)(j0 + jj) * n] : 0.0;
+      for (int b0 = 0; b0 < tjn; b0 += nthr * PF_U) {
+        double sv[PF_U];
+#pragma unroll
+        for (int u = 0; u < PF_U; ++u) {
+          const int e = b0 + tid + nthr * u;
+          const int jj = e / n, k = e % n;
+          sv[u] = (e < tjn && j0 + jj < j1) ? a[k + (size_t)(j0 + jj) * n] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < PF_U; ++u) {
+          const int e = b0 + tid + nthr * u;
+          if (e < tjn) stage[e] = sv[u];
+        }
       }
       __syncthreads();
       const int rows = upper ? j1 - 1 : n;
@@ -458,7 +579,7 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
 #pragma unroll
         for (int jj = 0; jj < TJ; ++jj) acc[jj] = on ? stage[jj * n + (on ? i : 0)] : 0.0;
         const int kend = min(rows, i0 + (tid & ~63) + 64) - 1;  // wave's largest row
-        constexpr int KU = 4;
+        constexpr int KU = 8;
         for (int k = 0; k < kend; k += KU) {
           double mv[KU];
 #pragma unroll
@@ -479,40 +600,71 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
       __syncthreads();
     }
     if (upper) {
-      for (int e = tid; e < nn; e += nthr) {
-        const int row = e % n, col = e / n;
-        if (row == col) a[e] = 0.0;
-        else if (row > col) a[e] = -a[col + (size_t)row * n];
+      for (int b0 = 0; b0 < nno; b0 += nthr * PF_U) {
+        double vv[PF_U];
+#pragma unroll
+        for (int u = 0; u < PF_U; ++u) {
+          const int e = b0 + tid + nthr * u;
+          const int row = e % n, col = e / n;
+          if (e < nno && row > col) vv[u] = a[col + (size_t)row * n];
+        }
+#pragma unroll
+        for (int u = 0; u < PF_U; ++u) {
+          const int e = b0 + tid + nthr * u;
+          const int row = e % n, col = e / n;
+          if (e < nno) {
+            if (row == col) a[e] = 0.0;
+            else if (row > col) a[e] = -vv[u];
+          }
+        }
       }
       __syncthreads();
     }
   }
-  // column then row permutations (sequential swap order per row / column),
-  // then the sign (calc_m_all's final DSCAL by -1).
-  for (int e = tid; e < np * n; e += nthr) {
-    const int p = e / n, r = e % n;
-    if (sh.info[p] != 0) continue;
-    double* a = planes + (size_t)p * nn;
+  // column then row permutations: the sequential swaps (j <-> P[j]-1,
+  // j ascending) compose into one index map g, applied to rows and columns
+  // as a gather (pure data movement), then the sign (calc_m_all's final
+  // DSCAL by -1).  g lives in ks, the gather goes through `work`.
+  if (tid < np) {
+    const int p = tid;
+    int* g = &sh.KS(p, 0);
+    for (int k = 0; k < n; ++k) g[k] = k;
     for (int j = 0; j < n; ++j) {
       const int t = sh.P(p, j) - 1;
-      if (t != j) {
-        double x = a[r + (size_t)j * n];
-        a[r + (size_t)j * n] = a[r + (size_t)t * n];
-        a[r + (size_t)t * n] = x;
-      }
+      if (t != j) { const int x = g[j]; g[j] = g[t]; g[t] = x; }
     }
   }
   __syncthreads();
-  for (int e = tid; e < np * n; e += nthr) {
-    const int p = e / n, c = e % n;
-    if (sh.info[p] != 0) continue;
-    double* col = planes + (size_t)p * nn + (size_t)c * n;
-    for (int i = 0; i < n; ++i) {
-      const int t = sh.P(p, i) - 1;
-      if (t != i) { double x = col[i]; col[i] = col[t]; col[t] = x; }
+  for (int b0 = 0; b0 < np * nn; b0 += nthr * PF_U) {
+    double vv[PF_U];
+#pragma unroll
+    for (int u = 0; u < PF_U; ++u) {
+      const int e = b0 + tid + nthr * u;
+      if (e < np * nn) {
+        const int p = e / nn, r = e % nn;
+        const int row = r % n, col = r / n;
+        if (sh.info[p] == 0) vv[u] = planes[(size_t)p * nn + sh.KS(p, row) + (size_t)sh.KS(p, col) * n];
+      }
     }
-    if (sign != 1.0)
-      for (int i = 0; i < n; ++i) col[i] = -col[i];
+#pragma unroll
+    for (int u = 0; u < PF_U; ++u) {
+      const int e = b0 + tid + nthr * u;
+      if (e < np * nn && sh.info[e / nn] == 0) work[e] = sign != 1.0 ? -vv[u] : vv[u];
+    }
+  }
+  __syncthreads();
+  for (int b0 = 0; b0 < np * nn; b0 += nthr * PF_U) {
+    double vv[PF_U];
+#pragma unroll
+    for (int u = 0; u < PF_U; ++u) {
+      const int e = b0 + tid + nthr * u;
+      if (e < np * nn && sh.info[e / nn] == 0) vv[u] = work[e];
+    }
+#pragma unroll
+    for (int u = 0; u < PF_U; ++u) {
+      const int e = b0 + tid + nthr * u;
+      if (e < np * nn && sh.info[e / nn] == 0) planes[e] = vv[u];
+    }
   }
   __syncthreads();
 }

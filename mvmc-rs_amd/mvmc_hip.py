@@ -179,7 +179,7 @@ class Ensemble:
         t = np.zeros(8)
         lib().mvmc_ens_timings(self.h, _p(t))
         return {"sample_s": t[0], "measure_s": t[1], "sr_s": t[2], "proposals": t[3], "accepted": t[4],
-                "block_ms": t[5], "block_launches": t[6]}
+                "block_ms": t[5], "block_launches": t[6], "rebuild_ms": t[7]}
 
     PHASES = ("candidate", "projection", "staging", "ratio", "metropolis", "accept_pass", "rebuild", "exchange",
               "block_end")
