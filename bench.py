@@ -37,6 +37,20 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(workload, walkers):
+    """HBM bytes per k_block launch from the two PMC passes (FETCH_SIZE, WRITE_SIZE,
+    each scaled by a 1 GiB copy) recorded under profiles/ for this workload; None
+    when no such record exists for this configuration."""
+    path = os.path.join(ROOT, "profiles", f"k_block_{workload}_traffic.json")
+    try:
+        rec = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    if rec.get("walkers") != walkers:
+        return None
+    return rec.get("traffic_bytes_per_launch")
+
+
 def cpu_baseline(workload_dir, walkers, threads):
     """Oracle leg: one untimed warm-up SR step, then one timed steady step."""
     import oracle_py
@@ -146,8 +160,9 @@ def main():
         "acceptance": accs / max(props, 1),
         "energy_per_site": [r[0] / info["nsite"] for r in rows],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_block", "avg_launch_ms": avg_launch_s * 1e3, "launches": launches},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload, walkers),
+                     "kernel": "k_block", "avg_launch_ms": avg_launch_s * 1e3, "launches": launches,
+                     "algorithmic_bytes_per_launch": bytes_total / max(launches, 1)},
     }
     if not args.no_cpu:
         try:
