@@ -99,10 +99,15 @@ int mvmc_ens_timings(mvmc_ens* e, double* timings8);
  * the 100 MHz constant clock): 0 candidate, 1 projection deltas + ratio,
  * 2 inverse/Slater row staging, 3 Pfaffian ratio dots, 4 Metropolis,
  * 5 accepted-hop inverse pass, 6 periodic rebuild, 7 exchange path,
- * 8 block end.  Only libraries built with MVMC_PHASE_CLOCKS collect them;
- * others return MVMC_E_UNAVAILABLE.  No reference counterpart (profiling aid,
- * like the reference's timer sections in crates/mvmc-core/src/timer.rs). */
-int mvmc_ens_phase_clocks(mvmc_ens* e, long long* clocks16);
+ * 8 block end; then of the periodic rebuild's Pfaffian/inverse kernel:
+ * 16 pivot search, 17 deferred scaling + trailing catch-up, 18 swaps,
+ * 19 panel record + eager columns, 20 panel close, 25 Pfaffian, 21 inverse
+ * of the unit triangle, 22 triangular solve, 23 triangular product and
+ * skew fill, 24 permutations.  clocks32 receives 32 values.  Only libraries
+ * built with MVMC_PHASE_CLOCKS collect them; others return
+ * MVMC_E_UNAVAILABLE.  No reference counterpart (profiling aid, like the
+ * reference's timer sections in crates/mvmc-core/src/timer.rs). */
+int mvmc_ens_phase_clocks(mvmc_ens* e, long long* clocks32);
 /* Sticky device status word (0 ok; 1 init retries exhausted, 2 bad input,
  * 3 rebuild failure, 4 non-finite, 5 CG failure). */
 int mvmc_ens_status(mvmc_ens* e);

@@ -423,15 +423,6 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
   HIPCHK(hipFuncSetAttribute((const void*)k_block, hipFuncAttributeMaxDynamicSharedMemorySize, (int)E->lds_samp));
   HIPCHK(hipFuncSetAttribute((const void*)k_begin, hipFuncAttributeMaxDynamicSharedMemorySize, (int)E->lds_samp));
   HIPCHK(hipFuncSetAttribute((const void*)k_measure, hipFuncAttributeMaxDynamicSharedMemorySize, (int)E->lds_meas));
-  // rebuild kernels: as many planes as fit in LDS next to their scratch
-  // (two workgroups per CU), else the eliminations run in place in HBM
-  {
-    const size_t budget = 72 * 1024, base = lds_bytes_rebuild_base(v);
-    const size_t per = (size_t)v.n * v.n * sizeof(double);
-    // only when every plane fits: the planes then stay in lock step (one
-    // barrier per step for all of them); chunks would serialise the steps
-    v.rb_np = (budget > base && (budget - base) / per >= (size_t)v.nqp) ? v.nqp : 0;
-  }
   E->lds_rebuild = lds_bytes_rebuild(v);
   HIPCHK(hipFuncSetAttribute((const void*)k_rebuild, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)E->lds_rebuild));
@@ -453,8 +444,8 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
   v.pend = dalloc<double>(A, (size_t)E->slots * 2 * v.nqp * v.n);
   v.nflag = dalloc<int>(A, 1);
 #ifdef MVMC_PHASE_CLOCKS
-  v.clk = dalloc<long long>(A, (size_t)E->slots * 16);
-  HIPCHK(hipMemset(v.clk, 0, sizeof(long long) * E->slots * 16));
+  v.clk = dalloc<long long>(A, (size_t)E->slots * 32);
+  HIPCHK(hipMemset(v.clk, 0, sizeof(long long) * E->slots * 32));
 #else
   v.clk = nullptr;
 #endif
@@ -821,11 +812,11 @@ int mvmc_ens_get_cg(mvmc_ens* e, double* x, int* n_active) {
 int mvmc_ens_phase_clocks(mvmc_ens* e, long long* out) {
   return guarded([&] {
     if (!e->v.clk) throw Fail(MVMC_E_UNAVAILABLE, "library built without MVMC_PHASE_CLOCKS");
-    std::vector<long long> h((size_t)e->slots * 16);
+    std::vector<long long> h((size_t)e->slots * 32);
     HIPCHK(hipMemcpy(h.data(), e->v.clk, sizeof(long long) * h.size(), hipMemcpyDeviceToHost));
-    for (int k = 0; k < 16; ++k) out[k] = 0;
+    for (int k = 0; k < 32; ++k) out[k] = 0;
     for (int sl = 0; sl < e->slots; ++sl)
-      for (int k = 0; k < 16; ++k) out[k] += h[(size_t)sl * 16 + k];
+      for (int k = 0; k < 32; ++k) out[k] += h[(size_t)sl * 32 + k];
   });
 }
 

@@ -15,7 +15,6 @@ struct EnsView {
   int ntrans, nintra, ninter, nhund, nexch;
   int orb_injective;  // each orbital-index row is injective
   int exp_cr;         // 1: correctly rounded exp/log (always on the device)
-  int rb_np;          // planes per LDS-resident chunk in the rebuild kernels (0: in place in HBM)
   // model tables
   const double* slater;   // [nqp][n2][n2]
   const double* qpw;      // [nqp]

@@ -108,11 +108,9 @@ __device__ inline int block_exscan(int v, int* scan, int* total) {
 // calc_m_all for one walker: assemble, LTL, Pfaffian, inverse, negate.
 // native = true skips the all-zero guard (C's separate real setup).
 // Returns 0 on success, 1 if any plane failed.
-// lds_pl: when non-null, room for v.rb_np planes in LDS; the eliminations
-// then run on LDS copies of the assembled planes, rb_np planes at a time
-// (each plane's operation sequence is unchanged: planes are independent).
+// clk: optional phase clocks of the elimination (profiling builds).
 __device__ int wg_calc_m_all(const EnsView& v, const int* ele_idx_l, int* rs, double* planes, double* work,
-                             double* pf_l, char* pfscratch, bool native, double* lds_pl = nullptr) {
+                             double* pf_l, char* pfscratch, bool native, long long* clk = nullptr) {
   const int tid = threadIdx.x;
   __shared__ int bad_site;
   __shared__ double wave_max[16];
@@ -158,21 +156,7 @@ __device__ int wg_calc_m_all(const EnsView& v, const int* ele_idx_l, int* rs, do
     }
     if (allzero_flag) return 1;
   }
-  if (lds_pl && v.rb_np > 0) {
-    const int nn = v.n * v.n;
-    int bad = 0;
-    for (int q0 = 0; q0 < v.nqp; q0 += v.rb_np) {
-      const int cnt = min(v.rb_np, v.nqp - q0);
-      for (int e = tid; e < cnt * nn; e += blockDim.x) lds_pl[e] = planes[(size_t)q0 * nn + e];
-      __syncthreads();
-      wg_pfinv(lds_pl, work, v.n, cnt, sh, pf_l + q0, -1.0);
-      for (int p = 0; p < cnt; ++p) bad |= (sh.info[p] != 0);
-      for (int e = tid; e < cnt * nn; e += blockDim.x) planes[(size_t)q0 * nn + e] = lds_pl[e];
-      __syncthreads();
-    }
-    return bad;
-  }
-  wg_pfinv(planes, work, v.n, v.nqp, sh, pf_l, -1.0);
+  wg_pfinv(planes, work, v.n, v.nqp, sh, pf_l, -1.0, clk);
   int bad = 0;
   for (int p = 0; p < v.nqp; ++p) bad |= (sh.info[p] != 0);
   __syncthreads();
@@ -680,7 +664,7 @@ __device__ void stream_planes(double* planes, int n, int nqp, const double* pv1,
 #ifdef MVMC_PHASE_CLOCKS
 #define PHASE_INIT                   \
   long long clk_t0 = wall_clock64(); \
-  long long* clk = v.clk + (size_t)blockIdx.x * 16;
+  long long* clk = v.clk + (size_t)blockIdx.x * 32;
 #define PHASE(k)                           \
   do {                                     \
     if (threadIdx.x == 0) {                \
@@ -1129,11 +1113,8 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v, int out_step, int n
 // kernel of its own so the elimination's register budget does not limit the
 // sampler's occupancy.
 // ---------------------------------------------------------------------------
-__host__ __device__ inline size_t lds_bytes_rebuild_base(const EnsView& v) {
-  return 2 * rnd16((size_t)v.n * 4) + rnd16((size_t)v.nqp * 8) + rnd16(pf_bytes(v.n, v.nqp));
-}
 __host__ __device__ inline size_t lds_bytes_rebuild(const EnsView& v) {
-  return lds_bytes_rebuild_base(v) + rnd16((size_t)v.rb_np * v.n * v.n * 8);
+  return 2 * rnd16((size_t)v.n * 4) + rnd16((size_t)v.nqp * 8) + rnd16(pf_bytes(v.n, v.nqp));
 }
 
 __global__ void __launch_bounds__(256) k_rebuild(EnsView v) {
@@ -1143,7 +1124,6 @@ __global__ void __launch_bounds__(256) k_rebuild(EnsView v) {
   int* rs = c.take<int>(v.n);
   double* pfn = c.take<double>(v.nqp);
   char* pfscr = (char*)c.take<char>(pf_bytes(v.n, v.nqp));
-  double* lds_pl = v.rb_np > 0 ? c.take<double>((size_t)v.rb_np * v.n * v.n) : nullptr;
   const int tid = threadIdx.x;
   double* work = v.work + (size_t)blockIdx.x * pfinv_work_doubles(v.n, v.nqp);
   for (int w = blockIdx.x; w < v.W; w += gridDim.x) {
@@ -1155,7 +1135,8 @@ __global__ void __launch_bounds__(256) k_rebuild(EnsView v) {
     for (int i = tid; i < v.n; i += blockDim.x) ei[i] = v.ele_idx[(size_t)w * v.n + i];
     __syncthreads();
     double* planes = v.inv + (size_t)w * v.nqp * v.n * v.n;
-    const int bad = wg_calc_m_all(v, ei, rs, planes, work, pfn, pfscr, false, lds_pl);
+    const int bad = wg_calc_m_all(v, ei, rs, planes, work, pfn, pfscr, false,
+                                  v.clk ? v.clk + (size_t)blockIdx.x * 32 + 16 : nullptr);
     if (tid == 0) {
       if (!bad) {
         for (int q = 0; q < v.nqp; ++q) v.pf[(size_t)w * v.nqp + q] = pfn[q];
@@ -1214,7 +1195,6 @@ __global__ void __launch_bounds__(256) k_meas_rebuild(EnsView v, int s) {
   int* rs = c.take<int>(v.n);
   double* pfn = c.take<double>(v.nqp);
   char* pfscr = (char*)c.take<char>(pf_bytes(v.n, v.nqp));
-  double* lds_pl = v.rb_np > 0 ? c.take<double>((size_t)v.rb_np * v.n * v.n) : nullptr;
   const int tid = threadIdx.x;
   double* work = v.work + (size_t)blockIdx.x * pfinv_work_doubles(v.n, v.nqp);
   __shared__ int flag;
@@ -1231,7 +1211,7 @@ __global__ void __launch_bounds__(256) k_meas_rebuild(EnsView v, int s) {
     int valid = flag;
     if (valid) {
       double* planes = v.inv + (size_t)w * v.nqp * v.n * v.n;
-      valid = !wg_calc_m_all(v, ei, rs, planes, work, pfn, pfscr, false, lds_pl);
+      valid = !wg_calc_m_all(v, ei, rs, planes, work, pfn, pfscr, false);
     }
     if (tid == 0) {
       v.s_valid[ws] = valid;

@@ -98,8 +98,52 @@ __device__ inline void assemble_planes(double* planes, const double* slater, con
 // round trips, and a load after a store to the same buffer could not be
 // hoisted by the compiler.
 constexpr int PF_U = 8;
-__device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShared& sh, double* pf, double sign) {
-  const int tid = threadIdx.x, nthr = blockDim.x;
+// Opaque copies of a wave-uniform value / pointer: the compiler cannot prove
+// them equal to the original, so nothing derived from them is hoisted above
+// this point (used at the entry of routines called in a loop, whose
+// loop-invariant address arithmetic would otherwise stay live across the
+// whole loop and cost occupancy).
+__device__ inline int opaque_uniform(int x) {
+  asm volatile("" : "+v"(x));
+  return __builtin_amdgcn_readfirstlane(x);
+}
+template <class T>
+__device__ inline T* opaque_uniform(T* p) {
+  const unsigned long long u = (unsigned long long)p;
+  const unsigned lo = (unsigned)opaque_uniform((int)(unsigned)u);
+  const unsigned hi = (unsigned)opaque_uniform((int)(unsigned)(u >> 32));
+  return (T*)(((unsigned long long)hi << 32) | lo);
+}
+
+__device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShared& sh_in, double* pf, double sign,
+                         long long* clk = nullptr) {
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  const int nthr = opaque_uniform((int)blockDim.x);
+  n = opaque_uniform(n);
+  np = opaque_uniform(np);
+  planes = opaque_uniform(planes);
+  work = opaque_uniform(work);
+  pf = opaque_uniform(pf);
+  PfinvShared sh = sh_in;
+  sh.piv = opaque_uniform(sh.piv);
+  sh.colbuf = opaque_uniform(sh.colbuf);
+  sh.kp = opaque_uniform(sh.kp);
+  sh.info = opaque_uniform(sh.info);
+  sh.colmax = opaque_uniform(sh.colmax);
+  sh.ks = opaque_uniform(sh.ks);
+  sh.act = opaque_uniform(sh.act);
+  sh.zmax = opaque_uniform(sh.zmax);
+  sh.ld = opaque_uniform(sh.ld);
+  // optional phase clocks (lane 0, after the barrier that ends a phase)
+  long long ck0 = clk ? wall_clock64() : 0;
+  auto PFC = [&](int k) {
+    if (clk && tid == 0) {
+      const long long t_ = wall_clock64();
+      clk[k] += t_ - ck0;
+      ck0 = t_;
+    }
+  };
   const int lane = tid & 63, wave = tid >> 6, nwave = (nthr + 63) >> 6;
   const int nn = n * n;
   for (int e = tid; e < np * n; e += nthr) sh.P(e / n, e % n) = e % n + 1;
@@ -189,6 +233,7 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
       }
     }
     __syncthreads();
+    PFC(0);
     if (tp == 0) m_pan = max(0, kk0 - kw + 1);
     const int m = m_pan;  // trailing columns j < m; eager columns [m, kk0)
     // (B1) scale the previous x column (rows < k0)
@@ -226,6 +271,7 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
       }
     }
     __syncthreads();
+    PFC(1);
     // (C) swaps and sign flips (ltl.rs:131-158), items (plane, e), e < n - 1:
     //   e < kp: a[e][kk0] <-> a[e][kp];  e == kp: a[kp][kk0] = -a[kp][kk0];
     //   kp < e < kk0: a[e][kk0] <-> a[kp][e], both negated;
@@ -279,6 +325,7 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
       if (on && kk0 > sh.zmax[p]) sh.zmax[p] = kk0;
     }
     __syncthreads();
+    PFC(2);
     // (D) record the step (x, y unscaled, t1 = alpha*y[j], t2 = alpha*x[j])
     // and apply it to the eager columns [m, kk0): a lane owns row i of plane
     // p, loads x_i, y_i and its eager elements together; x_j, y_j are
@@ -322,6 +369,7 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
         }
       }
       __syncthreads();
+      PFC(3);
       scale_col = k0;
     } else {
       scale_col = -1;
@@ -329,13 +377,13 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
     for (int p = tid; p < np; p += nthr) sh.zmax[np + p] = sh.colmax[p] != 0.0 && kk0 >= 1;  // scale_on
     ++tp;
     // (E) close the panel: every element of the trailing triangle (i < j < kk0)
-    // takes its pending steps; work item = (plane, 64-row chunk) per wave, so
-    // plane and column are wave-uniform; a lane keeps its row's x, y of all
-    // panel steps in registers.
+    // takes its pending steps.
     if (tp == kw || kk0 <= 1) {
       __syncthreads();
       const int m = kk0;  // trailing columns j < m, rows i < m - 1
       if (m >= 2) {
+        // work item = (plane, 64-row chunk) per wave, the step scalars of a
+        // column are wave-uniform loads
         const int chunks = (m - 1 + 63) / 64;
         for (int it = __builtin_amdgcn_readfirstlane(wave); it < np * chunks; it += nwave) {
           const int p = it / chunks, c = it % chunks;
@@ -352,36 +400,23 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
           }
           const int ksi = on ? sh.KS(p, i) : tp;
           double* a = planes + (size_t)p * nn;
-          constexpr int JU = 1;
-          for (int j = c * 64 + 1; j < m; j += JU) {
-            double t1v[JU][LTL_K], t2v[JU][LTL_K], cv[JU];
-            int ksj[JU];
+          for (int j = c * 64 + 1; j < m; ++j) {
+            double t1v[LTL_K], t2v[LTL_K];
+            const int ksj = sh.KS(p, j);
 #pragma unroll
-            for (int u = 0; u < JU; ++u) {
-              const int jj = j + u;
-              if (jj < m) {
-                ksj[u] = sh.KS(p, jj);
+            for (int t = 0; t < LTL_K; ++t)
+              if (t < tp) {
+                t1v[t] = PT1(p, t)[j];
+                t2v[t] = PT2(p, t)[j];
+              }
+            if (on && j > i) {
+              const int s0 = max(ksi, ksj);
+              if (s0 < tp) {
+                double cc = a[i + (size_t)j * n];
 #pragma unroll
                 for (int t = 0; t < LTL_K; ++t)
-                  if (t < tp) {
-                    t1v[u][t] = PT1(p, t)[jj];
-                    t2v[u][t] = PT2(p, t)[jj];
-                  }
-                if (on && jj > i) cv[u] = a[i + (size_t)jj * n];
-              }
-            }
-#pragma unroll
-            for (int u = 0; u < JU; ++u) {
-              const int jj = j + u;
-              if (jj < m && on && jj > i) {
-                const int s0 = max(ksi, ksj[u]);
-                if (s0 < tp) {
-                  double cc = cv[u];
-#pragma unroll
-                  for (int t = 0; t < LTL_K; ++t)
-                    if (t >= s0 && ar[t]) cc = (cc + xr[t] * t1v[u][t]) - yr[t] * t2v[u][t];
-                  a[i + (size_t)jj * n] = cc;
-                }
+                  if (t >= s0 && ar[t]) cc = (cc + xr[t] * t1v[t]) - yr[t] * t2v[t];
+                a[i + (size_t)j * n] = cc;
               }
             }
           }
@@ -391,6 +426,7 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
       for (int e = tid; e < np * n; e += nthr) sh.ks[e] = 0;
       tp = 0;
       __syncthreads();
+      PFC(4);
     }
   }
   // the unblocked elimination zeroes the diagonal of every block it updated
@@ -416,6 +452,7 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
     }
   }
   __syncthreads();
+  PFC(9);
   // ---------------- inverse (utu2inv) ----------------
   // Planes one at a time; a lane owns a row and TJ columns of a column
   // block, whose original column values are staged in `stage` (global,
@@ -490,6 +527,7 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
       }
       __syncthreads();
     }
+    PFC(5);
     // Mt = M^T with M = I, M(ir, jr+1) = A(ir, jr+2) for ir <= jr < n-2;
     // vt(i) = -A(i, i+1)
     double* mt = work + (size_t)p * nn;
@@ -550,6 +588,7 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
       }
     }
     __syncthreads();
+    PFC(6);
     // T = M^T C: T(i,j) = C(i,j) + sum_{k<i} M(k,i) C(k,j), k ascending.
     // For n > 64 the reference forms only the upper triangle and restores
     // skew symmetry; for n <= 64 it forms every element.
@@ -621,6 +660,7 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
       __syncthreads();
     }
   }
+  PFC(7);
   // column then row permutations: the sequential swaps (j <-> P[j]-1,
   // j ascending) compose into one index map g, applied to rows and columns
   // as a gather (pure data movement), then the sign (calc_m_all's final
@@ -667,6 +707,7 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
     }
   }
   __syncthreads();
+  PFC(8);
 }
 
 }  // namespace mvmc_hip

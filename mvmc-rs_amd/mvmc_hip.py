@@ -183,14 +183,19 @@ class Ensemble:
 
     PHASES = ("candidate", "projection", "staging", "ratio", "metropolis", "accept_pass", "rebuild", "exchange",
               "block_end")
+    REBUILD_PHASES = ("pf_pivot", "pf_scale_catchup", "pf_swap", "pf_record_eager", "pf_close", "inv_trtri",
+                      "inv_solve", "inv_trmm_fill", "inv_perm", "pfaffian")
 
     def phase_clocks(self):
-        """Per-phase device seconds of the sampling kernel (summed over walkers);
-        None unless the library was built with MVMC_PHASE_CLOCKS."""
-        c = np.zeros(16, dtype=np.int64)
+        """Per-phase device seconds of the sampling kernel and of the periodic
+        rebuild's Pfaffian/inverse (summed over walkers); None unless the
+        library was built with MVMC_PHASE_CLOCKS."""
+        c = np.zeros(32, dtype=np.int64)
         if lib().mvmc_ens_phase_clocks(self.h, _p(c)) != 0:
             return None
-        return {k: c[i] / 1e8 for i, k in enumerate(self.PHASES)}
+        out = {k: c[i] / 1e8 for i, k in enumerate(self.PHASES)}
+        out.update({k: c[16 + i] / 1e8 for i, k in enumerate(self.REBUILD_PHASES)})
+        return out
 
     def status(self):
         return int(lib().mvmc_ens_status(self.h))
