@@ -109,7 +109,7 @@ __device__ inline int block_exscan(int v, int* scan, int* total) {
 // native = true skips the all-zero guard (C's separate real setup).
 // Returns 0 on success, 1 if any plane failed.
 // clk: optional phase clocks of the elimination (profiling builds).
-__device__ int wg_calc_m_all(const EnsView& v, const int* ele_idx_l, int* rs, double* planes, double* work,
+__device__ __forceinline__ int wg_calc_m_all(const EnsView& v, const int* ele_idx_l, int* rs, double* planes, double* work,
                              double* pf_l, char* pfscratch, bool native, long long* clk = nullptr) {
   const int tid = threadIdx.x;
   __shared__ int bad_site;

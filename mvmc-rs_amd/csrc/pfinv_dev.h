@@ -19,17 +19,28 @@ namespace mvmc_hip {
 constexpr int LTL_K = 8;
 // Columns per block of the row-owner passes of the inverse.
 constexpr int TJ = 8;
+// Most waves per workgroup wg_pfinv is called with (blockDim.x <= 256).
+constexpr int PF_WAVES = 4;
+
+// Orders the LDS accesses of one wave around a lane-to-lane exchange (LDS
+// operations of a wave execute in issue order; this keeps the compiler from
+// moving them across).
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // Doubles of global scratch wg_pfinv needs for np planes of order n.
 __host__ __device__ inline size_t pfinv_work_doubles(int n, int np) {
-  return (size_t)np * n * n + (size_t)TJ * n;
+  return (size_t)np * n * n;
 }
 
 // LDS scratch of wg_pfinv, carved from dynamic shared memory by the caller:
 // piv[np*n], colbuf[np*n], kp/info[np], colmax[np], ks[np*n] (panel step of
 // the last swap through each index), act[np*LTL_K] (step applied per plane),
 // zmax[2*np] (extent of the zeroed diagonal; whether the last x column
-// awaits scaling).
+// awaits scaling), wb[PF_WAVES*64] (one broadcast buffer per wave).
 struct PfinvShared {
   int* piv;
   double* colbuf;
@@ -39,6 +50,7 @@ struct PfinvShared {
   int* ks;
   int* act;
   int* zmax;
+  double* wb;
   __device__ int& P(int p, int i) { return piv[p * ld + i]; }
   __device__ double& C(int p, int i) { return colbuf[p * ld + i]; }
   __device__ int& KS(int p, int i) { return ks[p * ld + i]; }
@@ -50,7 +62,8 @@ __host__ __device__ inline size_t pf_rnd16(size_t b) { return (b + 15) & ~(size_
 // Bytes of LDS that carve_pfinv takes.
 __host__ __device__ inline size_t pf_bytes(int n, int np) {
   return 2 * pf_rnd16((size_t)np * n * sizeof(int)) + pf_rnd16((size_t)np * n * sizeof(double)) +
-         3 * pf_rnd16((size_t)np * 8) + pf_rnd16((size_t)np * LTL_K * sizeof(int)) + pf_rnd16(2 * (size_t)np * sizeof(int));
+         3 * pf_rnd16((size_t)np * 8) + pf_rnd16((size_t)np * LTL_K * sizeof(int)) + pf_rnd16(2 * (size_t)np * sizeof(int)) +
+         pf_rnd16((size_t)PF_WAVES * 64 * sizeof(double));
 }
 
 template <class Carver>
@@ -64,6 +77,7 @@ __device__ inline void carve_pfinv(Carver& c, PfinvShared& sh, int n, int np) {
   sh.ks = c.template take<int>((size_t)np * n);
   sh.act = c.template take<int>((size_t)np * LTL_K);
   sh.zmax = c.template take<int>(2 * (size_t)np);
+  sh.wb = c.template take<double>((size_t)PF_WAVES * 64);
 }
 
 // Argmax of |x| over a column with C IDAMAX semantics: first index of the
@@ -87,11 +101,12 @@ __device__ inline void assemble_planes(double* planes, const double* slater, con
 }
 
 // Core routine.  planes: np planes (stride n*n).  work: pfinv_work_doubles
-// of scratch (panel vectors of the elimination, then M^T, a staged column
-// block and the permutation gather).  On exit, for every plane with
-// info == 0, pf[p] holds the Pfaffian and the plane holds sign * A^{-1}
-// (sign = -1 gives mVMC's invM).  info[p]: 0 ok, k > 0 zero pivot at 1-based
-// row k, -1 non-finite Pfaffian.  All threads of the block must call it.
+// of scratch (panel vectors of the elimination, then M^T of every plane,
+// then the permutation gather).  On exit, for every plane with info == 0,
+// pf[p] holds the Pfaffian and the plane holds sign * A^{-1} (sign = -1
+// gives mVMC's invM).  info[p]: 0 ok, k > 0 zero pivot at 1-based row k, -1
+// non-finite Pfaffian.  All threads of the block (at most PF_WAVES waves)
+// must call it.
 //
 // Every phase is written so that a lane issues the loads of all its items
 // (PF_U per batch) before its first store: the work is bound by memory
@@ -115,11 +130,11 @@ __device__ inline T* opaque_uniform(T* p) {
   return (T*)(((unsigned long long)hi << 32) | lo);
 }
 
-__device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShared& sh_in, double* pf, double sign,
+__device__ __forceinline__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShared& sh_in, double* pf, double sign,
                          long long* clk = nullptr) {
   int tid = threadIdx.x;
   asm volatile("" : "+v"(tid));
-  const int nthr = opaque_uniform((int)blockDim.x);
+  int nthr = opaque_uniform((int)blockDim.x);
   n = opaque_uniform(n);
   np = opaque_uniform(np);
   planes = opaque_uniform(planes);
@@ -134,6 +149,7 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
   sh.ks = opaque_uniform(sh.ks);
   sh.act = opaque_uniform(sh.act);
   sh.zmax = opaque_uniform(sh.zmax);
+  sh.wb = opaque_uniform(sh.wb);
   sh.ld = opaque_uniform(sh.ld);
   // optional phase clocks (lane 0, after the barrier that ends a phase)
   long long ck0 = clk ? wall_clock64() : 0;
@@ -144,8 +160,24 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
       ck0 = t_;
     }
   };
-  const int lane = tid & 63, wave = tid >> 6, nwave = (nthr + 63) >> 6;
-  const int nn = n * n;
+  int lane = tid & 63, wave = tid >> 6, nwave = (nthr + 63) >> 6;
+  int nn = n * n;
+  // Fresh opaque copies at the start of each phase: expressions the phases
+  // share (item decompositions, addresses) are then computed per phase
+  // instead of once and kept live across the others.
+#define PF_FRESH()                                 \
+  do {                                             \
+    asm volatile("" : "+v"(tid));                  \
+    nthr = opaque_uniform(nthr);                   \
+    n = opaque_uniform(n);                         \
+    np = opaque_uniform(np);                       \
+    planes = opaque_uniform(planes);               \
+    work = opaque_uniform(work);                   \
+    lane = tid & 63;                               \
+    wave = tid >> 6;                               \
+    nwave = (nthr + 63) >> 6;                      \
+    nn = n * n;                                    \
+  } while (0)
   for (int e = tid; e < np * n; e += nthr) sh.P(e / n, e % n) = e % n + 1;
   for (int p = tid; p < np; p += nthr) sh.info[p] = 0;
   __syncthreads();
@@ -200,6 +232,7 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
   auto scale_on = [&](int p) { return sh.zmax[np + p] != 0; };
   for (int k0 = n - 1; k0 >= 1; --k0) {
     const int kk0 = k0 - 1;
+    PF_FRESH();
     // (A) pivot search on the (up to date) column k0: one wave per plane
     for (int p = wave; p < np; p += nwave) {
       const double* col = planes + (size_t)p * nn + (size_t)k0 * n;
@@ -236,6 +269,7 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
     PFC(0);
     if (tp == 0) m_pan = max(0, kk0 - kw + 1);
     const int m = m_pan;  // trailing columns j < m; eager columns [m, kk0)
+    PF_FRESH();
     // (B1) scale the previous x column (rows < k0)
     if (scale_col >= 0) {
       for (int b0 = 0; b0 < np * k0; b0 += nthr * PF_U) {
@@ -253,6 +287,7 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
         }
       }
     }
+    PF_FRESH();
     // (B2) bring the trailing elements a swap moves up to date: items
     // (plane, f), f < m - 1: f < kp: (f, kp), else (kp, f + 1)  [kp < m]
     if (tp > 0 && m >= 2) {
@@ -272,6 +307,7 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
     }
     __syncthreads();
     PFC(1);
+    PF_FRESH();
     // (C) swaps and sign flips (ltl.rs:131-158), items (plane, e), e < n - 1:
     //   e < kp: a[e][kk0] <-> a[e][kp];  e == kp: a[kp][kk0] = -a[kp][kk0];
     //   kp < e < kk0: a[e][kk0] <-> a[kp][e], both negated;
@@ -311,6 +347,7 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
         }
       }
     }
+    PF_FRESH();
     if (tid < np) {
       const int p = tid;
       if (sh.colmax[p] == 0.0) {
@@ -326,6 +363,7 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
     }
     __syncthreads();
     PFC(2);
+    PF_FRESH();
     // (D) record the step (x, y unscaled, t1 = alpha*y[j], t2 = alpha*x[j])
     // and apply it to the eager columns [m, kk0): a lane owns row i of plane
     // p, loads x_i, y_i and its eager elements together; x_j, y_j are
@@ -376,6 +414,7 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
     }
     for (int p = tid; p < np; p += nthr) sh.zmax[np + p] = sh.colmax[p] != 0.0 && kk0 >= 1;  // scale_on
     ++tp;
+    PF_FRESH();
     // (E) close the panel: every element of the trailing triangle (i < j < kk0)
     // takes its pending steps.
     if (tp == kw || kk0 <= 1) {
@@ -429,12 +468,14 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
       PFC(4);
     }
   }
+  PF_FRESH();
   // the unblocked elimination zeroes the diagonal of every block it updated
   for (int e = tid; e < np * n; e += nthr) {
     const int p = e / n, j = e % n;
     if (j < sh.zmax[p]) planes[(size_t)p * nn + j + (size_t)j * n] = 0.0;
   }
   __syncthreads();
+  PF_FRESH();
   // ---------------- Pfaffian (utu2pfa) ----------------
   if (tid < np) {
     const int p = tid;
@@ -453,214 +494,256 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
   }
   __syncthreads();
   PFC(9);
+  PF_FRESH();
   // ---------------- inverse (utu2inv) ----------------
-  // Planes one at a time; a lane owns a row and TJ columns of a column
-  // block, whose original column values are staged in `stage` (global,
-  // read by whole waves at one address) so results can be written in place.
-  double* stage = work + (size_t)np * nn;
-  for (int p = 0; p < np; ++p) {
-    if (sh.info[p] != 0) continue;
-    double* a = planes + (size_t)p * nn;
-    // opaque per-plane copies of the sizes keep the item decompositions of
-    // the batched loops inside the plane loop (not hoisted and kept live)
-    int nno = nn, tjn = TJ * n;
-    asm volatile("" : "+v"(nno), "+v"(tjn));
-    nno = __builtin_amdgcn_readfirstlane(nno);
-    tjn = __builtin_amdgcn_readfirstlane(tjn);
-    // trtri('U','U') on B(i,j) = A(i,j+1), 0 <= i < j < n-1:
-    // Bn(i,j) = -(B(i,j) + sum_{k=i+1}^{j-1} Bn(i,k) B(k,j)), k ascending.
-    for (int j0 = 1; j0 < n - 1; j0 += TJ) {
-      const int j1 = min(n - 1, j0 + TJ);
-      for (int b0 = 0; b0 < tjn; b0 += nthr * PF_U) {
-        double sv[PF_U];
+  // (1) trtri('U','U') on B(i,j) = A(i,j+1), 0 <= i < j < n-1, in place:
+  //   Bn(i,j) = -(B(i,j) + sum_{k=i+1}^{j-1} Bn(i,k) B(k,j)), k ascending.
+  // Rows are independent chains.  One wave per plane (no workgroup
+  // barriers); a lane owns rows g0 + lane + 64 r (r < TR) of a block of TJ
+  // columns.  The B values a row reads from other rows (B(k,j), k > i) are
+  // all read before the wave writes the block, and row groups go in
+  // ascending order, so no in-place write precedes a read of the value it
+  // replaces.  The B block of each batch of KB steps is one coalesced wave
+  // load, broadcast to the lanes through the wave's LDS buffer.
+  {
+    constexpr int TR = 4, KB = 4;  // KB * TJ <= 64
+    double* wb = sh.wb + wave * 64;
+    for (int p = __builtin_amdgcn_readfirstlane(wave); p < np; p += nwave) {
+      if (sh.info[p] != 0) continue;
+      double* a = planes + (size_t)p * nn;
+      for (int j0 = 1; j0 < n - 1; j0 += TJ) {
+        const int j1 = min(n - 1, j0 + TJ);
+        for (int g0 = 0; g0 < j1 - 1; g0 += 64 * TR) {
+          int ir[TR];
+          bool on[TR];
+          double acc[TR][TJ];
 #pragma unroll
-        for (int u = 0; u < PF_U; ++u) {
-          const int e = b0 + tid + nthr * u;
-          const int jj = e / n, k = e % n;
-          sv[u] = (e < tjn && j0 + jj < j1 && k < j0 + jj) ? a[k + (size_t)(j0 + jj + 1) * n] : 0.0;
-        }
+          for (int r = 0; r < TR; ++r) {
+            ir[r] = g0 + lane + 64 * r;
+            on[r] = ir[r] < j1 - 1;
 #pragma unroll
-        for (int u = 0; u < PF_U; ++u) {
-          const int e = b0 + tid + nthr * u;
-          if (e < tjn) stage[e] = sv[u];
-        }
-      }
-      __syncthreads();
-      for (int i0 = 0; i0 < j1 - 1; i0 += nthr) {
-        const int i = i0 + tid;
-        const bool on = i < j1 - 1;
-        double acc[TJ];
+            for (int jj = 0; jj < TJ; ++jj)
+              acc[r][jj] = (on[r] && j0 + jj < j1 && ir[r] < j0 + jj) ? (a + (size_t)(j0 + jj + 1) * n)[ir[r]] : 0.0;
+          }
+          // k < j0: finished columns of earlier blocks
+          const int bk = lane % KB, bj = lane / KB;
+          for (int k = g0 + 1; k < j0; k += KB) {
+            const double bv = (bj < TJ && k + bk < j0 && j0 + bj < j1) ? (a + (size_t)(j0 + bj + 1) * n)[k + bk] : 0.0;
+            double bn[TR][KB];
 #pragma unroll
-        for (int jj = 0; jj < TJ; ++jj) acc[jj] = (on && j0 + jj < j1 && i < j0 + jj) ? stage[jj * n + i] : 0.0;
-        // k < j0: finished columns of earlier blocks (KU loads issued together)
-        const int kst = __builtin_amdgcn_readfirstlane(i0 + (tid & ~63) + 1);  // wave-uniform start
-        constexpr int KU = 8;
-        for (int k = kst; k < j0; k += KU) {
-          double bn[KU];
+            for (int r = 0; r < TR; ++r)
 #pragma unroll
-          for (int u = 0; u < KU; ++u) {
-            const int kk = k + u;
-            bn[u] = (on && kk < j0 && kk > i) ? a[i + (size_t)(kk + 1) * n] : 0.0;
+              for (int u = 0; u < KB; ++u)
+                bn[r][u] = (on[r] && k + u > ir[r] && k + u < j0) ? (a + (size_t)(k + u + 1) * n)[ir[r]] : 0.0;
+            wave_sync();
+            wb[lane] = bv;  // wb[u + KB*jj] = B(k+u, j0+jj)
+            wave_sync();
+#pragma unroll
+            for (int u = 0; u < KB; ++u)
+#pragma unroll
+              for (int r = 0; r < TR; ++r)
+                if (on[r] && k + u > ir[r] && k + u < j0) {
+#pragma unroll
+                  for (int jj = 0; jj < TJ; ++jj)
+                    if (j0 + jj < j1) acc[r][jj] += bn[r][u] * wb[u + KB * jj];
+                }
+          }
+          // k in [j0, j): the block's own columns, finished by each lane in order
+          {
+            const int kk = lane % TJ, jj = lane / TJ;  // wb[kk + TJ*jj] = B(j0+kk, j0+jj)
+            const double bv = (j0 + jj < j1 && kk < jj) ? (a + (size_t)(j0 + jj + 1) * n)[j0 + kk] : 0.0;
+            wave_sync();
+            wb[lane] = bv;
+            wave_sync();
           }
 #pragma unroll
-          for (int u = 0; u < KU; ++u) {
-            const int kk = k + u;
-            if (on && kk < j0 && kk > i) {
+          for (int jj = 0; jj < TJ; ++jj) {
 #pragma unroll
-              for (int jj = 0; jj < TJ; ++jj)
-                if (j0 + jj < j1) acc[jj] += bn[u] * stage[jj * n + kk];
-            }
+            for (int kk = 0; kk < jj; ++kk)
+#pragma unroll
+              for (int r = 0; r < TR; ++r)
+                if (on[r] && j0 + jj < j1 && ir[r] < j0 + kk) acc[r][jj] += acc[r][kk] * wb[kk + TJ * jj];
+#pragma unroll
+            for (int r = 0; r < TR; ++r) acc[r][jj] = -acc[r][jj];
           }
-        }
-        // k in [j0, j'): columns of this block, finished by this lane in order
+          double* ao = opaque_uniform(a);  // store addresses not shared with the loads above
 #pragma unroll
-        for (int jj = 0; jj < TJ; ++jj) {
+          for (int r = 0; r < TR; ++r)
 #pragma unroll
-          for (int kk = 0; kk < jj; ++kk)
-            if (on && j0 + jj < j1 && i < j0 + kk) acc[jj] += acc[kk] * stage[jj * n + j0 + kk];
-          acc[jj] = -acc[jj];
-        }
-        // in place: this block reads its own columns only from `stage`
-#pragma unroll
-        for (int jj = 0; jj < TJ; ++jj)
-          if (on && j0 + jj < j1 && i < j0 + jj) a[i + (size_t)(j0 + jj + 1) * n] = acc[jj];
-      }
-      __syncthreads();
-    }
-    PFC(5);
-    // Mt = M^T with M = I, M(ir, jr+1) = A(ir, jr+2) for ir <= jr < n-2;
-    // vt(i) = -A(i, i+1)
-    double* mt = work + (size_t)p * nn;
-    for (int b0 = 0; b0 < nno; b0 += nthr * PF_U) {
-      double mv[PF_U];
-#pragma unroll
-      for (int u = 0; u < PF_U; ++u) {
-        const int e = b0 + tid + nthr * u;
-        const int col = e % n, row = e / n;  // Mt(col, row) = M(row, col) at col + row*n
-        double v = (row == col) ? 1.0 : 0.0;
-        if (e < nno && col >= 1 && col <= n - 2 && row <= col - 1) v = a[row + (size_t)(col + 1) * n];
-        mv[u] = v;
-      }
-#pragma unroll
-      for (int u = 0; u < PF_U; ++u) {
-        const int e = b0 + tid + nthr * u;
-        if (e < nno) mt[e] = mv[u];
-      }
-    }
-    for (int i = tid; i < n - 1; i += nthr) sh.C(p, i) = -a[i + (size_t)(i + 1) * n];
-    __syncthreads();
-    // skew-tridiagonal solve T C = M, C overwrites A (column per lane; the
-    // recurrence carries its last value in a register, right-hand sides are
-    // loaded SU at a time ahead of it)
-    for (int j = tid; j < n; j += nthr) {
-      constexpr int SU = 8;
-      const double* vt = &sh.C(p, 0);
-      double* c = a + (size_t)j * n;
-      double prev = mt[j + 0 * (size_t)n] / -vt[0];
-      c[1] = prev;
-      for (int i0 = 2; i0 < n; i0 += 2 * SU) {
-        double bb[SU];
-#pragma unroll
-        for (int u = 0; u < SU; ++u) bb[u] = (i0 + 2 * u < n) ? mt[j + (size_t)(i0 + 2 * u) * n] : 0.0;
-#pragma unroll
-        for (int u = 0; u < SU; ++u) {
-          const int i = i0 + 2 * u;
-          if (i < n) {
-            prev = (bb[u] - prev * vt[i - 1]) / -vt[i];
-            c[i + 1] = prev;
-          }
+            for (int jj = 0; jj < TJ; ++jj)
+              if (on[r] && j0 + jj < j1 && ir[r] < j0 + jj) (ao + (size_t)(j0 + jj + 1) * n)[ir[r]] = acc[r][jj];
+          wave_sync();
         }
       }
-      prev = mt[j + (size_t)(n - 1) * n] / vt[n - 2];
-      c[n - 2] = prev;
-      for (int i0 = n - 3; i0 >= 1; i0 -= 2 * SU) {
-        double bb[SU];
-#pragma unroll
-        for (int u = 0; u < SU; ++u) bb[u] = (i0 - 2 * u >= 1) ? mt[j + (size_t)(i0 - 2 * u) * n] : 0.0;
-#pragma unroll
-        for (int u = 0; u < SU; ++u) {
-          const int i = i0 - 2 * u;
-          if (i >= 1) {
-            prev = (bb[u] + prev * vt[i]) / vt[i - 1];
-            c[i - 1] = prev;
-          }
-        }
-      }
-    }
-    __syncthreads();
-    PFC(6);
-    // T = M^T C: T(i,j) = C(i,j) + sum_{k<i} M(k,i) C(k,j), k ascending.
-    // For n > 64 the reference forms only the upper triangle and restores
-    // skew symmetry; for n <= 64 it forms every element.
-    const bool upper = n > 64;
-    for (int j0 = 0; j0 < n; j0 += TJ) {
-      const int j1 = min(n, j0 + TJ);
-      for (int b0 = 0; b0 < tjn; b0 += nthr * PF_U) {
-        double sv[PF_U];
-#pragma unroll
-        for (int u = 0; u < PF_U; ++u) {
-          const int e = b0 + tid + nthr * u;
-          const int jj = e / n, k = e % n;
-          sv[u] = (e < tjn && j0 + jj < j1) ? a[k + (size_t)(j0 + jj) * n] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < PF_U; ++u) {
-          const int e = b0 + tid + nthr * u;
-          if (e < tjn) stage[e] = sv[u];
-        }
-      }
-      __syncthreads();
-      const int rows = upper ? j1 - 1 : n;
-      for (int i0 = 0; i0 < rows; i0 += nthr) {
-        const int i = i0 + tid;
-        const bool on = i < rows;
-        double acc[TJ];
-#pragma unroll
-        for (int jj = 0; jj < TJ; ++jj) acc[jj] = on ? stage[jj * n + (on ? i : 0)] : 0.0;
-        const int kend = min(rows, i0 + (tid & ~63) + 64) - 1;  // wave's largest row
-        constexpr int KU = 8;
-        for (int k = 0; k < kend; k += KU) {
-          double mv[KU];
-#pragma unroll
-          for (int u = 0; u < KU; ++u) mv[u] = (on && k + u < i) ? mt[i + (size_t)(k + u) * n] : 0.0;
-#pragma unroll
-          for (int u = 0; u < KU; ++u)
-            if (on && k + u < i) {
-#pragma unroll
-              for (int jj = 0; jj < TJ; ++jj) acc[jj] += mv[u] * stage[jj * n + k + u];
-            }
-        }
-#pragma unroll
-        for (int jj = 0; jj < TJ; ++jj) {
-          const int j = j0 + jj;
-          if (on && j < j1 && (!upper || i < j)) a[i + (size_t)j * n] = acc[jj];
-        }
-      }
-      __syncthreads();
-    }
-    if (upper) {
-      for (int b0 = 0; b0 < nno; b0 += nthr * PF_U) {
-        double vv[PF_U];
-#pragma unroll
-        for (int u = 0; u < PF_U; ++u) {
-          const int e = b0 + tid + nthr * u;
-          const int row = e % n, col = e / n;
-          if (e < nno && row > col) vv[u] = a[col + (size_t)row * n];
-        }
-#pragma unroll
-        for (int u = 0; u < PF_U; ++u) {
-          const int e = b0 + tid + nthr * u;
-          const int row = e % n, col = e / n;
-          if (e < nno) {
-            if (row == col) a[e] = 0.0;
-            else if (row > col) a[e] = -vv[u];
-          }
-        }
-      }
-      __syncthreads();
     }
   }
+  __syncthreads();
+  PFC(5);
+  // (2) Mt = M^T of every plane into work (Mt(col,row) at col + row*n), with
+  // M = I, M(ir, jr+1) = A(ir, jr+2) for ir <= jr < n-2; vt(i) = -A(i, i+1).
+  PF_FRESH();
+  for (int b0 = 0; b0 < np * nn; b0 += nthr * PF_U) {
+    double mv[PF_U];
+#pragma unroll
+    for (int u = 0; u < PF_U; ++u) {
+      const int e = b0 + tid + nthr * u;
+      const int p = e / nn, r = e % nn;
+      const int col = r % n, row = r / n;
+      double v = (row == col) ? 1.0 : 0.0;
+      if (e < np * nn && col >= 1 && col <= n - 2 && row <= col - 1 && sh.info[p] == 0)
+        v = planes[(size_t)p * nn + row + (size_t)(col + 1) * n];
+      mv[u] = v;
+    }
+#pragma unroll
+    for (int u = 0; u < PF_U; ++u) {
+      const int e = b0 + tid + nthr * u;
+      if (e < np * nn) work[e] = mv[u];
+    }
+  }
+  for (int e = tid; e < np * (n - 1); e += nthr) {
+    const int p = e / (n - 1), i = e % (n - 1);
+    sh.C(p, i) = -planes[(size_t)p * nn + i + (size_t)(i + 1) * n];
+  }
+  __syncthreads();
+  // (3) skew-tridiagonal solve T C = M, C overwrites A; items (plane, column
+  // j), the recurrence carries its last value in a register, right-hand
+  // sides are loaded SU at a time ahead of it.
+  PF_FRESH();
+  for (int e = tid; e < np * n; e += nthr) {
+    constexpr int SU = 8;
+    const int p = e / n, j = e % n;
+    if (sh.info[p] != 0) continue;
+    const double* vt = &sh.C(p, 0);
+    const double* mt = work + (size_t)p * nn;
+    double* c = planes + (size_t)p * nn + (size_t)j * n;
+    double prev = mt[j + 0 * (size_t)n] / -vt[0];
+    c[1] = prev;
+    for (int i0 = 2; i0 < n; i0 += 2 * SU) {
+      double bb[SU];
+#pragma unroll
+      for (int u = 0; u < SU; ++u) bb[u] = (i0 + 2 * u < n) ? mt[j + (size_t)(i0 + 2 * u) * n] : 0.0;
+#pragma unroll
+      for (int u = 0; u < SU; ++u) {
+        const int i = i0 + 2 * u;
+        if (i < n) {
+          prev = (bb[u] - prev * vt[i - 1]) / -vt[i];
+          c[i + 1] = prev;
+        }
+      }
+    }
+    prev = mt[j + (size_t)(n - 1) * n] / vt[n - 2];
+    c[n - 2] = prev;
+    for (int i0 = n - 3; i0 >= 1; i0 -= 2 * SU) {
+      double bb[SU];
+#pragma unroll
+      for (int u = 0; u < SU; ++u) bb[u] = (i0 - 2 * u >= 1) ? mt[j + (size_t)(i0 - 2 * u) * n] : 0.0;
+#pragma unroll
+      for (int u = 0; u < SU; ++u) {
+        const int i = i0 - 2 * u;
+        if (i >= 1) {
+          prev = (bb[u] + prev * vt[i]) / vt[i - 1];
+          c[i - 1] = prev;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  PFC(6);
+  // (4) T = M^T C in place: T(i,j) = C(i,j) + sum_{k<i} M(k,i) C(k,j), k
+  // ascending.  For n > 64 the reference forms only the upper triangle
+  // (i < j) and restores skew symmetry; for n <= 64 it forms every element.
+  // One wave per plane, a lane owns rows g0 + lane + 64 r of a block of TJ
+  // columns; row groups go in descending order (a row reads C of the rows
+  // above it, which a group only overwrites after its own reads), and the
+  // C block of each batch of KB steps is broadcast through the wave's LDS
+  // buffer.
+  PF_FRESH();
+  {
+    constexpr int TR = 4, KB = 4;  // KB * TJ <= 64
+    double* wb = sh.wb + wave * 64;
+    const bool upper = n > 64;
+    for (int p = __builtin_amdgcn_readfirstlane(wave); p < np; p += nwave) {
+      if (sh.info[p] != 0) continue;
+      double* a = planes + (size_t)p * nn;
+      const double* mt = work + (size_t)p * nn;
+      for (int j0 = 0; j0 < n; j0 += TJ) {
+        const int j1 = min(n, j0 + TJ);
+        const int rows = upper ? j1 - 1 : n;
+        for (int gi = (rows + 64 * TR - 1) / (64 * TR) - 1; gi >= 0; --gi) {
+          const int g0 = gi * 64 * TR;
+          int ir[TR];
+          bool on[TR];
+          double acc[TR][TJ];
+#pragma unroll
+          for (int r = 0; r < TR; ++r) {
+            ir[r] = g0 + lane + 64 * r;
+            on[r] = ir[r] < rows;
+#pragma unroll
+            for (int jj = 0; jj < TJ; ++jj)
+              acc[r][jj] = (on[r] && j0 + jj < j1) ? (a + (size_t)(j0 + jj) * n)[ir[r]] : 0.0;
+          }
+          const int kend = min(rows, g0 + 64 * TR) - 1;  // the group's largest row
+          const int bk = lane % KB, bj = lane / KB;
+          for (int k = 0; k < kend; k += KB) {
+            const double bv = (bj < TJ && k + bk < kend && j0 + bj < j1) ? (a + (size_t)(j0 + bj) * n)[k + bk] : 0.0;
+            double mv[TR][KB];
+#pragma unroll
+            for (int r = 0; r < TR; ++r)
+#pragma unroll
+              for (int u = 0; u < KB; ++u)
+                mv[r][u] = (on[r] && k + u < ir[r]) ? (mt + (size_t)(k + u) * n)[ir[r]] : 0.0;
+            wave_sync();
+            wb[lane] = bv;  // wb[u + KB*jj] = C(k+u, j0+jj)
+            wave_sync();
+#pragma unroll
+            for (int u = 0; u < KB; ++u)
+#pragma unroll
+              for (int r = 0; r < TR; ++r)
+                if (on[r] && k + u < ir[r]) {
+#pragma unroll
+                  for (int jj = 0; jj < TJ; ++jj)
+                    if (j0 + jj < j1) acc[r][jj] += mv[r][u] * wb[u + KB * jj];
+                }
+          }
+          double* ao = opaque_uniform(a);  // store addresses not shared with the loads above
+#pragma unroll
+          for (int r = 0; r < TR; ++r)
+#pragma unroll
+            for (int jj = 0; jj < TJ; ++jj)
+              if (on[r] && j0 + jj < j1 && (!upper || ir[r] < j0 + jj)) (ao + (size_t)(j0 + jj) * n)[ir[r]] = acc[r][jj];
+          wave_sync();
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (n > 64) {
+    // skew fill: lower triangle = -upper, zero diagonal
+    PF_FRESH();
+    for (int b0 = 0; b0 < np * nn; b0 += nthr * PF_U) {
+      double vv[PF_U];
+#pragma unroll
+      for (int u = 0; u < PF_U; ++u) {
+        const int e = b0 + tid + nthr * u;
+        const int p = e / nn, r = e % nn;
+        const int row = r % n, col = r / n;
+        if (e < np * nn && row > col && sh.info[p] == 0) vv[u] = planes[(size_t)p * nn + col + (size_t)row * n];
+      }
+#pragma unroll
+      for (int u = 0; u < PF_U; ++u) {
+        const int e = b0 + tid + nthr * u;
+        const int p = e / nn, r = e % nn;
+        const int row = r % n, col = r / n;
+        if (e < np * nn && sh.info[p] == 0) {
+          if (row == col) planes[e] = 0.0;
+          else if (row > col) planes[e] = -vv[u];
+        }
+      }
+    }
+    __syncthreads();
+  }
   PFC(7);
+  PF_FRESH();
   // column then row permutations: the sequential swaps (j <-> P[j]-1,
   // j ascending) compose into one index map g, applied to rows and columns
   // as a gather (pure data movement), then the sign (calc_m_all's final
@@ -708,6 +791,7 @@ __device__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShare
   }
   __syncthreads();
   PFC(8);
+#undef PF_FRESH
 }
 
 }  // namespace mvmc_hip

@@ -45,8 +45,11 @@ def test_pfinv_bit_exact_vs_oracle(M):
     import torch
     rng = np.random.default_rng(3)
     data = np.load(os.path.join(GOLDEN, "pfapack_real_inputs.npz"))
+    # 224: the benchmark plane order; 258 and 300 take more than one group of
+    # 256 rows in the row-owner passes of the inverse
     cases = [data[k] for k in data.files] + [_random_skew(rng, 224) for _ in range(3)] + \
-            [_random_skew(rng, 16) for _ in range(9)]
+            [_random_skew(rng, 16) for _ in range(9)] + [_random_skew(rng, 258)] + \
+            [_random_skew(rng, 300) for _ in range(2)]
     by_n = {}
     for A in cases:
         by_n.setdefault(A.shape[0], []).append(A)
