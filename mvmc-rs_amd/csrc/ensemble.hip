@@ -1117,7 +1117,7 @@ __host__ __device__ inline size_t lds_bytes_rebuild(const EnsView& v) {
   return 2 * rnd16((size_t)v.n * 4) + rnd16((size_t)v.nqp * 8) + rnd16(pf_bytes(v.n, v.nqp));
 }
 
-__global__ void __launch_bounds__(256) k_rebuild(EnsView v) {
+__global__ void __launch_bounds__(256, 3) k_rebuild(EnsView v) {
   extern __shared__ __align__(16) char lds[];
   Carver c{lds};
   int* ei = c.take<int>(v.n);
@@ -1188,7 +1188,7 @@ __device__ double direct_proj_ratio_moved(const EnsView& v, int source, int dest
 // Rebuild for measurement: the inverse and Pfaffians of saved sample s of
 // every walker into the walker's planes and pf; s_valid[w][s] = the sample
 // is not the all-zero / all-negative placeholder and calc_m_all succeeded.
-__global__ void __launch_bounds__(256) k_meas_rebuild(EnsView v, int s) {
+__global__ void __launch_bounds__(256, 3) k_meas_rebuild(EnsView v, int s) {
   extern __shared__ __align__(16) char lds[];
   Carver c{lds};
   int* ei = c.take<int>(v.n);

@@ -40,7 +40,7 @@ __host__ __device__ inline size_t pfinv_work_doubles(int n, int np) {
 // piv[np*n], colbuf[np*n], kp/info[np], colmax[np], ks[np*n] (panel step of
 // the last swap through each index), act[np*LTL_K] (step applied per plane),
 // zmax[2*np] (extent of the zeroed diagonal; whether the last x column
-// awaits scaling), wb[PF_WAVES*64] (one broadcast buffer per wave).
+// awaits scaling), wb[PF_WAVES*128] (one broadcast buffer per wave).
 struct PfinvShared {
   int* piv;
   double* colbuf;
@@ -63,7 +63,7 @@ __host__ __device__ inline size_t pf_rnd16(size_t b) { return (b + 15) & ~(size_
 __host__ __device__ inline size_t pf_bytes(int n, int np) {
   return 2 * pf_rnd16((size_t)np * n * sizeof(int)) + pf_rnd16((size_t)np * n * sizeof(double)) +
          3 * pf_rnd16((size_t)np * 8) + pf_rnd16((size_t)np * LTL_K * sizeof(int)) + pf_rnd16(2 * (size_t)np * sizeof(int)) +
-         pf_rnd16((size_t)PF_WAVES * 64 * sizeof(double));
+         pf_rnd16((size_t)PF_WAVES * 128 * sizeof(double));
 }
 
 template <class Carver>
@@ -77,7 +77,7 @@ __device__ inline void carve_pfinv(Carver& c, PfinvShared& sh, int n, int np) {
   sh.ks = c.template take<int>((size_t)np * n);
   sh.act = c.template take<int>((size_t)np * LTL_K);
   sh.zmax = c.template take<int>(2 * (size_t)np);
-  sh.wb = c.template take<double>((size_t)PF_WAVES * 64);
+  sh.wb = c.template take<double>((size_t)PF_WAVES * 128);
 }
 
 // Argmax of |x| over a column with C IDAMAX semantics: first index of the
@@ -421,11 +421,16 @@ __device__ __forceinline__ void wg_pfinv(double* planes, double* work, int n, in
       __syncthreads();
       const int m = kk0;  // trailing columns j < m, rows i < m - 1
       if (m >= 2) {
-        // work item = (plane, 64-row chunk) per wave, the step scalars of a
-        // column are wave-uniform loads
+        // work item = (plane, 64-row chunk) per wave; chunk index varies
+        // slowest so every wave gets a mix of long and short chunks.  A
+        // lane owns row i; JB columns at a time: their elements are loaded
+        // together, their t1, t2 of every step are one wave load each,
+        // broadcast through the wave's LDS buffer.
+        constexpr int JB = 8;  // LTL_K * JB == 64
+        double* wb = sh.wb + wave * 128;
         const int chunks = (m - 1 + 63) / 64;
         for (int it = __builtin_amdgcn_readfirstlane(wave); it < np * chunks; it += nwave) {
-          const int p = it / chunks, c = it % chunks;
+          const int p = it % np, c = it / np;
           const int i = c * 64 + lane;
           const bool on = i < m - 1;
           double xr[LTL_K], yr[LTL_K];
@@ -439,26 +444,38 @@ __device__ __forceinline__ void wg_pfinv(double* planes, double* work, int n, in
           }
           const int ksi = on ? sh.KS(p, i) : tp;
           double* a = planes + (size_t)p * nn;
-          for (int j = c * 64 + 1; j < m; ++j) {
-            double t1v[LTL_K], t2v[LTL_K];
-            const int ksj = sh.KS(p, j);
+          const int tl = lane % LTL_K, jl = lane / LTL_K;
+          for (int j0 = c * 64 + 1; j0 < m; j0 += JB) {
+            const bool lv = tl < tp && j0 + jl < m;
+            const double v1 = lv ? PT1(p, tl)[j0 + jl] : 0.0;
+            const double v2 = lv ? PT2(p, tl)[j0 + jl] : 0.0;
+            double cc[JB];
 #pragma unroll
-            for (int t = 0; t < LTL_K; ++t)
-              if (t < tp) {
-                t1v[t] = PT1(p, t)[j];
-                t2v[t] = PT2(p, t)[j];
-              }
-            if (on && j > i) {
-              const int s0 = max(ksi, ksj);
-              if (s0 < tp) {
-                double cc = a[i + (size_t)j * n];
+            for (int jj = 0; jj < JB; ++jj) {
+              const int j = j0 + jj;
+              cc[jj] = (on && j > i && j < m) ? (a + (size_t)j * n)[i] : 0.0;
+            }
+            wave_sync();
+            wb[lane] = v1;       // wb[t + LTL_K*jj] = t1 of step t, column j0+jj
+            wb[64 + lane] = v2;  // t2
+            wave_sync();
+            double* ao = opaque_uniform(a);  // store addresses not shared with the loads above
 #pragma unroll
-                for (int t = 0; t < LTL_K; ++t)
-                  if (t >= s0 && ar[t]) cc = (cc + xr[t] * t1v[t]) - yr[t] * t2v[t];
-                a[i + (size_t)j * n] = cc;
+            for (int jj = 0; jj < JB; ++jj) {
+              const int j = j0 + jj;
+              if (on && j > i && j < m) {
+                const int s0 = max(ksi, sh.KS(p, j));
+                if (s0 < tp) {
+                  double x = cc[jj];
+#pragma unroll
+                  for (int t = 0; t < LTL_K; ++t)
+                    if (t >= s0 && ar[t]) x = (x + xr[t] * wb[t + LTL_K * jj]) - yr[t] * wb[64 + t + LTL_K * jj];
+                  (ao + (size_t)j * n)[i] = x;
+                }
               }
             }
           }
+          wave_sync();
         }
       }
       __syncthreads();
@@ -507,7 +524,7 @@ __device__ __forceinline__ void wg_pfinv(double* planes, double* work, int n, in
   // load, broadcast to the lanes through the wave's LDS buffer.
   {
     constexpr int TR = 4, KB = 4;  // KB * TJ <= 64
-    double* wb = sh.wb + wave * 64;
+    double* wb = sh.wb + wave * 128;
     for (int p = __builtin_amdgcn_readfirstlane(wave); p < np; p += nwave) {
       if (sh.info[p] != 0) continue;
       double* a = planes + (size_t)p * nn;
@@ -660,7 +677,7 @@ __device__ __forceinline__ void wg_pfinv(double* planes, double* work, int n, in
   PF_FRESH();
   {
     constexpr int TR = 4, KB = 4;  // KB * TJ <= 64
-    double* wb = sh.wb + wave * 64;
+    double* wb = sh.wb + wave * 128;
     const bool upper = n > 64;
     for (int p = __builtin_amdgcn_readfirstlane(wave); p < np; p += nwave) {
       if (sh.info[p] != 0) continue;
