@@ -274,17 +274,17 @@ __device__ inline double lds_serial_dot(const double* a, const double* b, int n)
   return r;
 }
 
-// Serial sum_k a[k] (k ascending) over an LDS array, same prefetching.
-__device__ inline double lds_serial_sum(const double* a, int n) {
-  double r = 0.0;
-  for (int k0 = 0; k0 < n; k0 += DOT_U) {
+// Serial r + sum_{k0 <= k < k1} a[k] (k ascending) over an LDS array, same
+// prefetching; a sum split at any k is the unsplit sum.
+__device__ inline double lds_serial_sum(const double* a, int k0, int k1, double r) {
+  for (int kb = k0; kb < k1; kb += DOT_U) {
     double av[DOT_U];
 #pragma unroll
     for (int u = 0; u < DOT_U; ++u)
-      if (k0 + u < n) av[u] = a[k0 + u];
+      if (kb + u < k1) av[u] = a[kb + u];
 #pragma unroll
     for (int u = 0; u < DOT_U; ++u)
-      if (k0 + u < n) r += av[u];
+      if (kb + u < k1) r += av[u];
   }
   return r;
 }
@@ -471,10 +471,11 @@ __device__ void hop_deltas(const EnsView& v, SampLds& L, int ri, int rj) {
   }
 }
 
-// log_proj_ratio over the changed entries in ascending parameter order
-// (zero terms leave a serial sum unchanged); also applies the deltas to
-// `cnt` when `apply` and clears them.  Result in L.dmisc[0] (all lanes).
-__device__ double proj_ratio_ordered(const EnsView& v, SampLds& L) {
+// The terms of log_proj_ratio, proj[p] * delta[p] over the changed entries
+// in ascending parameter order (zero terms leave a serial sum unchanged),
+// compacted into L.terms; returns their count (all lanes).  The parameter
+// loads of CB entries are issued together.
+__device__ int proj_terms(const EnsView& v, SampLds& L) {
   const int tid = threadIdx.x, nt = blockDim.x;
   const int np = v.nproj;
   const int chunk = (np + nt - 1) / nt;
@@ -488,15 +489,24 @@ __device__ double proj_ratio_ordered(const EnsView& v, SampLds& L) {
     if (tid == 0) set_status(v.status, ST_BAD_INPUT);
     total = cap;
   }
-  for (int p = p0; p < p1; ++p)
-    if (L.delta[p] != 0) {
-      if (off < cap) L.terms[off] = v.proj[p] * (double)L.delta[p];
-      ++off;
-    }
+  constexpr int CB = 8;
+  for (int b = p0; b < p1; b += CB) {
+    int d[CB];
+    double pv[CB];
+#pragma unroll
+    for (int u = 0; u < CB; ++u) d[u] = b + u < p1 ? L.delta[b + u] : 0;
+#pragma unroll
+    for (int u = 0; u < CB; ++u)
+      if (d[u] != 0) pv[u] = v.proj[b + u];
+#pragma unroll
+    for (int u = 0; u < CB; ++u)
+      if (d[u] != 0) {
+        if (off < cap) L.terms[off] = pv[u] * (double)d[u];
+        ++off;
+      }
+  }
   __syncthreads();
-  if (tid == 0) L.dmisc[0] = lds_serial_sum(L.terms, total);
-  __syncthreads();
-  return L.dmisc[0];
+  return total;
 }
 
 __device__ inline void apply_clear_deltas(const EnsView& v, SampLds& L, int* cnt, bool apply) {
@@ -822,23 +832,30 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v, int out_step, int n
           hop_deltas(v, L, rj, ri);
           __syncthreads();
         }
-        const double lpd = proj_ratio_ordered(v, L);
+        const int nterms = proj_terms(v, L);
         PHASE(1);
         // ---- Pfaffian ratios ----
+        // The projection sum is one serial chain: on the hop path lane 0 of
+        // wave 0 forms it (in two parts, carried in a register) while the
+        // other waves stage the rows and form the Pfaffian-ratio dots.
         double* stage = L.uni;            // [nqp][n]
         double* slt = L.uni + nqp * n;    // [nqp][n]
         double* vec1 = L.uni + 2 * nqp * n;
         if (type == 0) {
           const int msa = mi + s * ne;
           const int rsa = L.ele_idx[msa] + s * ns;
-          // row msa of the (pending-updated) inverse and the Slater row,
-          // loads of STAGE_U elements per lane issued together
+          const int ksplit = nterms / 2;
+          double lsum = 0.0;
+          if (tid == 0) lsum = lds_serial_sum(L.terms, 0, ksplit, 0.0);
+          // row msa of the (pending-updated) inverse and the Slater row
+          // (waves 1..), loads of STAGE_U elements per lane issued together
           constexpr int STAGE_U = 4;
-          for (int e0 = 0; e0 < nqp * n; e0 += nt * STAGE_U) {
+          const int st_id = tid - 64, st_n = nt - 64;
+          for (int e0 = 0; e0 < nqp * n && tid >= 64; e0 += st_n * STAGE_U) {
             double xv[STAGE_U], sv[STAGE_U], a1[STAGE_U], a2[STAGE_U];
 #pragma unroll
             for (int u = 0; u < STAGE_U; ++u) {
-              const int e = e0 + tid + u * nt;
+              const int e = e0 + st_id + u * st_n;
               if (e < nqp * n) {
                 const int q = e / n, j = e % n;
                 xv[u] = planes[(size_t)q * nn + (size_t)msa * n + j];
@@ -849,7 +866,7 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v, int out_step, int n
             }
 #pragma unroll
             for (int u = 0; u < STAGE_U; ++u) {
-              const int e = e0 + tid + u * nt;
+              const int e = e0 + st_id + u * st_n;
               if (e < nqp * n) {
                 const int q = e / n, j = e % n;
                 double x = xv[u];
@@ -861,11 +878,13 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v, int out_step, int n
           }
           __syncthreads();
           PHASE(2);
-          for (int q = tid; q < nqp; q += nt) {
+          if (tid == 0) L.dmisc[0] = lds_serial_sum(L.terms, ksplit, nterms, lsum);
+          for (int q = tid - 64; q < nqp && tid >= 64; q += nt - 64) {
             const double ratio = lds_serial_dot(stage + q * n, slt + q * n, n);
             L.pf_new[q] = -ratio * L.pf[q];
           }
         } else {
+          if (tid == 0) L.dmisc[0] = lds_serial_sum(L.terms, 0, nterms, 0.0);
           // two-electron ratio (updates.rs:395-441, 1539-1622) on the
           // fully updated inverse
           if (pm >= 0) {
@@ -922,6 +941,7 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v, int out_step, int n
         PHASE(type == 0 ? 3 : 7);
         // ---- Metropolis (lane 0) ----
         if (tid == 0) {
+          const double lpd = L.dmisc[0];
           const double lip_new = ddm::cr_log(fabs(ip_of_l(L.qpw, L.pf_new, nqp)));
           const double ex = 2.0 * (((lpd + 0.0) + lip_new) - L.dmisc[1]);
           double wgt = ddm::cr_exp(ex);
