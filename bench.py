@@ -17,10 +17,18 @@ per launch = proposals * NQP * 2n*8 (inverse row + Slater row) + accepted
 moves * NQP * 2n^2*8 (inverse read + write), divided by the launch's
 average device time (HIP events on the kernel's stream).
 cpu_baseline: the FP64 oracle (same algorithm and operation order) on a
-bounded walker subset of the same workload, one thread per walker.
+bounded walker subset of the same workload, one thread per walker, timed
+over at least 10 s of steady SR steps.
+precision_forms.mixed_fp32_inverse: the same run (inputs, seeds, steps) with
+the sampler's resident inverse in FP32 (mvmc_ens_set_precision 1); its
+samples/s, speed-up over the FP64 line and |dE/site| against the FP64 rows
+(Monte Carlo size once a chain's decisions differ; the FP64 standard error
+is reported beside it).
 """
 import argparse
 import json
+
+import numpy as np
 import os
 import sys
 import tempfile
@@ -51,20 +59,58 @@ def measured_traffic(workload, walkers):
     return rec.get("traffic_bytes_per_launch")
 
 
-def cpu_baseline(workload_dir, walkers, threads):
-    """Oracle leg: one untimed warm-up SR step, then one timed steady step."""
+def cpu_baseline(workload_dir, walkers, threads, min_seconds=10.0, max_steps=60):
+    """Oracle leg: one untimed warm-up SR step (thermalisation), then steady SR
+    steps until at least min_seconds of CPU work are timed."""
     import oracle_py
     run = oracle_py.Run(os.path.join(workload_dir, "namelist.def"), os.path.join(workload_dir, "initial.def"),
                         walkers=walkers, threads=threads)
     t0 = time.time()
     run.step(do_sr=True)
     t1 = time.time()
-    run.step(do_sr=True)
+    steps = 0
+    while steps < max_steps and (steps == 0 or time.time() - t1 < min_seconds):
+        run.step(do_sr=True)
+        steps += 1
     t2 = time.time()
-    samples = walkers * run.nsample
+    samples = walkers * run.nsample * steps
     return {"value": samples / (t2 - t1), "unit": "samples/s", "cores": threads, "kind": "port",
-            "sample": f"{walkers} walkers of the same workload, 1 steady-state SR step after 1 warm-up step "
+            "sample": f"{walkers} walkers of the same workload, {steps} steady-state SR steps after 1 warm-up step "
                       f"(warm-up {t1 - t0:.1f} s, timed {t2 - t1:.1f} s), one thread per walker"}
+
+
+def mixed_form(info, walkers, args, rows64, total64):
+    """Warm-up and timed steps of the FP64 run repeated with the sampler's
+    inverse in FP32 (same inputs and seeds); |dE/site| per timed step against
+    the FP64 run's zvo rows."""
+    import mvmc_hip
+    ens = mvmc_hip.Ensemble(info["namelist"], info["initial"], walkers=walkers)
+    ens.set_precision("mixed")
+    for s in range(args.warmup):
+        t = time.time()
+        ens.opt_step()
+        log(f"mixed warmup step {s}: {time.time() - t:.1f} s")
+    tm0 = ens.timings()
+    times, rows = [], []
+    for s in range(args.steps):
+        t = time.time()
+        rows.append(ens.opt_step())
+        times.append(time.time() - t)
+        log(f"mixed step {s}: E/site = {rows[-1][0] / info['nsite']:.8f}  {times[-1]:.2f} s")
+    tm1 = ens.timings()
+    total = sum(times)
+    ens.close()
+    de = [abs(r[0] - r64[0]) / info["nsite"] for r, r64 in zip(rows, rows64)]
+    # Monte Carlo standard error of E/site of the FP64 step (zvo row: Etot, ., Etot2, ...):
+    # once a decision flips, the chains differ and dE is of this size
+    se = [float(np.sqrt(max(r64[2] - r64[0] ** 2, 0.0) / (walkers * ens.nsample))) / info["nsite"] for r64 in rows64]
+    return {"value": walkers * ens.nsample * args.steps / total, "unit": "samples/s",
+            "speedup_vs_fp64": total64 / total, "sr_step_s": total / args.steps,
+            "sampling_s_per_step": (tm1["sample_s"] - tm0["sample_s"]) / args.steps,
+            "energy_per_site": [r[0] / info["nsite"] for r in rows], "abs_dE_per_site_vs_fp64": de,
+            "fp64_stat_err_per_site": se,
+            "arithmetic": "sampler inverse stored in FP32, its rank-2 updates in FP32; ratios, vec1 sums, "
+                          "Pfaffians, periodic rebuilds, measurement, SR in FP64"}
 
 
 def main():
@@ -80,6 +126,8 @@ def main():
     ap.add_argument("--cpu-walkers", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the precision-form run (sampler inverse in FP32) reported beside the FP64 line")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before the run, copy a buffer of this many MiB once (device-to-device), the known-size "
                          "kernel that scales FETCH_SIZE / WRITE_SIZE counters in a profiler run")
@@ -164,6 +212,12 @@ def main():
                      "kernel": "k_block", "avg_launch_ms": avg_launch_s * 1e3, "launches": launches,
                      "algorithmic_bytes_per_launch": bytes_total / max(launches, 1)},
     }
+    if not args.no_mixed:
+        # precision form (1): the same workload, seeds and step count with the
+        # sampler's resident inverse in FP32; energies against the FP64 rows
+        ens.close()
+        del ens
+        out["precision_forms"] = {"mixed_fp32_inverse": mixed_form(info, walkers, args, rows, total)}
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(wl_dir, args.cpu_walkers, args.cpu_threads)

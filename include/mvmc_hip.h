@@ -64,6 +64,18 @@ int mvmc_ens_dims(mvmc_ens* e, int* dims16);
 int mvmc_ens_get_params(mvmc_ens* e, double* proj, double* slater);
 int mvmc_ens_set_params(mvmc_ens* e, const double* proj, const double* slater);
 
+/* Precision form of the sampler (mode 0, default: FP64 throughout; mode 1,
+ * mixed: the resident inverse the Metropolis sweep streams and updates is
+ * stored in FP32 (rounded to nearest on every store; ratios, updates and
+ * Pfaffians are computed in FP64), while the periodic from-scratch
+ * Pfaffian/inverse, the measurement, the SR-CG and every accumulator stay
+ * FP64).  Takes effect at the next mvmc_ens_sample (which recomputes the
+ * inverse from the configuration).  Mode 1 allocates W*NQP*n^2 floats once.
+ * No reference counterpart as an entry point: it is the precision form of
+ * the sweep kernel (updates.rs:588-638,1184-1224) that the reference's CUDA
+ * backend offers only as a delayed-update FP64 variant. */
+int mvmc_ens_set_precision(mvmc_ens* e, int mode);
+
 /* One SR step's sampling on every walker (vmc_make_sample,
  * crates/mvmc-core/src/sampling/driver.rs:175-458): first call warm-up
  * (NVMCWarmUp + NVMCSample outer steps), later calls restore the burn state

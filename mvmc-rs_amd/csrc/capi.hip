@@ -235,6 +235,10 @@ struct mvmc_ens {
   double t_sample = 0, t_measure = 0, t_sr = 0, proposals = 0, accepted = 0;
   float block_ms = 0;
   int block_launches = 0;
+  // precision form: 0 FP64 throughout; 1 the sampler's inverse stored in
+  // FP32 (inv32), everything else FP64
+  int precision = 0;
+  float* inv32 = nullptr;
   ~mvmc_ens() {
     cg.release();
     for (void* p : allocs) (void)hipFree(p);
@@ -420,7 +424,10 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
   if (E->lds_samp > (size_t)maxlds || E->lds_meas > (size_t)maxlds)
     throw Fail(MVMC_E_INVALID_SHAPE, "model needs " + std::to_string(std::max(E->lds_samp, E->lds_meas)) +
                                          " B of LDS per workgroup, device allows " + std::to_string(maxlds));
-  HIPCHK(hipFuncSetAttribute((const void*)k_block, hipFuncAttributeMaxDynamicSharedMemorySize, (int)E->lds_samp));
+  HIPCHK(hipFuncSetAttribute((const void*)k_block<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)E->lds_samp));
+  HIPCHK(hipFuncSetAttribute((const void*)k_block<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)E->lds_samp));
   HIPCHK(hipFuncSetAttribute((const void*)k_begin, hipFuncAttributeMaxDynamicSharedMemorySize, (int)E->lds_samp));
   HIPCHK(hipFuncSetAttribute((const void*)k_measure, hipFuncAttributeMaxDynamicSharedMemorySize, (int)E->lds_meas));
   E->lds_rebuild = lds_bytes_rebuild(v);
@@ -429,7 +436,12 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
   HIPCHK(hipFuncSetAttribute((const void*)k_meas_rebuild, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)E->lds_rebuild));
   int occ_s = 0, occ_m = 0, occ_r = 0;
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, k_block, 256, E->lds_samp));
+  {
+    int occ_d = 0, occ_f = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_d, k_block<double>, 256, E->lds_samp));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_f, k_block<float>, 256, E->lds_samp));
+    occ_s = std::min(occ_d, occ_f);  // one grid serves both forms
+  }
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_m, k_measure, 256, E->lds_meas));
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r, k_rebuild, 256, E->lds_rebuild));
   occ_s = std::max(1, occ_s);
@@ -494,7 +506,10 @@ void sample_impl(mvmc_ens* E) {
     for (;;) {
       HIPCHK(hipMemsetAsync(v.nflag, 0, sizeof(int), E->st));
       HIPCHK(hipEventRecord(e0, E->st));
-      hipLaunchKernelGGL(k_block, dim3(E->grid_samp), dim3(256), E->lds_samp, E->st, v, out, n_in);
+      if (E->precision == 1)
+        hipLaunchKernelGGL(k_block<float>, dim3(E->grid_samp), dim3(256), E->lds_samp, E->st, v, E->inv32, out, n_in);
+      else
+        hipLaunchKernelGGL(k_block<double>, dim3(E->grid_samp), dim3(256), E->lds_samp, E->st, v, v.inv, out, n_in);
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(e1, E->st));
       int nf = 0;
@@ -786,7 +801,13 @@ int mvmc_ens_get_tables(mvmc_ens* e, int w, double* inv, double* pf) {
     const EnsView& v = e->v;
     if (w < 0 || w >= v.W) throw Fail(MVMC_E_INVALID_SHAPE, "walker out of range");
     const size_t nn = (size_t)v.n * v.n;
-    if (inv) HIPCHK(hipMemcpy(inv, v.inv + (size_t)w * v.nqp * nn, sizeof(double) * v.nqp * nn, hipMemcpyDeviceToHost));
+    if (inv && e->precision == 1) {  // the sampler's FP32 inverse, widened
+      std::vector<float> h((size_t)v.nqp * nn);
+      HIPCHK(hipMemcpy(h.data(), e->inv32 + (size_t)w * v.nqp * nn, sizeof(float) * h.size(), hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < h.size(); ++k) inv[k] = h[k];
+    } else if (inv) {
+      HIPCHK(hipMemcpy(inv, v.inv + (size_t)w * v.nqp * nn, sizeof(double) * v.nqp * nn, hipMemcpyDeviceToHost));
+    }
     if (pf) HIPCHK(hipMemcpy(pf, v.pf + (size_t)w * v.nqp, sizeof(double) * v.nqp, hipMemcpyDeviceToHost));
   });
 }
@@ -807,6 +828,16 @@ int mvmc_ens_get_cg(mvmc_ens* e, double* x, int* n_active) {
   if (x) std::copy(e->cg_x.begin(), e->cg_x.end(), x);
   if (n_active) *n_active = (int)e->cg_x.size();
   return (int)e->cg_info;
+}
+
+int mvmc_ens_set_precision(mvmc_ens* e, int mode) {
+  return guarded([&] {
+    if (mode != 0 && mode != 1) throw Fail(MVMC_E_INVALID_SHAPE, "precision form must be 0 (FP64) or 1 (mixed)");
+    if (mode == 1 && !e->inv32)
+      e->inv32 = dalloc<float>(e->allocs, (size_t)e->v.W * e->v.nqp * e->v.n * e->v.n);
+    e->precision = mode;
+    e->v.inv32 = mode == 1 ? e->inv32 : nullptr;
+  });
 }
 
 int mvmc_ens_phase_clocks(mvmc_ens* e, long long* out) {

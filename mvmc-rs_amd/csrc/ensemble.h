@@ -42,6 +42,7 @@ struct EnsView {
   int* ele_num;       // [W][n2]
   int* proj_cnt;      // [W][nproj]
   double* inv;        // [W][nqp][n*n]
+  float* inv32;       // [W][nqp][n*n] the sampler's inverse in the mixed-precision form, else null
   double* pf;         // [W][nqp]
   double* log_ip;     // [W]
   int* wstate;        // [W][8]: 0 burn flag, 1 n_accept window, 2 n_out, 3 status,
@@ -60,7 +61,7 @@ struct EnsView {
   double* qorb;       // [slots][nqp][nslater]
   double* pend;       // [slots][2][nqp][n]: vec1 / vec2 of the not yet applied hop update
   int* status;        // [1] sticky
-  long long* clk;     // [slots][16] phase clocks (MVMC_PHASE_CLOCKS builds), else null
+  long long* clk;     // [slots][32] phase clocks (MVMC_PHASE_CLOCKS builds), else null
 };
 
 }  // namespace mvmc_hip

@@ -383,6 +383,20 @@ __device__ void make_proj_cnt(const EnsView& v, const int* ele_num, int* cnt) {
 // begin: burn restore or validated initial placement, initial tables, log|ip|
 // (driver.rs:239-330)
 // ---------------------------------------------------------------------------
+// The mixed-precision form keeps the sampler's inverse in FP32: after a
+// from-scratch FP64 calc_m_all, walker w's planes are copied rounded to
+// nearest (n is even, so pairs never straddle the end).
+__device__ inline void copy_inv32(const EnsView& v, int w) {
+  if (!v.inv32) return;
+  const size_t m2 = (size_t)v.nqp * v.n * v.n / 2;
+  const double2* src = reinterpret_cast<const double2*>(v.inv + (size_t)w * 2 * m2);
+  float2* dst = reinterpret_cast<float2*>(v.inv32 + (size_t)w * 2 * m2);
+  for (size_t e = threadIdx.x; e < m2; e += blockDim.x) {
+    const double2 x = src[e];
+    dst[e] = make_float2((float)x.x, (float)x.y);
+  }
+}
+
 __global__ void __launch_bounds__(256) k_begin(EnsView v) {
   extern __shared__ __align__(16) char lds[];
   SampLds L = carve_sampler(v, lds);
@@ -420,6 +434,7 @@ __global__ void __launch_bounds__(256) k_begin(EnsView v) {
       attempt = 0;
       burn = 0;
     }
+    copy_inv32(v, w);
     for (int q = tid; q < v.nqp; q += blockDim.x) v.pf[(size_t)w * v.nqp + q] = L.pf[q];
     if (tid == 0) {
       v.log_ip[w] = lip;
@@ -523,13 +538,12 @@ __device__ inline void apply_clear_deltas(const EnsView& v, SampLds& L, int* cnt
 // block of proposals (engine.rs:188-475) and sample saving (driver.rs:419-443)
 // ---------------------------------------------------------------------------
 constexpr int CG_RCH = 32, CG_SCH = 16, CG_CMAX = 32;  // SR-CG reduction blocking
-constexpr int VEC1_COLS = 4;  // columns per lane held in registers by stream_planes
 
 // One not yet applied hop update (updates.rs:1184-1224: vec1, vec2 = scaled
 // old row `pm`) on element (i, j), with the operation order of the in-place
 // update so the delayed element equals the eagerly updated one bit for bit.
-__device__ inline double apply_pend(double x, double v1i, double v2i, double v1j, double v2j, int i, int j,
-                                    int pm) {
+template <class CT>
+__device__ inline CT apply_pend(CT x, CT v1i, CT v2i, CT v1j, CT v2j, int i, int j, int pm) {
   x += v1i * v2j - v1j * v2i;
   if (j == pm) x -= v2i;
   if (i == pm) x += v2j;
@@ -540,81 +554,128 @@ __device__ inline double apply_pend(double x, double v1i, double v2i, double v1j
 // plus a wave-uniform row offset in a scalar register, so a tile of loads
 // costs one address register instead of one 64-bit pointer per load.
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
 constexpr int BUF_WORD3 = 0x00020000;  // raw untyped buffer (gfx9 family)
 // Cache policy of the stream (0: default; the non-temporal hint measured
 // slower: the walker's inverse is re-read at its next accepted hop).
 constexpr int BUF_NT = 0;
-__device__ inline double2 buf_load2(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, BUF_NT));
-}
-__device__ inline void buf_store2(double2 x, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
-  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, x), r, voff, soff, BUF_NT);
-}
 
-// One streaming pass over a walker's planes.  A lane owns VEC1_COLS
+// Storage of the planes seen as 16-byte vectors of EPV elements: FP64
+// (EPV = 2) or FP32, the mixed-precision form (EPV = 4; the rank-2 updates
+// of the stream in FP32, everything else in FP64).  cnt = elements of the
+// vector inside the row (EPV, or 2 for an FP32 vector at the end of a row
+// whose length is 2 mod 4).
+template <class T>
+struct PlaneIO;
+template <>
+struct PlaneIO<double> {
+  static constexpr int EPV = 2;
+  static __device__ inline u32x4_t load(__amdgpu_buffer_rsrc_t r, int voff, int soff, int) {
+    return __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, BUF_NT);
+  }
+  static __device__ inline void store(u32x4_t x, __amdgpu_buffer_rsrc_t r, int voff, int soff, int) {
+    __builtin_amdgcn_raw_buffer_store_b128(x, r, voff, soff, BUF_NT);
+  }
+  static __device__ inline double get(u32x4_t x, int h) {
+    const double2 d = __builtin_bit_cast(double2, x);
+    return h == 0 ? d.x : d.y;
+  }
+  static __device__ inline void set(u32x4_t& x, int h, double v) {
+    double2 d = __builtin_bit_cast(double2, x);
+    if (h == 0) d.x = v; else d.y = v;
+    x = __builtin_bit_cast(u32x4_t, d);
+  }
+};
+template <>
+struct PlaneIO<float> {
+  static constexpr int EPV = 4;
+  static __device__ inline u32x4_t load(__amdgpu_buffer_rsrc_t r, int voff, int soff, int cnt) {
+    if (cnt == 4) return __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, BUF_NT);
+    const u32x2_t h = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, BUF_NT);
+    u32x4_t x = {h.x, h.y, 0u, 0u};
+    return x;
+  }
+  static __device__ inline void store(u32x4_t x, __amdgpu_buffer_rsrc_t r, int voff, int soff, int cnt) {
+    if (cnt == 4) __builtin_amdgcn_raw_buffer_store_b128(x, r, voff, soff, BUF_NT);
+    else __builtin_amdgcn_raw_buffer_store_b64(u32x2_t{x.x, x.y}, r, voff, soff, BUF_NT);
+  }
+  static __device__ inline float get(u32x4_t x, int h) {
+    const unsigned u = h == 0 ? x.x : h == 1 ? x.y : h == 2 ? x.z : x.w;
+    return __builtin_bit_cast(float, u);
+  }
+  static __device__ inline void set(u32x4_t& x, int h, float v) {
+    const unsigned u = __builtin_bit_cast(unsigned, v);
+    if (h == 0) x.x = u; else if (h == 1) x.y = u; else if (h == 2) x.z = u; else x.w = u;
+  }
+};
+
+// One streaming pass over a walker's planes.  A lane owns CPL = 2 * EPV
 // contiguous columns of one plane (q = tid / LP; columns
-// LP*VEC1_COLS*g + VEC1_COLS*(tid % LP) + c), read and written as 16-byte
-// pairs (n is even), and issues the loads of STREAM_ROWS rows before using
-// any of them: each lane keeps STREAM_ROWS * VEC1_COLS loads in flight, which
-// is what bounds a pass over HBM at this occupancy.
-//   pm >= 0 : apply the pending update (pv1, pv2, row pm) and write back;
-//   ACC     : out[q][j] = sum_i -X[i][j] * b[q][i], serial in i (the
-//             update_one_real vec1 = -(X^T b)).
+// LP*CPL*g + CPL*(tid % LP) + c), read and written as two 16-byte vectors
+// per row, and issues the loads of STREAM_ROWS rows before using any of
+// them: each lane keeps STREAM_ROWS rows of its columns in flight, which is
+// what bounds a pass over HBM at this occupancy (the FP32 form therefore
+// covers twice the columns per round trip).
+//   pm >= 0 : apply the pending update (pv1, pv2, row pm) and write back,
+//             in the storage type's arithmetic;
+//   ACC     : out[q][j] = sum_i -X[i][j] * b[q][i] in FP64, serial in i
+//             (the update_one_real vec1 = -(X^T b)), X as stored.
 constexpr int STREAM_ROWS = 4;
-template <bool ACC>
-__device__ void stream_planes(double* planes, int n, int nqp, const double* pv1, const double* pv2, int pm,
+template <bool ACC, class T>
+__device__ void stream_planes(T* planes, int n, int nqp, const double* pv1, const double* pv2, int pm,
                               const double* b, double* out) {
-  static_assert(VEC1_COLS % 2 == 0, "columns are handled in pairs");
-  constexpr int NP2 = VEC1_COLS / 2;
+  using IO = PlaneIO<T>;
+  constexpr int EPV = IO::EPV, NV = 2, CPL = NV * EPV;
+  constexpr int ES = (int)sizeof(T);
   const int nt = blockDim.x, tid = threadIdx.x;
   const int nn = n * n;
   const int PG = nqp < nt ? nqp : nt;
   const int LP = nt / PG;
   const int qo = tid / LP, jl = tid % LP;
-  const int span = LP * VEC1_COLS;  // columns per group
+  const int span = LP * CPL;  // columns per group
   const __amdgpu_buffer_rsrc_t rs =
-      __builtin_amdgcn_make_buffer_rsrc(planes, 0, (int)((size_t)nqp * nn * sizeof(double)), BUF_WORD3);
-  const int rowb = n * (int)sizeof(double);
+      __builtin_amdgcn_make_buffer_rsrc(planes, 0, (int)((size_t)nqp * nn * ES), BUF_WORD3);
+  const int rowb = n * ES;
   for (int q0 = 0; q0 < nqp; q0 += PG) {
     const int q = q0 + qo;
     const bool on = qo < PG && q < nqp;
-    const int qoff = (on ? q : 0) * nn * (int)sizeof(double);
+    const int qoff = (on ? q : 0) * nn * ES;
     const double* p1 = pv1 + (on ? q : 0) * n;
     const double* p2 = pv2 + (on ? q : 0) * n;
     const double* bq = b + (on ? q : 0) * n;
     for (int g0 = 0; g0 < n; g0 += span) {
-      int jp[NP2];  // first column of each pair, -1 when outside the plane
-      double acc[VEC1_COLS], c1[VEC1_COLS], c2[VEC1_COLS];
+      const int jb = g0 + CPL * jl;  // first column of the lane
+      int cnt[NV];                   // elements of vector m inside the row
+      double acc[CPL];
+      T c1[CPL], c2[CPL];
 #pragma unroll
-      for (int m = 0; m < NP2; ++m) {
-        const int j = g0 + VEC1_COLS * jl + 2 * m;
-        jp[m] = (on && j < n) ? j : -1;
+      for (int m = 0; m < NV; ++m) cnt[m] = on ? max(0, min(EPV, n - (jb + EPV * m))) : 0;
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          acc[2 * m + h] = 0.0;
-          c1[2 * m + h] = (pm >= 0 && jp[m] >= 0) ? p1[j + h] : 0.0;
-          c2[2 * m + h] = (pm >= 0 && jp[m] >= 0) ? p2[j + h] : 0.0;
-        }
+      for (int c = 0; c < CPL; ++c) {
+        const bool in = c % EPV < cnt[c / EPV];
+        acc[c] = 0.0;
+        c1[c] = (pm >= 0 && in) ? (T)p1[jb + c] : (T)0;
+        c2[c] = (pm >= 0 && in) ? (T)p2[jb + c] : (T)0;
       }
-      const int vbase = qoff + (g0 + VEC1_COLS * jl) * (int)sizeof(double);
+      const int vbase = qoff + jb * ES;
       // Two tiles in flight: the loads of tile i0 + STREAM_ROWS (inverse
       // rows, pending-vector and b rows) are issued before tile i0 is
       // updated and stored (a load after a buffer store could not be moved
       // above it by the compiler).
-      double2 x[STREAM_ROWS][NP2], xn[STREAM_ROWS][NP2];
-      double ra[STREAM_ROWS], rb[STREAM_ROWS], rsv[STREAM_ROWS];
-      double ran[STREAM_ROWS], rbn[STREAM_ROWS], rsvn[STREAM_ROWS];
-      auto load_tile = [&](int i0, double2 (&tx)[STREAM_ROWS][NP2], double (&ta)[STREAM_ROWS],
-                           double (&tb)[STREAM_ROWS], double (&ts)[STREAM_ROWS]) {
+      u32x4_t x[STREAM_ROWS][NV], xn[STREAM_ROWS][NV];
+      T ra[STREAM_ROWS], rb[STREAM_ROWS], ran[STREAM_ROWS], rbn[STREAM_ROWS];
+      double rsv[STREAM_ROWS], rsvn[STREAM_ROWS];
+      auto load_tile = [&](int i0, u32x4_t (&tx)[STREAM_ROWS][NV], T (&ta)[STREAM_ROWS], T (&tb)[STREAM_ROWS],
+                           double (&ts)[STREAM_ROWS]) {
 #pragma unroll
         for (int r = 0; r < STREAM_ROWS; ++r)
           if (i0 + r < n) {
 #pragma unroll
-            for (int m = 0; m < NP2; ++m)
-              if (jp[m] >= 0) tx[r][m] = buf_load2(rs, vbase + 16 * m, (i0 + r) * rowb);
+            for (int m = 0; m < NV; ++m)
+              if (cnt[m] > 0) tx[r][m] = IO::load(rs, vbase + 16 * m, (i0 + r) * rowb, cnt[m]);
             if (pm >= 0) {
-              ta[r] = p1[i0 + r];
-              tb[r] = p2[i0 + r];
+              ta[r] = (T)p1[i0 + r];
+              tb[r] = (T)p2[i0 + r];
             }
             if (ACC) ts[r] = bq[i0 + r];
           }
@@ -627,30 +688,33 @@ __device__ void stream_planes(double* planes, int n, int nqp, const double* pv1,
           const int i = i0 + r;
           if (i < n) {
             if (pm >= 0) {
-              const double a0 = ra[r], b0 = rb[r];
+              const T a0 = ra[r], b0 = rb[r];
 #pragma unroll
-              for (int m = 0; m < NP2; ++m)
-                if (jp[m] >= 0) {
-                  x[r][m].x = apply_pend(x[r][m].x, a0, b0, c1[2 * m], c2[2 * m], i, jp[m], pm);
-                  x[r][m].y = apply_pend(x[r][m].y, a0, b0, c1[2 * m + 1], c2[2 * m + 1], i, jp[m] + 1, pm);
-                  buf_store2(x[r][m], rs, vbase + 16 * m, i * rowb);
+              for (int m = 0; m < NV; ++m)
+                if (cnt[m] > 0) {
+#pragma unroll
+                  for (int h = 0; h < EPV; ++h)
+                    if (h < cnt[m])
+                      IO::set(x[r][m], h,
+                              apply_pend(IO::get(x[r][m], h), a0, b0, c1[EPV * m + h], c2[EPV * m + h], i,
+                                         jb + EPV * m + h, pm));
+                  IO::store(x[r][m], rs, vbase + 16 * m, i * rowb, cnt[m]);
                 }
             }
             if (ACC) {
               const double s0 = rsv[r];
 #pragma unroll
-              for (int m = 0; m < NP2; ++m)
-                if (jp[m] >= 0) {
-                  acc[2 * m] += -x[r][m].x * s0;
-                  acc[2 * m + 1] += -x[r][m].y * s0;
-                }
+              for (int m = 0; m < NV; ++m)
+#pragma unroll
+                for (int h = 0; h < EPV; ++h)
+                  if (h < cnt[m]) acc[EPV * m + h] += -(double)IO::get(x[r][m], h) * s0;
             }
           }
         }
 #pragma unroll
         for (int r = 0; r < STREAM_ROWS; ++r) {
 #pragma unroll
-          for (int m = 0; m < NP2; ++m) x[r][m] = xn[r][m];
+          for (int m = 0; m < NV; ++m) x[r][m] = xn[r][m];
           ra[r] = ran[r];
           rb[r] = rbn[r];
           rsv[r] = rsvn[r];
@@ -658,11 +722,8 @@ __device__ void stream_planes(double* planes, int n, int nqp, const double* pv1,
       }
       if (ACC) {
 #pragma unroll
-        for (int m = 0; m < NP2; ++m)
-          if (jp[m] >= 0) {
-            out[q * n + jp[m]] = acc[2 * m];
-            out[q * n + jp[m] + 1] = acc[2 * m + 1];
-          }
+        for (int c = 0; c < CPL; ++c)
+          if (c % EPV < cnt[c / EPV]) out[q * n + jb + c] = acc[c];
       }
     }
   }
@@ -692,7 +753,10 @@ __device__ void stream_planes(double* planes, int n, int nqp, const double* pv1,
 
 enum { M_TYPE = 0, M_REJ, M_MI, M_S, M_RI, M_RJ, M_MJ, M_T, M_ACC, M_NACC, M_NOUT, M_SKIP, M_RES, M_DONE };
 
-__global__ void __launch_bounds__(256, 2) k_block(EnsView v, int out_step, int n_in) {
+// Elem: storage type of the sampler's inverse (double; float in the
+// mixed-precision form, whose arithmetic stays FP64).
+template <class Elem>
+__global__ void __launch_bounds__(256, 2) k_block(EnsView v, Elem* inv, int out_step, int n_in) {
   extern __shared__ __align__(16) char lds[];
   SampLds L = carve_sampler(v, lds);
   load_tables(v, L);
@@ -717,7 +781,7 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v, int out_step, int n
     const int resume = L.misc[M_RES];
     if (out_step >= n_out || L.misc[M_DONE]) { __syncthreads(); continue; }
     load_walker(v, w, L, true);
-    double* planes = v.inv + (size_t)w * nqp * nn;
+    Elem* planes = inv + (size_t)w * nqp * nn;
     int* cnt = v.proj_cnt + (size_t)w * v.nproj;
     long long* ctr = v.counter + (size_t)w * 4;
     for (int q = tid; q < nqp; q += nt) L.pf[q] = v.pf[(size_t)w * nqp + q];
@@ -911,7 +975,7 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v, int out_step, int n
           __syncthreads();
           for (int e = tid; e < nqp * n; e += nt) {
             const int q = e / n, i = e % n;
-            const double* row = planes + (size_t)q * nn + (size_t)i * n;
+            const Elem* row = planes + (size_t)q * nn + (size_t)i * n;
             const double* a = va + q * n;
             double acc = 0.0;
             for (int j = 0; j < n; ++j) acc += row[j] * a[j];
@@ -1010,7 +1074,7 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v, int out_step, int n
             __syncthreads();
             for (int e = tid; e < nqp * n; e += nt) {
               const int q = e / n, i = e % n;
-              const double* row = planes + (size_t)q * nn + (size_t)i * n;
+              const Elem* row = planes + (size_t)q * nn + (size_t)i * n;
               double p = 0.0, qq = 0.0;
               for (int j = 0; j < n; ++j) {
                 const double x = row[j];
@@ -1063,7 +1127,7 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v, int out_step, int n
               if (i == msb) x += a * tj + b * sj - f * idet * pj;
               if (i == msa && j == msb) x += f * idet;
               if (i == msb && j == msa) x -= f * idet;
-              planes[e] = x;
+              planes[e] = static_cast<Elem>(x);
             }
             __syncthreads();
           }
@@ -1157,6 +1221,7 @@ __global__ void __launch_bounds__(256, 3) k_rebuild(EnsView v) {
     double* planes = v.inv + (size_t)w * v.nqp * v.n * v.n;
     const int bad = wg_calc_m_all(v, ei, rs, planes, work, pfn, pfscr, false,
                                   v.clk ? v.clk + (size_t)blockIdx.x * 32 + 16 : nullptr);
+    copy_inv32(v, w);
     if (tid == 0) {
       if (!bad) {
         for (int q = 0; q < v.nqp; ++q) v.pf[(size_t)w * v.nqp + q] = pfn[q];

@@ -60,7 +60,7 @@ EXPORTED_SYMBOLS = [
     "mvmc_ens_destroy", "mvmc_ens_dims", "mvmc_ens_get_params", "mvmc_ens_set_params", "mvmc_ens_sample",
     "mvmc_ens_measure", "mvmc_ens_sr_cg", "mvmc_ens_opt_step", "mvmc_ens_get_energy", "mvmc_ens_get_operands",
     "mvmc_ens_get_o_store", "mvmc_ens_get_walker", "mvmc_ens_get_tables", "mvmc_ens_get_config", "mvmc_ens_get_cg",
-    "mvmc_ens_timings", "mvmc_ens_phase_clocks", "mvmc_ens_status", "mvmc_sr_cg_solve",
+    "mvmc_ens_timings", "mvmc_ens_phase_clocks", "mvmc_ens_status", "mvmc_sr_cg_solve", "mvmc_ens_set_precision",
 ]
 
 
@@ -133,6 +133,11 @@ class Ensemble:
         s = np.zeros(max(self.nslater, 1))
         lib().mvmc_ens_get_params(self.h, _p(p), _p(s))
         return p[:self.nproj].copy(), s[:self.nslater].copy()
+
+    def set_precision(self, mode):
+        """0: FP64 throughout; 1 ("mixed"): the sampler's resident inverse in
+        FP32, FP64 arithmetic and everything else FP64."""
+        _check(lib().mvmc_ens_set_precision(self.h, int({"fp64": 0, "mixed": 1}.get(mode, mode))))
 
     def set_params(self, proj, slater):
         p = np.ascontiguousarray(proj, dtype=np.float64)

@@ -205,3 +205,32 @@ def test_sr_cg_solver_c_abi(M, name, variant):
         sto, xo, _, _ = O.cg_solve(mean, diag, Osamp, g, 1.0 / samples, 1e-5, 0.0, limit, bool(variant))
         assert st == sto == status
         assert np.array_equal(x.cpu().numpy(), xo), limit
+
+
+@pytest.mark.parametrize("namelist,W", [(HUB, 8), (HEIS, 4)])
+def test_mixed_precision_form(M, namelist, W):
+    """Precision form (1): the sampler's resident inverse stored in FP32.
+    Against the FP64 path on the same seeds: the Metropolis decisions do not
+    flip on these chains, so every configuration is identical, and the
+    measurement (a from-scratch FP64 rebuild per sample) and SR give
+    bit-identical zvo rows; the FP32 inverse after a sampling call holds
+    FP32 values within 1e-5 (relative to the plane's largest element) of
+    the FP64 inverse (the drift of at most Nsite rounded rank-2 updates)."""
+    log(f"mixed precision {os.path.basename(os.path.dirname(namelist))} W={W}")
+    e64 = M.Ensemble(namelist, walkers=W)
+    e32 = M.Ensemble(namelist, walkers=W)
+    e32.set_precision("mixed")
+    for step in range(3):
+        r64, r32 = e64.opt_step(), e32.opt_step()
+        for w in range(W):
+            assert np.array_equal(e64.walker(w)["ele_idx"], e32.walker(w)["ele_idx"]), (step, w)
+        assert np.array_equal(r64, r32), step
+    e64.sample()
+    e32.sample()
+    for w in range(W):
+        t64, t32 = e64.tables(w)[0], e32.tables(w)[0]
+        assert np.array_equal(t32, t32.astype(np.float32).astype(np.float64))
+        scale = np.abs(t64).max()
+        assert np.abs(t32 - t64).max() <= 1e-5 * scale, (w, np.abs(t32 - t64).max() / scale)
+    with pytest.raises(M.InvalidShape):
+        e32.set_precision(7)
