@@ -454,7 +454,8 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
   v.work = dalloc<double>(A, (size_t)E->slots * pfinv_work_doubles(v.n, v.nqp));
   v.qorb = dalloc<double>(A, (size_t)E->slots * v.nqp * std::max(v.nslater, 1));
   v.pend = dalloc<double>(A, (size_t)E->slots * 2 * v.nqp * v.n);
-  v.nflag = dalloc<int>(A, 1);
+  v.nflag = dalloc<int>(A, 4);
+  v.rlist = dalloc<int>(A, (size_t)walkers);
 #ifdef MVMC_PHASE_CLOCKS
   v.clk = dalloc<long long>(A, (size_t)E->slots * 32);
   HIPCHK(hipMemset(v.clk, 0, sizeof(long long) * E->slots * 32));
@@ -504,7 +505,7 @@ void sample_impl(mvmc_ens* E) {
   for (int out = 0; out < n_out; ++out) {
     hipLaunchKernelGGL(k_reset_resume, dim3(nblk(v.W)), dim3(256), 0, E->st, v);
     for (;;) {
-      HIPCHK(hipMemsetAsync(v.nflag, 0, sizeof(int), E->st));
+      HIPCHK(hipMemsetAsync(v.nflag, 0, 4 * sizeof(int), E->st));
       HIPCHK(hipEventRecord(e0, E->st));
       if (E->precision == 1)
         hipLaunchKernelGGL(k_block<float>, dim3(E->grid_samp), dim3(256), E->lds_samp, E->st, v, E->inv32, out, n_in);

@@ -47,7 +47,9 @@ struct EnsView {
   double* log_ip;     // [W]
   int* wstate;        // [W][8]: 0 burn flag, 1 n_accept window, 2 n_out, 3 status,
                       //   4 next proposal of the outer step, 5 rebuild pending, 6 outer step done, 7 spare
-  int* nflag;         // [1] walkers handed to the rebuild kernel by the last k_block launch
+  int* nflag;         // [4]: 0 walkers handed to the rebuild kernel by the last k_block launch,
+                      //   1 next walker taken by a k_block slot, 2 next hand-over entry taken by k_rebuild
+  int* rlist;         // [W] walkers handed to the rebuild kernel, in hand-over order
   long long* counter; // [W][4]
   // saved samples
   int* s_idx;  int* s_cfg;  int* s_num;  int* s_proj;  // [W][nsample][...]

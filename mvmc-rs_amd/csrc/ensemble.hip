@@ -766,7 +766,16 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v, Elem* inv, int out_
   double* pv1 = v.pend + (size_t)blockIdx.x * 2 * nqp * n;  // pending hop update: vec1
   double* pv2 = pv1 + (size_t)nqp * n;                      //                     vec2
   PHASE_INIT
-  for (int w = blockIdx.x; w < v.W; w += gridDim.x) {
+  __shared__ int next_w;
+  for (;;) {
+    // walkers are taken dynamically, so the launch ends when the work does
+    // (a walker's results do not depend on the slot that runs it: the
+    // slot's scratch is rewritten before it is read)
+    if (tid == 0) next_w = atomicAdd(v.nflag + 1, 1);
+    __syncthreads();
+    const int w = next_w;
+    __syncthreads();
+    if (w >= v.W) break;
     int* ws = v.wstate + (size_t)w * 8;
     int pm = -1;  // row of the pending update (-1: none); uniform over the workgroup
     if (tid == 0) {
@@ -1153,7 +1162,7 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v, Elem* inv, int out_
         ws[1] = L.misc[M_NACC];
         ws[4] = stop_at;
         ws[5] = 1;
-        atomicAdd(v.nflag, 1);
+        v.rlist[atomicAdd(v.nflag, 1)] = w;
       }
       __syncthreads();
       PHASE(6);
@@ -1210,12 +1219,18 @@ __global__ void __launch_bounds__(256, 3) k_rebuild(EnsView v) {
   char* pfscr = (char*)c.take<char>(pf_bytes(v.n, v.nqp));
   const int tid = threadIdx.x;
   double* work = v.work + (size_t)blockIdx.x * pfinv_work_doubles(v.n, v.nqp);
-  for (int w = blockIdx.x; w < v.W; w += gridDim.x) {
-    int* ws = v.wstate + (size_t)w * 8;
-    __shared__ int flag;
-    if (tid == 0) flag = ws[5];
+  const int nhand = *v.nflag;  // walkers handed over by the k_block launch before
+  __shared__ int next_i;
+  for (;;) {
+    // hand-over entries are taken dynamically: every slot stays busy until
+    // the list is done
+    if (tid == 0) next_i = atomicAdd(v.nflag + 2, 1);
     __syncthreads();
-    if (!flag) continue;
+    const int li = next_i;
+    __syncthreads();
+    if (li >= nhand) break;
+    const int w = v.rlist[li];
+    int* ws = v.wstate + (size_t)w * 8;
     for (int i = tid; i < v.n; i += blockDim.x) ei[i] = v.ele_idx[(size_t)w * v.n + i];
     __syncthreads();
     double* planes = v.inv + (size_t)w * v.nqp * v.n * v.n;
