@@ -212,6 +212,30 @@ def main():
                      "kernel": "k_block", "avg_launch_ms": avg_launch_s * 1e3, "launches": launches,
                      "algorithmic_bytes_per_launch": bytes_total / max(launches, 1)},
     }
+    # the other rows' kernels in the timed step, against their algorithmic bytes
+    # (HBM 8 TB/s) and flops (FP64 vector 78.6 TFLOP/s)
+    tl = ens.timings()
+    plane_b = 2 * n * n * 8 + 12            # read A, write invM (+ status, Pf)
+    plane_f = 2 * n ** 3                   # LTL (2/3 n^3) + trtri (1/3) + trmm (1)
+    sec = {}
+    for key, cnt, ms in (("k_rebuild (a): periodic Pf + inverse", tl["rebuilt"], tl["rebuild_ms"]),
+                         ("k_meas_rebuild (a): Pf + inverse per saved sample", tl["measured"], tl["meas_rebuild_ms"])):
+        if ms > 0 and cnt > 0:
+            sec[key] = {"planes": cnt * nqp, "ms": ms, "achieved_GBs": cnt * nqp * plane_b / (ms / 1e3) / 1e9,
+                        "frac_hbm": cnt * nqp * plane_b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                        "achieved_TFLOPs": cnt * nqp * plane_f / (ms / 1e3) / 1e12,
+                        "frac_fp64": cnt * nqp * plane_f / (ms / 1e3) / 1e12 / 78.6}
+    if tl["measure_ms"] > 0:
+        mb = tl["measured"] * nqp * (n * n + info["ntransfer"] * n) * 8
+        sec["k_measure (c): Green ratios + O"] = {"samples": tl["measured"], "ms": tl["measure_ms"],
+                                                   "achieved_GBs": mb / (tl["measure_ms"] / 1e3) / 1e9,
+                                                   "frac_hbm": mb / (tl["measure_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS}
+    if tl["cg_ms"] > 0 and tl["cg_iter"] > 0:
+        cb = tl["cg_iter"] * tl["cg_rows"] * tl["cg_na"] * 8   # one pass over the packed samples per S.x
+        sec["k_cg_* (d): SR-CG"] = {"iterations": tl["cg_iter"], "ms": tl["cg_ms"],
+                                    "achieved_GBs": cb / (tl["cg_ms"] / 1e3) / 1e9,
+                                    "frac_hbm": cb / (tl["cg_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS}
+    out["secondary_rooflines"] = sec
     if not args.no_mixed:
         # precision form (1): the same workload, seeds and step count with the
         # sampler's resident inverse in FP32; energies against the FP64 rows

@@ -239,6 +239,9 @@ struct mvmc_ens {
   // FP32 (inv32), everything else FP64
   int precision = 0;
   float* inv32 = nullptr;
+  // device times of the last calls (HIP events on the library's stream)
+  double rebuilt = 0, meas_rebuild_ms = 0, measure_ms = 0, measured = 0, cg_ms = 0, cg_iter = 0, cg_na = 0,
+         cg_rows = 0;
   ~mvmc_ens() {
     cg.release();
     for (void* p : allocs) (void)hipFree(p);
@@ -502,6 +505,7 @@ void sample_impl(mvmc_ens* E) {
   HIPCHK(hipEventCreate(&r1));
   double block_ms = 0, rebuild_ms = 0;
   int launches = 0, rebuilds = 0;
+  long long rebuilt = 0;
   for (int out = 0; out < n_out; ++out) {
     hipLaunchKernelGGL(k_reset_resume, dim3(nblk(v.W)), dim3(256), 0, E->st, v);
     for (;;) {
@@ -521,6 +525,7 @@ void sample_impl(mvmc_ens* E) {
       block_ms += ms;
       ++launches;
       if (nf == 0) break;
+      rebuilt += nf;
       HIPCHK(hipEventRecord(r0, E->st));
       hipLaunchKernelGGL(k_rebuild, dim3(E->grid_rebuild), dim3(256), E->lds_rebuild, E->st, v);
       HIPCHK(hipGetLastError());
@@ -536,6 +541,7 @@ void sample_impl(mvmc_ens* E) {
   E->block_launches = launches;
   E->rebuild_ms = rebuild_ms;
   E->rebuild_launches = rebuilds;
+  E->rebuilt = (double)rebuilt;
   E->check_status();
   std::vector<long long> c((size_t)v.W * 4);
   HIPCHK(hipMemcpy(c.data(), v.counter, sizeof(long long) * c.size(), hipMemcpyDeviceToHost));
@@ -549,12 +555,28 @@ void sample_impl(mvmc_ens* E) {
 void measure_impl(mvmc_ens* E) {
   EnsView& v = E->v;
   const double t0 = now_s();
+  hipEvent_t ev[3];
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  double ms_rb = 0, ms_m = 0;
   for (int s = 0; s < v.nsample; ++s) {
+    HIPCHK(hipEventRecord(ev[0], E->st));
     hipLaunchKernelGGL(k_meas_rebuild, dim3(E->grid_rebuild), dim3(256), E->lds_rebuild, E->st, v, s);
+    HIPCHK(hipEventRecord(ev[1], E->st));
     hipLaunchKernelGGL(k_measure, dim3(E->grid_meas), dim3(256), E->lds_meas, E->st, v, E->d_gscr, E->gstride, 1,
                        s);
+    HIPCHK(hipEventRecord(ev[2], E->st));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventSynchronize(ev[2]));
+    float a = 0, b = 0;
+    HIPCHK(hipEventElapsedTime(&a, ev[0], ev[1]));
+    HIPCHK(hipEventElapsedTime(&b, ev[1], ev[2]));
+    ms_rb += a;
+    ms_m += b;
   }
-  HIPCHK(hipGetLastError());
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  E->meas_rebuild_ms = ms_rb;
+  E->measure_ms = ms_m;
+  E->measured = (double)v.W * v.nsample;
   hipLaunchKernelGGL(k_reduce_energy, dim3(1), dim3(64), 0, E->st, v, E->d_out5);
   hipLaunchKernelGGL(k_reduce_ops, dim3(nblk(v.size)), dim3(256), 0, E->st, v, E->d_ho, E->d_mean, E->d_diag);
   HIPCHK(hipGetLastError());
@@ -634,9 +656,25 @@ int sr_impl(mvmc_ens* E, long long* info_out) {
   HIPCHK(hipGetLastError());
   const int max_it = m.mp.nsr_opt_cg_max_iter > 0 ? m.mp.nsr_opt_cg_max_iter : na;
   const double inv_w = 1.0 / E->wc;
+  hipEvent_t c0, c1;
+  HIPCHK(hipEventCreate(&c0));
+  HIPCHK(hipEventCreate(&c1));
+  HIPCHK(hipEventRecord(c0, st));
   long long status = cg_solve_dev(na, v.W, v.nsample, E->d_pack, E->d_cmean, E->d_cdiag, E->d_cg, inv_w,
                                   m.mp.dsr_opt_sta_del, m.mp.dsr_opt_cg_tol, max_it, m.mp.use_diag_scale != 0,
                                   E->d_cx, E->cg, st);
+  HIPCHK(hipEventRecord(c1, st));
+  HIPCHK(hipEventSynchronize(c1));
+  {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c0, c1));
+    E->cg_ms = ms;
+    E->cg_iter = (double)(status >= 0 ? status : -status - 1);
+    E->cg_na = na;
+    E->cg_rows = (double)rows;
+  }
+  (void)hipEventDestroy(c0);
+  (void)hipEventDestroy(c1);
   long long info = status;
   if (status < 0) {
     if (m.mp.nsrcg_fallback) {
@@ -855,6 +893,8 @@ int mvmc_ens_phase_clocks(mvmc_ens* e, long long* out) {
 int mvmc_ens_timings(mvmc_ens* e, double* t) {
   t[0] = e->t_sample; t[1] = e->t_measure; t[2] = e->t_sr; t[3] = e->proposals; t[4] = e->accepted;
   t[5] = e->block_ms; t[6] = e->block_launches; t[7] = e->rebuild_ms;
+  t[8] = e->rebuilt; t[9] = e->meas_rebuild_ms; t[10] = e->measure_ms; t[11] = e->measured;
+  t[12] = e->cg_ms; t[13] = e->cg_iter; t[14] = e->cg_na; t[15] = e->cg_rows;
   return MVMC_OK;
 }
 
