@@ -99,6 +99,22 @@ int mvmc_ens_opt_step(mvmc_ens* e, double* energy_row6);
 int mvmc_ens_get_energy(mvmc_ens* e, double* wc_etot_etot2_sz_sz2);
 int mvmc_ens_get_operands(mvmc_ens* e, double* oo_2size, double* ho_size);
 int mvmc_ens_get_o_store(mvmc_ens* e, int walker, double* o_nsample_size);
+
+/* Differentiation form.  The measurement's O rows are the per-sample
+ * adjoints O[r] = [1, d log|psi(x_r)| / d p] (projection counts and the
+ * Slater derivative, slater_derivative.rs:289-428,490-548,
+ * observables.rs:202-366), r = walker * NVMCSample + sample.
+ * mvmc_ens_vjp: grad[k] = sum_r cot[r] * O[r][k] (r ascending; host arrays
+ * cot[W*NVMCSample], grad[1+NPara]) over the last mvmc_ens_measure.
+ * mvmc_ens_log_psi: log|psi(x_r)| = log|sum_q w_q Pf_q| + sum_p proj_p
+ * cnt_p(x_r) of every saved sample at the current parameters (NaN for an
+ * invalid sample), the function the O rows differentiate (finite-difference
+ * checks).  It rebuilds the measurement planes, so call it between
+ * mvmc_ens_measure and the next mvmc_ens_sample only for checks.  No
+ * reference entry points: they expose the derivative the reference computes
+ * inside its measurement. */
+int mvmc_ens_vjp(mvmc_ens* e, const double* cot, double* grad);
+int mvmc_ens_log_psi(mvmc_ens* e, double* log_psi);
 int mvmc_ens_get_walker(mvmc_ens* e, int walker, int* saved_ele_idx, long long* counters4, uint32_t* rng625);
 int mvmc_ens_get_tables(mvmc_ens* e, int walker, double* inv, double* pf);
 int mvmc_ens_get_config(mvmc_ens* e, int walker, int* ele_idx, int* ele_cfg, int* ele_num, int* proj_cnt);

@@ -239,6 +239,8 @@ struct mvmc_ens {
   // FP32 (inv32), everything else FP64
   int precision = 0;
   float* inv32 = nullptr;
+  double* d_rows = nullptr;  // [W*nsample] cotangents / log|psi| of the saved samples
+  double* d_grad = nullptr;  // [size]
   // device times of the last calls (HIP events on the library's stream)
   double rebuilt = 0, meas_rebuild_ms = 0, measure_ms = 0, measured = 0, cg_ms = 0, cg_iter = 0, cg_na = 0,
          cg_rows = 0;
@@ -867,6 +869,36 @@ int mvmc_ens_get_cg(mvmc_ens* e, double* x, int* n_active) {
   if (x) std::copy(e->cg_x.begin(), e->cg_x.end(), x);
   if (n_active) *n_active = (int)e->cg_x.size();
   return (int)e->cg_info;
+}
+
+int mvmc_ens_log_psi(mvmc_ens* e, double* out) {
+  return guarded([&] {
+    EnsView& v = e->v;
+    const size_t rows = (size_t)v.W * v.nsample;
+    if (!e->d_rows) e->d_rows = dalloc<double>(e->allocs, rows);
+    e->upload_tables();
+    for (int s = 0; s < v.nsample; ++s) {
+      hipLaunchKernelGGL(k_meas_rebuild, dim3(e->grid_rebuild), dim3(256), e->lds_rebuild, e->st, v, s);
+      hipLaunchKernelGGL(k_log_psi, dim3(nblk(v.W)), dim3(256), 0, e->st, v, s, e->d_rows);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, e->d_rows, sizeof(double) * rows, hipMemcpyDeviceToHost, e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
+  });
+}
+
+int mvmc_ens_vjp(mvmc_ens* e, const double* cot, double* grad) {
+  return guarded([&] {
+    EnsView& v = e->v;
+    const size_t rows = (size_t)v.W * v.nsample;
+    if (!e->d_rows) e->d_rows = dalloc<double>(e->allocs, rows);
+    if (!e->d_grad) e->d_grad = dalloc<double>(e->allocs, (size_t)v.size);
+    HIPCHK(hipMemcpyAsync(e->d_rows, cot, sizeof(double) * rows, hipMemcpyHostToDevice, e->st));
+    hipLaunchKernelGGL(k_vjp, dim3(nblk(v.size)), dim3(256), 0, e->st, v, e->d_rows, e->d_grad);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(grad, e->d_grad, sizeof(double) * v.size, hipMemcpyDeviceToHost, e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
+  });
 }
 
 int mvmc_ens_set_precision(mvmc_ens* e, int mode) {

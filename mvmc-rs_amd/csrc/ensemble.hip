@@ -1592,6 +1592,37 @@ __global__ void __launch_bounds__(256, 2) k_measure(EnsView v, double* gscratch,
 // multichain.rs:474-600): per walker the sum over its samples, then a left
 // fold over walkers.
 // ---------------------------------------------------------------------------
+// log|psi(x)| = log|<x|phi>| + sum_p proj_p cnt_p(x) of saved sample s of
+// every walker at the current parameters (calculate_ip_real,
+// observables.rs:128-145, and the projection factor), after
+// k_meas_rebuild(s); NaN for an invalid sample.  One thread per walker.
+__global__ void k_log_psi(EnsView v, int s, double* out) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= v.W) return;
+  const size_t ws = (size_t)w * v.nsample + s;
+  double r = __builtin_nan("");
+  if (v.s_valid[ws]) {
+    const double ip = ip_of(v, v.pf + (size_t)w * v.nqp);
+    double lp = 0.0;
+    for (int p = 0; p < v.nproj; ++p) lp += v.proj[p] * (double)v.s_proj[ws * v.nproj + p];
+    r = ddm::cr_log(fabs(ip)) + lp;
+  }
+  out[ws] = r;
+}
+
+// Vector-Jacobian product over the stored samples: grad[k] = sum_r cot[r] *
+// O[r][k], r ascending (r = walker * NVMCSample + sample), O = [1,
+// d log|psi| / d p] per sample as the measurement stored it.  One thread
+// per component; consecutive threads read consecutive components.
+__global__ void k_vjp(EnsView v, const double* cot, double* grad) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= v.size) return;
+  const size_t rows = (size_t)v.W * v.nsample;
+  double acc = 0.0;
+  for (size_t r = 0; r < rows; ++r) acc += cot[r] * v.o_store[r * v.size + k];
+  grad[k] = acc;
+}
+
 __global__ void k_reduce_energy(EnsView v, double* out5) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   double tot[5] = {0, 0, 0, 0, 0};

@@ -61,6 +61,7 @@ EXPORTED_SYMBOLS = [
     "mvmc_ens_measure", "mvmc_ens_sr_cg", "mvmc_ens_opt_step", "mvmc_ens_get_energy", "mvmc_ens_get_operands",
     "mvmc_ens_get_o_store", "mvmc_ens_get_walker", "mvmc_ens_get_tables", "mvmc_ens_get_config", "mvmc_ens_get_cg",
     "mvmc_ens_timings", "mvmc_ens_phase_clocks", "mvmc_ens_status", "mvmc_sr_cg_solve", "mvmc_ens_set_precision",
+    "mvmc_ens_vjp", "mvmc_ens_log_psi",
 ]
 
 
@@ -133,6 +134,22 @@ class Ensemble:
         s = np.zeros(max(self.nslater, 1))
         lib().mvmc_ens_get_params(self.h, _p(p), _p(s))
         return p[:self.nproj].copy(), s[:self.nslater].copy()
+
+    def vjp(self, cot):
+        """sum_r cot[r] * O[r] over the measured samples (r = walker * NVMCSample
+        + sample): [sum cot, sum cot * d log|psi| / d p]."""
+        c = np.ascontiguousarray(cot, dtype=np.float64).reshape(-1)
+        if c.size != self.walkers * self.nsample:
+            raise InvalidShape(f"cotangent length {c.size} != walkers * NVMCSample")
+        g = np.zeros(self.size)
+        _check(lib().mvmc_ens_vjp(self.h, _p(c), _p(g)))
+        return g
+
+    def log_psi(self):
+        """log|psi| of every saved sample at the current parameters, [W, NVMCSample]."""
+        out = np.zeros(self.walkers * self.nsample)
+        _check(lib().mvmc_ens_log_psi(self.h, _p(out)))
+        return out.reshape(self.walkers, self.nsample)
 
     def set_precision(self, mode):
         """0: FP64 throughout; 1 ("mixed"): the sampler's resident inverse in
