@@ -126,6 +126,7 @@ def main():
     ap.add_argument("--cpu-walkers", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-direct", action="store_true", help="skip the direct-SR measurement")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the precision-form run (sampler inverse in FP32) reported beside the FP64 line")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -235,6 +236,18 @@ def main():
         sec["k_cg_* (d): SR-CG"] = {"iterations": tl["cg_iter"], "ms": tl["cg_ms"],
                                     "achieved_GBs": cb / (tl["cg_ms"] / 1e3) / 1e9,
                                     "frac_hbm": cb / (tl["cg_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS}
+    # direct SR (NSRCG 0 path) on the same measured operands, after the timed
+    # steps: Gram by DGEMM on the matrix cores + Cholesky
+    if not args.no_direct:
+        ens.sr_direct()   # first call: rocBLAS / rocSOLVER initialisation
+        ens.sr_direct()
+        td = ens.timings()
+        na_, rows_ = td["cg_na"], td["cg_rows"]
+        fl = 2.0 * na_ * na_ * rows_ + na_ ** 3 / 3.0 + 2.0 * na_ * na_   # DGEMM (full Gram) + Cholesky
+        sec["direct SR (d, NSRCG 0): DGEMM + DPOTRF/DPOTRS"] = {
+            "active_parameters": na_, "rows": rows_, "ms": td["cg_ms"],
+            "achieved_TFLOPs": fl / (td["cg_ms"] / 1e3) / 1e12,
+            "frac_fp64_mfma": fl / (td["cg_ms"] / 1e3) / 1e12 / 78.6}
     out["secondary_rooflines"] = sec
     if not args.no_mixed:
         # precision form (1): the same workload, seeds and step count with the

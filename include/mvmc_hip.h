@@ -90,6 +90,14 @@ int mvmc_ens_measure(mvmc_ens* e);
  * (crates/mvmc-core/src/sr_cg.rs:201-484, sync.rs:18-104).  cg_info gets the C
  * status (iterations or -(iter+1)). */
 int mvmc_ens_sr_cg(mvmc_ens* e, long long* cg_info);
+/* Direct SR on the reduced operands (crates/mvmc-core/src/sr.rs:60-160,
+ * stochastic_opt_real; assembly sr_backend.rs:484-516): Gram of the packed
+ * active samples by rocBLAS DGEMM (matrix cores), S and g as the reference
+ * assembles them, Cholesky by rocSOLVER DPOTRF/DPOTRS, parameter update and
+ * gauge sync.  The path NSRCG 0 selects in mvmc_ens_opt_step; skipped gets
+ * 1 when the factorisation fails or the solution is not finite (no update),
+ * as the reference's cholesky_solve. */
+int mvmc_ens_sr_direct(mvmc_ens* e, int* skipped);
 
 /* sample + measure + SR-CG; energy_row6 gets the zvo_out.dat row
  * (Etot, Im Etot, Etot2, (Etot2-Etot^2)/Etot^2, Sztot, Sztot2). */

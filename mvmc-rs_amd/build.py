@@ -36,7 +36,8 @@ def build_library(defines=(), out=None):
     tmp = out + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-Wno-unused-result"] + [f"-D{d}" for d in defines] + ["-o", tmp] + srcs
+           "-Wno-unused-result"] + [f"-D{d}" for d in defines] + ["-o", tmp] + srcs + \
+          ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
     _run(cmd)
     os.replace(tmp, out)
     return out

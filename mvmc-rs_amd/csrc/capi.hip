@@ -3,6 +3,8 @@
 // kernels of ensemble.hip (crates/mvmc-core/src/run.rs:1294-1475 for the
 // step order; multichain.rs:647-689 for the walker = rank mapping).
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <chrono>
@@ -35,6 +37,12 @@ struct Fail : std::runtime_error {
     if (e_ != hipSuccess)                                                                         \
       throw Fail(MVMC_E_BACKEND, std::string(#x) + ": " + hipGetErrorString(e_) + " (" + __FILE__ + \
                                      ":" + std::to_string(__LINE__) + ")");                       \
+  } while (0)
+#define BLASCHK(x)                                                                                \
+  do {                                                                                            \
+    rocblas_status s_ = (x);                                                                      \
+    if (s_ != rocblas_status_success)                                                             \
+      throw Fail(MVMC_E_BACKEND, std::string(#x) + ": rocBLAS status " + std::to_string((int)s_));  \
   } while (0)
 
 double now_s() {
@@ -240,12 +248,19 @@ struct mvmc_ens {
   int precision = 0;
   float* inv32 = nullptr;
   double* d_rows = nullptr;  // [W*nsample] cotangents / log|psi| of the saved samples
+  // direct SR (NSRCG 0): rocBLAS handle on the library's stream, S [na][na], Cholesky info
+  rocblas_handle blas = nullptr;
+  double* d_S = nullptr;
+  size_t S_cap = 0;
+  int* d_info = nullptr;
   double* d_grad = nullptr;  // [size]
   // device times of the last calls (HIP events on the library's stream)
   double rebuilt = 0, meas_rebuild_ms = 0, measure_ms = 0, measured = 0, cg_ms = 0, cg_iter = 0, cg_na = 0,
          cg_rows = 0;
   ~mvmc_ens() {
     cg.release();
+    if (d_S) (void)hipFree(d_S);
+    if (blas) (void)rocblas_destroy_handle(blas);
     for (void* p : allocs) (void)hipFree(p);
     if (st) (void)hipStreamDestroy(st);
   }
@@ -605,13 +620,94 @@ void measure_impl(mvmc_ens* E) {
   E->t_measure += now_s() - t0;
 }
 
-// sr_cg.rs:201-484 (real parameters, offset 1, no RescaleSmat)
-int sr_impl(mvmc_ens* E, long long* info_out) {
+// Parameter update of an SR step (sr.rs:408-421, sync.rs): p[map[si]] += x[si].
+void apply_sr_update(mvmc_ens* E, const std::vector<int>& map, const std::vector<double>& x) {
+  mvmc::Model& m = E->m;
+  for (size_t si = 0; si < map.size(); ++si) {
+    const int pi = map[si];
+    if (pi < m.n_proj()) m.proj[pi] += x[si];
+    else m.slater[pi - m.n_proj()] += x[si];
+  }
+  mvmc::sync_modified_parameter(m, true);
+}
+
+// Direct SR (sr.rs:60-160 real path; sr_backend.rs:484-516 assembly): the
+// Gram inv_w * P P^T of the packed active samples (P = [na][rows] column
+// major) by rocBLAS DGEMM on the matrix cores, S = Gram - mean mean^T with
+// the diagonal scaled by 1 + DSROptStaDel, then Cholesky (rocSOLVER DPOTRF,
+// DPOTRS, upper) on g in place.  A failed factorisation or a non-finite
+// solution skips the update (return 1), as the reference's cholesky_solve.
+int direct_sr(mvmc_ens* E, const std::vector<int>& map, int na, size_t rows, double inv_w, double t0,
+              long long* info_out) {
+  mvmc::Model& m = E->m;
+  hipStream_t st = E->st;
+  if (!E->blas) {
+    BLASCHK(rocblas_create_handle(&E->blas));
+    E->d_info = dalloc<int>(E->allocs, 4);
+  }
+  BLASCHK(rocblas_set_stream(E->blas, st));
+  const size_t nn = (size_t)na * na;
+  if (E->S_cap < nn) {
+    if (E->d_S) (void)hipFree(E->d_S);
+    E->d_S = nullptr;
+    HIPCHK(hipMalloc(&E->d_S, sizeof(double) * nn));
+    E->S_cap = nn;
+  }
+  hipEvent_t c0, c1;
+  HIPCHK(hipEventCreate(&c0));
+  HIPCHK(hipEventCreate(&c1));
+  HIPCHK(hipEventRecord(c0, st));
+  const double zero = 0.0;
+  // the full Gram by DGEMM (P P^T): twice the flops of DSYRK, but DGEMM is
+  // the matrix-core kernel the library tunes (measured ~9x faster here)
+  BLASCHK(rocblas_dgemm(E->blas, rocblas_operation_none, rocblas_operation_transpose, na, na, (rocblas_int)rows,
+                        &inv_w, E->d_pack, na, E->d_pack, na, &zero, E->d_S, na));
+  hipLaunchKernelGGL(k_sr_assemble, dim3(nblk((long long)nn)), dim3(256), 0, st, na, E->d_cmean,
+                     1.0 + m.mp.dsr_opt_sta_del, E->d_S);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(E->d_cx, E->d_cg, sizeof(double) * na, hipMemcpyDeviceToDevice, st));
+  BLASCHK(rocsolver_dpotrf(E->blas, rocblas_fill_upper, na, E->d_S, na, E->d_info));
+  int info = 0;
+  HIPCHK(hipMemcpyAsync(&info, E->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (info == 0) BLASCHK(rocsolver_dpotrs(E->blas, rocblas_fill_upper, na, 1, E->d_S, na, E->d_cx, na));
+  HIPCHK(hipEventRecord(c1, st));
+  HIPCHK(hipEventSynchronize(c1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, c0, c1));
+  (void)hipEventDestroy(c0);
+  (void)hipEventDestroy(c1);
+  E->cg_ms = ms;
+  E->cg_iter = 0;
+  E->cg_na = na;
+  E->cg_rows = (double)rows;
+  E->cg_info = 0;
+  *info_out = 0;
+  E->cg_x.clear();
+  if (info != 0) {
+    E->t_sr += now_s() - t0;
+    return 1;
+  }
+  std::vector<double> x(na);
+  HIPCHK(hipMemcpy(x.data(), E->d_cx, sizeof(double) * na, hipMemcpyDeviceToHost));
+  E->cg_x = x;
+  for (double xv : x)
+    if (!std::isfinite(xv)) {
+      E->t_sr += now_s() - t0;
+      return 1;
+    }
+  apply_sr_update(E, map, x);
+  E->t_sr += now_s() - t0;
+  return 0;
+}
+
+// sr_cg.rs:201-484 (real parameters, offset 1, no RescaleSmat); NSRCG 0
+// takes the direct path above.
+int sr_impl(mvmc_ens* E, long long* info_out, bool force_direct = false) {
   mvmc::Model& m = E->m;
   EnsView& v = E->v;
   const double t0 = now_s();
   const int np = m.n_para(), size = m.sr_opt_size();
-  if (m.mp.nsrcg == 0) throw Fail(MVMC_E_INPUT, "the device solver implements SR-CG (NSRCG 1) only");
   if (m.mp.rescale_smat) throw Fail(MVMC_E_INPUT, "RescaleSmat is not supported");
   std::vector<double> var(np);
   bool nan = false;
@@ -656,8 +752,9 @@ int sr_impl(mvmc_ens* E, long long* info_out) {
   }
   hipLaunchKernelGGL(k_cg_pack, dim3(nblk((long long)(rows * na))), dim3(256), 0, st, v, E->d_map, na, E->d_pack);
   HIPCHK(hipGetLastError());
-  const int max_it = m.mp.nsr_opt_cg_max_iter > 0 ? m.mp.nsr_opt_cg_max_iter : na;
   const double inv_w = 1.0 / E->wc;
+  if (m.mp.nsrcg == 0 || force_direct) return direct_sr(E, map, na, rows, inv_w, t0, info_out);
+  const int max_it = m.mp.nsr_opt_cg_max_iter > 0 ? m.mp.nsr_opt_cg_max_iter : na;
   hipEvent_t c0, c1;
   HIPCHK(hipEventCreate(&c0));
   HIPCHK(hipEventCreate(&c1));
@@ -701,12 +798,7 @@ int sr_impl(mvmc_ens* E, long long* info_out) {
       E->t_sr += now_s() - t0;
       return 1;
     }
-  for (int si = 0; si < na; ++si) {
-    const int pi = map[si];
-    if (pi < m.n_proj()) m.proj[pi] += x[si];
-    else m.slater[pi - m.n_proj()] += x[si];
-  }
-  mvmc::sync_modified_parameter(m, true);
+  apply_sr_update(E, map, x);
   E->t_sr += now_s() - t0;
   return 0;
 }
@@ -727,7 +819,7 @@ int mvmc_pfinv_f64(const double* planes, int n, int count, double* pf, double* i
     hipStream_t st = (hipStream_t)stream;
     if ((const double*)inv != planes)
       HIPCHK(hipMemcpyAsync(inv, planes, sizeof(double) * n * n * count, hipMemcpyDeviceToDevice, st));
-    const int np = n <= 32 ? 8 : (n <= 64 ? 4 : 1);
+    const int np = n <= 32 ? 8 : 4;  // the inverse runs one wave per plane: 4 planes fill a workgroup
     const int blocks = (count + np - 1) / np;
     double* work = nullptr;
     HIPCHK(hipMallocAsync((void**)&work, sizeof(double) * (size_t)blocks * pfinv_work_doubles(n, np), st));
@@ -787,6 +879,14 @@ int mvmc_ens_measure(mvmc_ens* e) { return guarded([&] { measure_impl(e); }); }
 int mvmc_ens_sr_cg(mvmc_ens* e, long long* info) {
   long long dummy = 0;
   return guarded([&] { sr_impl(e, info ? info : &dummy); });
+}
+
+int mvmc_ens_sr_direct(mvmc_ens* e, int* skipped) {
+  return guarded([&] {
+    long long info = 0;
+    const int rc = sr_impl(e, &info, true);
+    if (skipped) *skipped = rc;
+  });
 }
 
 int mvmc_ens_opt_step(mvmc_ens* e, double* row) {

@@ -1592,6 +1592,18 @@ __global__ void __launch_bounds__(256, 2) k_measure(EnsView v, double* gscratch,
 // multichain.rs:474-600): per walker the sum over its samples, then a left
 // fold over walkers.
 // ---------------------------------------------------------------------------
+// Direct SR assembly on the upper triangle (sr_backend.rs:484-505):
+// S[i][j] = Gram[i][j] - mean[i] * mean[j], the diagonal times ratio_diag.
+__global__ void k_sr_assemble(int na, const double* mean, double ratio_diag, double* S) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)na * na) return;
+  const int i = (int)(e % na), j = (int)(e / na);
+  if (i > j) return;
+  double x = S[e] - mean[i] * mean[j];
+  if (i == j) x *= ratio_diag;
+  S[e] = x;
+}
+
 // log|psi(x)| = log|<x|phi>| + sum_p proj_p cnt_p(x) of saved sample s of
 // every walker at the current parameters (calculate_ip_real,
 // observables.rs:128-145, and the projection factor), after

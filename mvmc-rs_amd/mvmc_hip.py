@@ -61,7 +61,7 @@ EXPORTED_SYMBOLS = [
     "mvmc_ens_measure", "mvmc_ens_sr_cg", "mvmc_ens_opt_step", "mvmc_ens_get_energy", "mvmc_ens_get_operands",
     "mvmc_ens_get_o_store", "mvmc_ens_get_walker", "mvmc_ens_get_tables", "mvmc_ens_get_config", "mvmc_ens_get_cg",
     "mvmc_ens_timings", "mvmc_ens_phase_clocks", "mvmc_ens_status", "mvmc_sr_cg_solve", "mvmc_ens_set_precision",
-    "mvmc_ens_vjp", "mvmc_ens_log_psi",
+    "mvmc_ens_vjp", "mvmc_ens_log_psi", "mvmc_ens_sr_direct",
 ]
 
 
@@ -123,6 +123,12 @@ class Ensemble:
         info = ctypes.c_longlong(0)
         _check(lib().mvmc_ens_sr_cg(self.h, ctypes.byref(info)))
         return info.value
+
+    def sr_direct(self):
+        """Direct SR (DSYRK + Cholesky) on the last measurement; 1 if skipped."""
+        sk = ctypes.c_int(0)
+        _check(lib().mvmc_ens_sr_direct(self.h, ctypes.byref(sk)))
+        return sk.value
 
     def opt_step(self):
         row = np.zeros(6)

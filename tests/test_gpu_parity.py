@@ -277,3 +277,43 @@ def test_derivative_form_vjp_and_finite_differences(M):
         col = 1 + k if kind == "proj" else 1 + len(proj) + k
         assert np.all(np.abs(fd - O[:, col]) <= 1e-6 * np.maximum(1.0, np.abs(O[:, col]))), (kind, k, fd, O[:, col])
     assert np.array_equal(ens.log_psi().reshape(-1), lp0)
+
+
+def test_direct_sr_matches_dense_solve(M, tmp_path):
+    """Direct SR (NSRCG 0; sr.rs:60-160): the device Gram (rocBLAS DGEMM), S/g
+    assembly (sr_backend.rs:484-505: S = <OO> - <O><O>^T, diagonal times
+    1 + DSROptStaDel; g = -2 dt (<HO> - <H><O>)) and Cholesky solve give the
+    solution of the same system formed on the host from the downloaded O rows
+    (numpy, independent summation order: tolerance scaled by cond(S)).
+    Bit-level parity with the reference's OpenBLAS/LAPACK order: unpinned."""
+    import shutil
+    log("direct SR")
+    dst = tmp_path / "hub_direct"
+    shutil.copytree(os.path.dirname(HUB), dst)
+    mp = dst / "modpara.def"
+    txt = mp.read_text()
+    assert "NSRCG 1" in txt
+    mp.write_text(txt.replace("NSRCG 1", "NSRCG 0"))
+    W = 4
+    ens = M.Ensemble(str(dst / "namelist.def"), walkers=W)
+    ens.sample()
+    ens.measure()
+    size = ens.size
+    wc = ens.energy()[0]
+    oo, ho = ens.operands()
+    mean, var = oo[1:size], oo[size + 1:] - oo[1:size] ** 2
+    act = np.nonzero(var >= var.max() * 1e-8)[0]          # DSROptRedCut 1e-8
+    O = np.concatenate([ens.o_store(w).reshape(ens.nsample, size) for w in range(W)])
+    P = O[:, 1 + act]
+    S = P.T @ P / wc - np.outer(mean[act], mean[act])
+    S[np.diag_indices_from(S)] *= 1.0 + 0.01              # DSROptStaDel
+    g = -2.0 * 0.003 * (ho[1 + act] - ho[0] * mean[act])  # DSROptStepDt
+    x_ref = np.linalg.solve(S, g)
+    assert ens.sr_direct() == 0
+    _, x = ens.cg()
+    assert len(x) == len(act)
+    tol = max(1e-12, 1e-14 * np.linalg.cond(S)) * np.abs(x_ref).max()
+    assert np.abs(x - x_ref).max() <= tol, (np.abs(x - x_ref).max(), tol)
+    # a full step through the NSRCG 0 path
+    row = ens.opt_step()
+    assert np.isfinite(row).all()
