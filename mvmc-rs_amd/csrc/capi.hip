@@ -11,6 +11,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <set>
 #include <stdexcept>
 #include <string>
@@ -193,6 +195,19 @@ long long cg_solve_dev(int na, int W, int NS, const double* pack, const double* 
   HIPCHK(hipMemcpyAsync(x_out, wk.x, sizeof(double) * na, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipStreamSynchronize(st));
   return status;
+}
+
+// A kernel's dynamic-LDS limit is a process-wide attribute while every
+// ensemble (and batched call) needs its own size: the limit only grows, so
+// handles of different models can live and run side by side.
+void raise_lds_limit(const void* fn, size_t bytes) {
+  static std::mutex mu;
+  static std::map<const void*, size_t> cur;
+  std::lock_guard<std::mutex> lk(mu);
+  size_t& c = cur[fn];
+  if (bytes <= c) return;
+  HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  c = bytes;
 }
 
 template <class T>
@@ -444,17 +459,13 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
   if (E->lds_samp > (size_t)maxlds || E->lds_meas > (size_t)maxlds)
     throw Fail(MVMC_E_INVALID_SHAPE, "model needs " + std::to_string(std::max(E->lds_samp, E->lds_meas)) +
                                          " B of LDS per workgroup, device allows " + std::to_string(maxlds));
-  HIPCHK(hipFuncSetAttribute((const void*)k_block<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)E->lds_samp));
-  HIPCHK(hipFuncSetAttribute((const void*)k_block<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)E->lds_samp));
-  HIPCHK(hipFuncSetAttribute((const void*)k_begin, hipFuncAttributeMaxDynamicSharedMemorySize, (int)E->lds_samp));
-  HIPCHK(hipFuncSetAttribute((const void*)k_measure, hipFuncAttributeMaxDynamicSharedMemorySize, (int)E->lds_meas));
+  raise_lds_limit((const void*)k_block<double>, E->lds_samp);
+  raise_lds_limit((const void*)k_block<float>, E->lds_samp);
+  raise_lds_limit((const void*)k_begin, E->lds_samp);
+  raise_lds_limit((const void*)k_measure, E->lds_meas);
   E->lds_rebuild = lds_bytes_rebuild(v);
-  HIPCHK(hipFuncSetAttribute((const void*)k_rebuild, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)E->lds_rebuild));
-  HIPCHK(hipFuncSetAttribute((const void*)k_meas_rebuild, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)E->lds_rebuild));
+  raise_lds_limit((const void*)k_rebuild, E->lds_rebuild);
+  raise_lds_limit((const void*)k_meas_rebuild, E->lds_rebuild);
   int occ_s = 0, occ_m = 0, occ_r = 0;
   {
     int occ_d = 0, occ_f = 0;
@@ -824,7 +835,7 @@ int mvmc_pfinv_f64(const double* planes, int n, int count, double* pf, double* i
     double* work = nullptr;
     HIPCHK(hipMallocAsync((void**)&work, sizeof(double) * (size_t)blocks * pfinv_work_doubles(n, np), st));
     size_t lds = pf_bytes(n, np) + 64;
-    HIPCHK(hipFuncSetAttribute((const void*)k_pfinv_batched, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    raise_lds_limit((const void*)k_pfinv_batched, lds);
     hipLaunchKernelGGL(k_pfinv_batched, dim3(blocks), dim3(256), lds, st, inv, work, n, count, np, pf, status);
     HIPCHK(hipGetLastError());
     HIPCHK(hipFreeAsync(work, st));

@@ -317,3 +317,48 @@ def test_direct_sr_matches_dense_solve(M, tmp_path):
     # a full step through the NSRCG 0 path
     row = ens.opt_step()
     assert np.isfinite(row).all()
+
+
+def test_uneven_lattices_side_by_side(M, tmp_path):
+    """Batching form (2), lattices of uneven Nsize on one GPU: ensembles of a
+    16-site chain (Nsize 16), a 24-site chain (Nsize 24) and the 8x8 square
+    lattice (Nsize 64, 8 QP planes) live in one process and are stepped
+    concurrently from host threads, one HIP stream each; every zvo row equals
+    the one of the same ensemble run alone (the handles share no state; the
+    kernels' LDS limits only grow)."""
+    import threading
+    import time
+    import lattice
+    log("uneven lattices side by side")
+    cfgs = [("chain16", {}, 32), ("chain16", {"L": 24, "nup": 12}, 32), ("square8", {}, 64)]
+    files = []
+    for k, (name, ov, W) in enumerate(cfgs):
+        info = lattice.write_workload(name, str(tmp_path / f"wl{k}"), sr_steps=2, warmup=2, overrides=ov)
+        files.append((info["namelist"], info["initial"], W, info["nsize"]))
+    assert sorted(f[3] for f in files) == [16, 24, 64]
+    t0 = time.time()
+    alone = []
+    for nl, ini, W, _ in files:
+        ens = M.Ensemble(nl, ini, walkers=W)
+        alone.append([ens.opt_step() for _ in range(2)])
+        ens.close()
+    t_alone = time.time() - t0
+    ens = [M.Ensemble(nl, ini, walkers=W) for nl, ini, W, _ in files]
+    out = [None] * len(ens)
+
+    def run(k):
+        out[k] = [ens[k].opt_step() for _ in range(2)]
+
+    t0 = time.time()
+    th = [threading.Thread(target=run, args=(k,)) for k in range(len(ens))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    t_side = time.time() - t0
+    log(f"  alone {t_alone:.2f} s, side by side {t_side:.2f} s")
+    for k in range(len(ens)):
+        assert out[k] is not None
+        for a, b in zip(alone[k], out[k]):
+            assert np.array_equal(a, b), k
+        ens[k].close()
