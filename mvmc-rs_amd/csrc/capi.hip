@@ -146,16 +146,24 @@ void cg_apply_dev(int na, int rows, const double* pack, const double* mean, cons
 }
 
 // C 1.4.0 StochasticOptCG_Main / PreconCG_DiagScale (sr_cg.rs:700-880).
-// Iterations run on the device; a done flag set by the head / guard kernels
-// turns the rest of a batch into no-ops, and the host reads the flag once per
-// batch of CG_BATCH iterations.
-constexpr int CG_BATCH = 16;
+// Iterations run on the device as a HIP graph of one residual-refresh period
+// (CG_PERIOD iterations, the refresh S x at its last one), captured once per
+// solve and replayed: the kernels read the period's first iteration from
+// ctl[CTL_ITER] (k_cg_advance moves it on), and a done flag set by the head /
+// guard kernels turns the rest into no-ops.  The host reads the flag once per
+// period.  The graph holds kernel nodes only.
+constexpr int CG_PERIOD = 20;
 
 long long cg_solve_dev(int na, int W, int NS, const double* pack, const double* mean, const double* diag,
                        const double* g, double inv_w, double shift, double tol, int max_it, bool diag_scale,
-                       double* x_out, CgWork& wk, hipStream_t st) {
+                       double* x_out, CgWork& wk, hipStream_t st_in) {
   const int rows = W * NS;
   wk.ensure(na, rows);
+  // graph capture needs a created stream: the null stream's work is ordered
+  // with a default-flag stream by the runtime
+  hipStream_t own = nullptr;
+  if (!st_in) HIPCHK(hipStreamCreate(&own));
+  hipStream_t st = st_in ? st_in : own;
   const double threshold = tol * tol * (double)na * (double)na;
   const double pfac = 1.0 + shift;
   const dim3 blk(256);
@@ -168,32 +176,46 @@ long long cg_solve_dev(int na, int W, int NS, const double* pack, const double* 
   else
     HIPCHK(hipMemcpyAsync(wk.d, g, sizeof(double) * na, hipMemcpyDeviceToDevice, st));
   hipLaunchKernelGGL(k_cg_init, dim3(1), blk, 0, st, na, wk.r, wk.d, diag, pfac, diag_scale ? 1 : 0, wk.sc, wk.ctl);
+  HIPCHK(hipGetLastError());
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  for (int j = 0; j < CG_PERIOD; ++j) {
+    hipLaunchKernelGGL(k_cg_head, dim3(1), dim3(64), 0, st, j, max_it, threshold, wk.sc, wk.ctl);
+    cg_apply_dev(na, rows, pack, mean, diag, wk.d, nullptr, wk.q, inv_w, shift, wk, st);
+    hipLaunchKernelGGL(k_cg_alpha, dim3(1), blk, 0, st, j, na, wk.d, wk.q, wk.sc, wk.ctl);
+    const int refresh = (j == CG_PERIOD - 1);  // (iter + 1) % 20 == 0 with periods starting at multiples of 20
+    hipLaunchKernelGGL(k_cg_update, dim3(nblk(na)), blk, 0, st, na, wk.sc, wk.ctl, wk.d, wk.x, wk.q, wk.r, refresh);
+    if (refresh) cg_apply_dev(na, rows, pack, mean, diag, wk.x, wk.g, wk.r, inv_w, shift, wk, st);
+    hipLaunchKernelGGL(k_cg_beta, dim3(1), blk, 0, st, j, na, wk.r, wk.q, diag, pfac, diag_scale ? 1 : 0, wk.sc,
+                       wk.ctl);
+    hipLaunchKernelGGL(k_cg_dir, dim3(nblk(na)), blk, 0, st, na, wk.sc, wk.ctl, diag_scale ? wk.q : wk.r, wk.d);
+  }
+  hipLaunchKernelGGL(k_cg_advance, dim3(1), dim3(64), 0, st, wk.ctl, CG_PERIOD);
+  HIPCHK(hipStreamEndCapture(st, &graph));
+  HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   long long status = 0;
-  for (int iter0 = 0;; iter0 += CG_BATCH) {
-    for (int iter = iter0; iter < iter0 + CG_BATCH && iter <= max_it; ++iter) {
-      hipLaunchKernelGGL(k_cg_head, dim3(1), dim3(64), 0, st, iter, max_it, threshold, wk.sc, wk.ctl);
-      if (iter == max_it) break;
-      cg_apply_dev(na, rows, pack, mean, diag, wk.d, nullptr, wk.q, inv_w, shift, wk, st);
-      hipLaunchKernelGGL(k_cg_alpha, dim3(1), blk, 0, st, iter, na, wk.d, wk.q, wk.sc, wk.ctl);
-      const int refresh = ((iter + 1) % 20 == 0);
-      hipLaunchKernelGGL(k_cg_update, dim3(nblk(na)), blk, 0, st, na, wk.sc, wk.ctl, wk.d, wk.x, wk.q, wk.r,
-                         refresh);
-      if (refresh) cg_apply_dev(na, rows, pack, mean, diag, wk.x, wk.g, wk.r, inv_w, shift, wk, st);
-      hipLaunchKernelGGL(k_cg_beta, dim3(1), blk, 0, st, iter, na, wk.r, wk.q, diag, pfac, diag_scale ? 1 : 0, wk.sc,
-                         wk.ctl);
-      hipLaunchKernelGGL(k_cg_dir, dim3(nblk(na)), blk, 0, st, na, wk.sc, wk.ctl, diag_scale ? wk.q : wk.r, wk.d);
-    }
+  bool done = false;
+  for (int it0 = 0; it0 <= max_it; it0 += CG_PERIOD) {
+    HIPCHK(hipGraphLaunch(exec, st));
     int ctl[4];
     HIPCHK(hipMemcpyAsync(ctl, wk.ctl, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     if (ctl[0]) {
       status = (long long)(((unsigned long long)(unsigned)ctl[2] << 32) | (unsigned)ctl[1]);
+      done = true;
       break;
     }
-    if (iter0 + CG_BATCH > max_it) throw Fail(MVMC_E_BACKEND, "SR-CG control flag not set at the iteration limit");
+  }
+  (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  if (!done) {
+    if (own) (void)hipStreamDestroy(own);
+    throw Fail(MVMC_E_BACKEND, "SR-CG control flag not set at the iteration limit");
   }
   HIPCHK(hipMemcpyAsync(x_out, wk.x, sizeof(double) * na, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipStreamSynchronize(st));
+  if (own) HIPCHK(hipStreamDestroy(own));
   return status;
 }
 

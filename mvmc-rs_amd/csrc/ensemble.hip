@@ -1696,7 +1696,9 @@ __global__ void k_cg_pack(EnsView v, const int* map, int na, double* pack) {
   pack[e] = v.o_store[ws * v.size + map[si] + 1];
 }
 
-enum { CTL_DONE = 0, CTL_STATUS_LO = 1, CTL_STATUS_HI = 2 };
+// ctl: 0 done, 1..2 status (int64), 3 first iteration of the current period
+// of the replayed CG graph (kernels get their offset in the period)
+enum { CTL_DONE = 0, CTL_STATUS_LO = 1, CTL_STATUS_HI = 2, CTL_ITER = 3 };
 
 __device__ inline void cg_finish(int* ctl, long long status) {
   ctl[CTL_STATUS_LO] = (int)(status & 0xffffffffLL);
@@ -1861,8 +1863,9 @@ __global__ void __launch_bounds__(256) k_cg_init(int na, const double* r, double
 }
 
 // Head of iteration `iter` (the loop condition of the C solver).
-__global__ void k_cg_head(int iter, int max_it, double threshold, const double* sc, int* ctl) {
+__global__ void k_cg_head(int j, int max_it, double threshold, const double* sc, int* ctl) {
   if (threadIdx.x || blockIdx.x || ctl[CTL_DONE]) return;
+  const int iter = ctl[CTL_ITER] + j;
   const double delta = sc[0];
   if (iter >= max_it) {
     cg_finish(ctl, fabs(delta) >= threshold ? -(long long)(max_it + 1) : (long long)iter);
@@ -1873,9 +1876,10 @@ __global__ void k_cg_head(int iter, int max_it, double threshold, const double* 
 }
 
 // alpha = delta / (d.q) with the C guards
-__global__ void __launch_bounds__(256) k_cg_alpha(int iter, int na, const double* d, const double* q, double* sc,
+__global__ void __launch_bounds__(256) k_cg_alpha(int j, int na, const double* d, const double* q, double* sc,
                                                   int* ctl) {
   if (ctl[CTL_DONE]) return;
+  const int iter = ctl[CTL_ITER] + j;
   __shared__ double red[256];
   const double dq = block_bdot(d, q, na, red);
   if (threadIdx.x == 0) {
@@ -1898,9 +1902,10 @@ __global__ void k_cg_update(int na, const double* sc, const int* ctl, const doub
 }
 
 // dnew (diag-scale: q = precond(r), dnew = r.q; plain: r.r), beta, delta
-__global__ void __launch_bounds__(256) k_cg_beta(int iter, int na, const double* r, double* q, const double* diag,
+__global__ void __launch_bounds__(256) k_cg_beta(int j, int na, const double* r, double* q, const double* diag,
                                                  double pfac, int diag_scale, double* sc, int* ctl) {
   if (ctl[CTL_DONE]) return;
+  const int iter = ctl[CTL_ITER] + j;
   __shared__ double red[256];
   double p = 0.0;
   for (int i = threadIdx.x; i < na; i += 256) {
@@ -1924,6 +1929,12 @@ __global__ void __launch_bounds__(256) k_cg_beta(int iter, int na, const double*
 }
 
 // d = src + beta d
+// End of a replayed period of the CG graph: the next period starts `period`
+// iterations later.
+__global__ void k_cg_advance(int* ctl, int period) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && !ctl[CTL_DONE]) ctl[CTL_ITER] += period;
+}
+
 __global__ void k_cg_dir(int na, const double* sc, const int* ctl, const double* src, double* d) {
   if (ctl[CTL_DONE]) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -21,6 +21,12 @@ constexpr int LTL_K = 8;
 constexpr int TJ = 8;
 // Most waves per workgroup wg_pfinv is called with (blockDim.x <= 256).
 constexpr int PF_WAVES = 4;
+// Columns per batch of the panel close, and the per-wave LDS broadcast
+// buffer (the close's t1, t2 of a batch; a 64-value block in the inverse).
+#ifndef PF_CLOSE_JB
+#define PF_CLOSE_JB 8
+#endif
+constexpr int PF_WB = 2 * PF_CLOSE_JB * LTL_K > 64 ? 2 * PF_CLOSE_JB * LTL_K : 64;
 
 // Orders the LDS accesses of one wave around a lane-to-lane exchange (LDS
 // operations of a wave execute in issue order; this keeps the compiler from
@@ -40,7 +46,7 @@ __host__ __device__ inline size_t pfinv_work_doubles(int n, int np) {
 // piv[np*n], colbuf[np*n], kp/info[np], colmax[np], ks[np*n] (panel step of
 // the last swap through each index), act[np*LTL_K] (step applied per plane),
 // zmax[2*np] (extent of the zeroed diagonal; whether the last x column
-// awaits scaling), wb[PF_WAVES*128] (one broadcast buffer per wave).
+// awaits scaling), wb[PF_WAVES*PF_WB] (one broadcast buffer per wave).
 struct PfinvShared {
   int* piv;
   double* colbuf;
@@ -63,7 +69,7 @@ __host__ __device__ inline size_t pf_rnd16(size_t b) { return (b + 15) & ~(size_
 __host__ __device__ inline size_t pf_bytes(int n, int np) {
   return 2 * pf_rnd16((size_t)np * n * sizeof(int)) + pf_rnd16((size_t)np * n * sizeof(double)) +
          3 * pf_rnd16((size_t)np * 8) + pf_rnd16((size_t)np * LTL_K * sizeof(int)) + pf_rnd16(2 * (size_t)np * sizeof(int)) +
-         pf_rnd16((size_t)PF_WAVES * 128 * sizeof(double));
+         pf_rnd16((size_t)PF_WAVES * PF_WB * sizeof(double));
 }
 
 template <class Carver>
@@ -77,7 +83,7 @@ __device__ inline void carve_pfinv(Carver& c, PfinvShared& sh, int n, int np) {
   sh.ks = c.template take<int>((size_t)np * n);
   sh.act = c.template take<int>((size_t)np * LTL_K);
   sh.zmax = c.template take<int>(2 * (size_t)np);
-  sh.wb = c.template take<double>((size_t)PF_WAVES * 128);
+  sh.wb = c.template take<double>((size_t)PF_WAVES * PF_WB);
 }
 
 // Argmax of |x| over a column with C IDAMAX semantics: first index of the
@@ -426,8 +432,10 @@ __device__ __forceinline__ void wg_pfinv(double* planes, double* work, int n, in
         // lane owns row i; JB columns at a time: their elements are loaded
         // together, their t1, t2 of every step are one wave load each,
         // broadcast through the wave's LDS buffer.
-        constexpr int JB = 8;  // LTL_K * JB == 64
-        double* wb = sh.wb + wave * 128;
+        constexpr int JB = PF_CLOSE_JB;           // columns per batch
+        constexpr int NH = JB * LTL_K / 64;       // wave loads per step-scalar array
+        static_assert(NH * 64 == JB * LTL_K, "a batch's step scalars fill whole wave loads");
+        double* wb = sh.wb + wave * PF_WB;
         const int chunks = (m - 1 + 63) / 64;
         for (int it = __builtin_amdgcn_readfirstlane(wave); it < np * chunks; it += nwave) {
           const int p = it % np, c = it / np;
@@ -446,9 +454,14 @@ __device__ __forceinline__ void wg_pfinv(double* planes, double* work, int n, in
           double* a = planes + (size_t)p * nn;
           const int tl = lane % LTL_K, jl = lane / LTL_K;
           for (int j0 = c * 64 + 1; j0 < m; j0 += JB) {
-            const bool lv = tl < tp && j0 + jl < m;
-            const double v1 = lv ? PT1(p, tl)[j0 + jl] : 0.0;
-            const double v2 = lv ? PT2(p, tl)[j0 + jl] : 0.0;
+            double v1[NH], v2[NH];
+#pragma unroll
+            for (int h = 0; h < NH; ++h) {
+              const int jc = jl + h * (64 / LTL_K);
+              const bool lv = tl < tp && j0 + jc < m;
+              v1[h] = lv ? PT1(p, tl)[j0 + jc] : 0.0;
+              v2[h] = lv ? PT2(p, tl)[j0 + jc] : 0.0;
+            }
             double cc[JB];
 #pragma unroll
             for (int jj = 0; jj < JB; ++jj) {
@@ -456,8 +469,11 @@ __device__ __forceinline__ void wg_pfinv(double* planes, double* work, int n, in
               cc[jj] = (on && j > i && j < m) ? (a + (size_t)j * n)[i] : 0.0;
             }
             wave_sync();
-            wb[lane] = v1;       // wb[t + LTL_K*jj] = t1 of step t, column j0+jj
-            wb[64 + lane] = v2;  // t2
+#pragma unroll
+            for (int h = 0; h < NH; ++h) {
+              wb[lane + 64 * h] = v1[h];                // wb[t + LTL_K*jj] = t1 of step t, column j0+jj
+              wb[JB * LTL_K + lane + 64 * h] = v2[h];   // t2
+            }
             wave_sync();
             double* ao = opaque_uniform(a);  // store addresses not shared with the loads above
 #pragma unroll
@@ -469,7 +485,8 @@ __device__ __forceinline__ void wg_pfinv(double* planes, double* work, int n, in
                   double x = cc[jj];
 #pragma unroll
                   for (int t = 0; t < LTL_K; ++t)
-                    if (t >= s0 && ar[t]) x = (x + xr[t] * wb[t + LTL_K * jj]) - yr[t] * wb[64 + t + LTL_K * jj];
+                    if (t >= s0 && ar[t])
+                      x = (x + xr[t] * wb[t + LTL_K * jj]) - yr[t] * wb[JB * LTL_K + t + LTL_K * jj];
                   (ao + (size_t)j * n)[i] = x;
                 }
               }
@@ -524,7 +541,7 @@ __device__ __forceinline__ void wg_pfinv(double* planes, double* work, int n, in
   // load, broadcast to the lanes through the wave's LDS buffer.
   {
     constexpr int TR = 4, KB = 4;  // KB * TJ <= 64
-    double* wb = sh.wb + wave * 128;
+    double* wb = sh.wb + wave * PF_WB;
     for (int p = __builtin_amdgcn_readfirstlane(wave); p < np; p += nwave) {
       if (sh.info[p] != 0) continue;
       double* a = planes + (size_t)p * nn;
@@ -677,7 +694,7 @@ __device__ __forceinline__ void wg_pfinv(double* planes, double* work, int n, in
   PF_FRESH();
   {
     constexpr int TR = 4, KB = 4;  // KB * TJ <= 64
-    double* wb = sh.wb + wave * 128;
+    double* wb = sh.wb + wave * PF_WB;
     const bool upper = n > 64;
     for (int p = __builtin_amdgcn_readfirstlane(wave); p < np; p += nwave) {
       if (sh.info[p] != 0) continue;
