@@ -249,6 +249,10 @@ def main():
             "achieved_TFLOPs": fl / (td["cg_ms"] / 1e3) / 1e12,
             "frac_fp64_mfma": fl / (td["cg_ms"] / 1e3) / 1e12 / 78.6}
     out["secondary_rooflines"] = sec
+    out["autotuned"] = {"rebuild_variant": ["3 workgroups/CU, 8-column close batches",
+                                            "2 workgroups/CU, 16-column close batches",
+                                            "not decided"][tl["rebuild_variant"] if tl["rebuild_variant"] >= 0 else 2],
+                        "rebuild_ms_per_walker_measured": list(tl["rebuild_ms_per_walker"])}
     if not args.no_mixed:
         # precision form (1): the same workload, seeds and step count with the
         # sampler's resident inverse in FP32; energies against the FP64 rows

@@ -127,14 +127,17 @@ int mvmc_ens_get_walker(mvmc_ens* e, int walker, int* saved_ele_idx, long long* 
 int mvmc_ens_get_tables(mvmc_ens* e, int walker, double* inv, double* pf);
 int mvmc_ens_get_config(mvmc_ens* e, int walker, int* ele_idx, int* ele_cfg, int* ele_num, int* proj_cnt);
 int mvmc_ens_get_cg(mvmc_ens* e, double* solution, int* n_active);
-/* timings[16]: sampling s, measurement s, SR s (wall, summed over calls),
+/* timings[20]: sampling s, measurement s, SR s (wall, summed over calls),
  * proposals, accepted, device ms of the last sampling call's k_block
  * launches, their number, device ms of its periodic-rebuild launches,
  * walkers those rebuilt; device ms of the last measurement call's rebuilds
  * and of its measurement launches, samples it measured; device ms of the
  * last SR call's CG solve, its iterations, active parameters, sample rows
- * (HIP events on the library's stream). */
-int mvmc_ens_timings(mvmc_ens* e, double* timings16);
+ * (HIP events on the library's stream); the rebuild-kernel variant in use
+ * (-1 while it is being measured; 0: three workgroups per CU, 8-column
+ * panel-close batches; 1: two per CU, 16 columns) and the device ms per
+ * rebuilt walker measured for variants 0 and 1; reserved. */
+int mvmc_ens_timings(mvmc_ens* e, double* timings20);
 /* Device phase clocks of the sampling kernel, summed over walkers (ticks of
  * the 100 MHz constant clock): 0 candidate, 1 projection deltas + ratio,
  * 2 inverse/Slater row staging, 3 Pfaffian ratio dots, 4 Metropolis,

@@ -256,6 +256,13 @@ struct mvmc_ens {
   std::vector<void*> allocs;
   hipStream_t st = nullptr;
   int grid_samp = 0, grid_meas = 0, grid_rebuild = 0, slots = 0;
+  // rebuild-kernel variants (0: 3 workgroups per CU, 8-column close batches;
+  // 1: 2 per CU, 16 columns), tuned on the first well-filled launches:
+  // rb_variant -1 while measuring, then the faster one
+  int grid_rb[2] = {0, 0};
+  int rb_variant = -1;
+  double rb_ms[2] = {0, 0}, rb_walkers[2] = {0, 0};
+  int rb_samples[2] = {0, 0};
   size_t lds_rebuild = 0;
   double rebuild_ms = 0, rebuild_launches = 0;
   size_t lds_samp = 0, lds_meas = 0;
@@ -486,8 +493,10 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
   raise_lds_limit((const void*)k_begin, E->lds_samp);
   raise_lds_limit((const void*)k_measure, E->lds_meas);
   E->lds_rebuild = lds_bytes_rebuild(v);
-  raise_lds_limit((const void*)k_rebuild, E->lds_rebuild);
-  raise_lds_limit((const void*)k_meas_rebuild, E->lds_rebuild);
+  raise_lds_limit((const void*)k_rebuild<3, 8>, E->lds_rebuild);
+  raise_lds_limit((const void*)k_rebuild<2, 16>, E->lds_rebuild);
+  raise_lds_limit((const void*)k_meas_rebuild<3, 8>, E->lds_rebuild);
+  raise_lds_limit((const void*)k_meas_rebuild<2, 16>, E->lds_rebuild);
   int occ_s = 0, occ_m = 0, occ_r = 0;
   {
     int occ_d = 0, occ_f = 0;
@@ -496,14 +505,19 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
     occ_s = std::min(occ_d, occ_f);  // one grid serves both forms
   }
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_m, k_measure, 256, E->lds_meas));
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r, k_rebuild, 256, E->lds_rebuild));
+  int occ_r1 = 0;
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r, k_rebuild<3, 8>, 256, E->lds_rebuild));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r1, k_rebuild<2, 16>, 256, E->lds_rebuild));
+  occ_r1 = std::max(1, occ_r1);
   occ_s = std::max(1, occ_s);
   occ_m = std::max(1, occ_m);
   occ_r = std::max(1, occ_r);
   E->grid_samp = (int)std::min<long long>(walkers, (long long)occ_s * ncu);
   E->grid_meas = (int)std::min<long long>(walkers, (long long)occ_m * ncu);
   E->grid_rebuild = (int)std::min<long long>(walkers, (long long)occ_r * ncu);
-  E->slots = std::max(std::max(E->grid_samp, E->grid_meas), E->grid_rebuild);
+  E->grid_rb[0] = E->grid_rebuild;
+  E->grid_rb[1] = (int)std::min<long long>(walkers, (long long)occ_r1 * ncu);
+  E->slots = std::max(std::max(E->grid_samp, E->grid_meas), std::max(E->grid_rb[0], E->grid_rb[1]));
   v.work = dalloc<double>(A, (size_t)E->slots * pfinv_work_doubles(v.n, v.nqp));
   v.qorb = dalloc<double>(A, (size_t)E->slots * v.nqp * std::max(v.nslater, 1));
   v.pend = dalloc<double>(A, (size_t)E->slots * 2 * v.nqp * v.n);
@@ -529,6 +543,22 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
   hipLaunchKernelGGL(k_seed, dim3(nblk(walkers, 64)), dim3(64), 0, E->st, v, seed, n_active);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(E->st));
+}
+
+void launch_rebuild(mvmc_ens* E, int var) {
+  if (var == 1)
+    hipLaunchKernelGGL((k_rebuild<2, 16>), dim3(E->grid_rb[1]), dim3(256), E->lds_rebuild, E->st, E->v);
+  else
+    hipLaunchKernelGGL((k_rebuild<3, 8>), dim3(E->grid_rb[0]), dim3(256), E->lds_rebuild, E->st, E->v);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_meas_rebuild(mvmc_ens* E, int s) {
+  if (E->rb_variant == 1)
+    hipLaunchKernelGGL((k_meas_rebuild<2, 16>), dim3(E->grid_rb[1]), dim3(256), E->lds_rebuild, E->st, E->v, s);
+  else
+    hipLaunchKernelGGL((k_meas_rebuild<3, 8>), dim3(E->grid_rb[0]), dim3(256), E->lds_rebuild, E->st, E->v, s);
+  HIPCHK(hipGetLastError());
 }
 
 void sample_impl(mvmc_ens* E) {
@@ -576,14 +606,23 @@ void sample_impl(mvmc_ens* E) {
       ++launches;
       if (nf == 0) break;
       rebuilt += nf;
+      // variant: the tuned one, else alternate on launches that fill every
+      // slot twice over (smaller ones run variant 0 and are not counted)
+      const bool tune = E->rb_variant < 0 && nf >= 2 * std::max(E->grid_rb[0], E->grid_rb[1]);
+      const int var = E->rb_variant >= 0 ? E->rb_variant : (tune ? (E->rb_samples[0] <= E->rb_samples[1] ? 0 : 1) : 0);
       HIPCHK(hipEventRecord(r0, E->st));
-      hipLaunchKernelGGL(k_rebuild, dim3(E->grid_rebuild), dim3(256), E->lds_rebuild, E->st, v);
-      HIPCHK(hipGetLastError());
+      launch_rebuild(E, var);
       HIPCHK(hipEventRecord(r1, E->st));
       HIPCHK(hipEventSynchronize(r1));
       HIPCHK(hipEventElapsedTime(&ms, r0, r1));
       rebuild_ms += ms;
       ++rebuilds;
+      if (tune) {
+        E->rb_ms[var] += ms;
+        E->rb_walkers[var] += nf;
+        if (++E->rb_samples[var] >= 2 && E->rb_samples[1 - var] >= 2)
+          E->rb_variant = E->rb_ms[0] / E->rb_walkers[0] <= E->rb_ms[1] / E->rb_walkers[1] ? 0 : 1;
+      }
     }
   }
   for (hipEvent_t ev : {e0, e1, r0, r1}) (void)hipEventDestroy(ev);
@@ -610,7 +649,7 @@ void measure_impl(mvmc_ens* E) {
   double ms_rb = 0, ms_m = 0;
   for (int s = 0; s < v.nsample; ++s) {
     HIPCHK(hipEventRecord(ev[0], E->st));
-    hipLaunchKernelGGL(k_meas_rebuild, dim3(E->grid_rebuild), dim3(256), E->lds_rebuild, E->st, v, s);
+    launch_meas_rebuild(E, s);
     HIPCHK(hipEventRecord(ev[1], E->st));
     hipLaunchKernelGGL(k_measure, dim3(E->grid_meas), dim3(256), E->lds_meas, E->st, v, E->d_gscr, E->gstride, 1,
                        s);
@@ -1011,7 +1050,7 @@ int mvmc_ens_log_psi(mvmc_ens* e, double* out) {
     if (!e->d_rows) e->d_rows = dalloc<double>(e->allocs, rows);
     e->upload_tables();
     for (int s = 0; s < v.nsample; ++s) {
-      hipLaunchKernelGGL(k_meas_rebuild, dim3(e->grid_rebuild), dim3(256), e->lds_rebuild, e->st, v, s);
+      launch_meas_rebuild(e, s);
       hipLaunchKernelGGL(k_log_psi, dim3(nblk(v.W)), dim3(256), 0, e->st, v, s, e->d_rows);
     }
     HIPCHK(hipGetLastError());
@@ -1060,6 +1099,10 @@ int mvmc_ens_timings(mvmc_ens* e, double* t) {
   t[5] = e->block_ms; t[6] = e->block_launches; t[7] = e->rebuild_ms;
   t[8] = e->rebuilt; t[9] = e->meas_rebuild_ms; t[10] = e->measure_ms; t[11] = e->measured;
   t[12] = e->cg_ms; t[13] = e->cg_iter; t[14] = e->cg_na; t[15] = e->cg_rows;
+  t[16] = e->rb_variant;
+  t[17] = e->rb_walkers[0] > 0 ? e->rb_ms[0] / e->rb_walkers[0] : 0.0;
+  t[18] = e->rb_walkers[1] > 0 ? e->rb_ms[1] / e->rb_walkers[1] : 0.0;
+  t[19] = 0.0;
   return MVMC_OK;
 }
 

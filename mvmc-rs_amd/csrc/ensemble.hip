@@ -109,6 +109,7 @@ __device__ inline int block_exscan(int v, int* scan, int* total) {
 // native = true skips the all-zero guard (C's separate real setup).
 // Returns 0 on success, 1 if any plane failed.
 // clk: optional phase clocks of the elimination (profiling builds).
+template <int JB = PF_JB>
 __device__ __forceinline__ int wg_calc_m_all(const EnsView& v, const int* ele_idx_l, int* rs, double* planes, double* work,
                              double* pf_l, char* pfscratch, bool native, long long* clk = nullptr) {
   const int tid = threadIdx.x;
@@ -156,7 +157,7 @@ __device__ __forceinline__ int wg_calc_m_all(const EnsView& v, const int* ele_id
     }
     if (allzero_flag) return 1;
   }
-  wg_pfinv(planes, work, v.n, v.nqp, sh, pf_l, -1.0, clk);
+  wg_pfinv<JB>(planes, work, v.n, v.nqp, sh, pf_l, -1.0, clk);
   int bad = 0;
   for (int p = 0; p < v.nqp; ++p) bad |= (sh.info[p] != 0);
   __syncthreads();
@@ -1210,7 +1211,11 @@ __host__ __device__ inline size_t lds_bytes_rebuild(const EnsView& v) {
   return 2 * rnd16((size_t)v.n * 4) + rnd16((size_t)v.nqp * 8) + rnd16(pf_bytes(v.n, v.nqp));
 }
 
-__global__ void __launch_bounds__(256, 3) k_rebuild(EnsView v) {
+// Two compiled variants, bit-identical in results (only the schedule of the
+// elimination differs), picked at run time by measurement: MINB workgroups
+// per CU, JB columns per batch of the panel close.
+template <int MINB, int JB>
+__global__ void __launch_bounds__(256, MINB) k_rebuild(EnsView v) {
   extern __shared__ __align__(16) char lds[];
   Carver c{lds};
   int* ei = c.take<int>(v.n);
@@ -1234,7 +1239,7 @@ __global__ void __launch_bounds__(256, 3) k_rebuild(EnsView v) {
     for (int i = tid; i < v.n; i += blockDim.x) ei[i] = v.ele_idx[(size_t)w * v.n + i];
     __syncthreads();
     double* planes = v.inv + (size_t)w * v.nqp * v.n * v.n;
-    const int bad = wg_calc_m_all(v, ei, rs, planes, work, pfn, pfscr, false,
+    const int bad = wg_calc_m_all<JB>(v, ei, rs, planes, work, pfn, pfscr, false,
                                   v.clk ? v.clk + (size_t)blockIdx.x * 32 + 16 : nullptr);
     copy_inv32(v, w);
     if (tid == 0) {
@@ -1288,7 +1293,8 @@ __device__ double direct_proj_ratio_moved(const EnsView& v, int source, int dest
 // Rebuild for measurement: the inverse and Pfaffians of saved sample s of
 // every walker into the walker's planes and pf; s_valid[w][s] = the sample
 // is not the all-zero / all-negative placeholder and calc_m_all succeeded.
-__global__ void __launch_bounds__(256, 3) k_meas_rebuild(EnsView v, int s) {
+template <int MINB, int JB>
+__global__ void __launch_bounds__(256, MINB) k_meas_rebuild(EnsView v, int s) {
   extern __shared__ __align__(16) char lds[];
   Carver c{lds};
   int* ei = c.take<int>(v.n);
@@ -1311,7 +1317,7 @@ __global__ void __launch_bounds__(256, 3) k_meas_rebuild(EnsView v, int s) {
     int valid = flag;
     if (valid) {
       double* planes = v.inv + (size_t)w * v.nqp * v.n * v.n;
-      valid = !wg_calc_m_all(v, ei, rs, planes, work, pfn, pfscr, false);
+      valid = !wg_calc_m_all<JB>(v, ei, rs, planes, work, pfn, pfscr, false);
     }
     if (tid == 0) {
       v.s_valid[ws] = valid;

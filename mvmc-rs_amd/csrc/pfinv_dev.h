@@ -21,12 +21,12 @@ constexpr int LTL_K = 8;
 constexpr int TJ = 8;
 // Most waves per workgroup wg_pfinv is called with (blockDim.x <= 256).
 constexpr int PF_WAVES = 4;
-// Columns per batch of the panel close, and the per-wave LDS broadcast
-// buffer (the close's t1, t2 of a batch; a 64-value block in the inverse).
-#ifndef PF_CLOSE_JB
-#define PF_CLOSE_JB 8
-#endif
-constexpr int PF_WB = 2 * PF_CLOSE_JB * LTL_K > 64 ? 2 * PF_CLOSE_JB * LTL_K : 64;
+// Columns per batch of the panel close: PF_JB by default, at most PF_JB_MAX
+// (a template parameter of wg_pfinv; the rebuild kernels pick it by
+// measurement).  The per-wave LDS broadcast buffer holds a batch's t1, t2
+// (and a 64-value block in the inverse).
+constexpr int PF_JB = 8, PF_JB_MAX = 16;
+constexpr int PF_WB = 2 * PF_JB_MAX * LTL_K;
 
 // Orders the LDS accesses of one wave around a lane-to-lane exchange (LDS
 // operations of a wave execute in issue order; this keeps the compiler from
@@ -136,6 +136,7 @@ __device__ inline T* opaque_uniform(T* p) {
   return (T*)(((unsigned long long)hi << 32) | lo);
 }
 
+template <int JB = PF_JB>
 __device__ __forceinline__ void wg_pfinv(double* planes, double* work, int n, int np, PfinvShared& sh_in, double* pf, double sign,
                          long long* clk = nullptr) {
   int tid = threadIdx.x;
@@ -432,7 +433,7 @@ __device__ __forceinline__ void wg_pfinv(double* planes, double* work, int n, in
         // lane owns row i; JB columns at a time: their elements are loaded
         // together, their t1, t2 of every step are one wave load each,
         // broadcast through the wave's LDS buffer.
-        constexpr int JB = PF_CLOSE_JB;           // columns per batch
+        static_assert(JB <= PF_JB_MAX, "the broadcast buffer holds PF_JB_MAX columns");
         constexpr int NH = JB * LTL_K / 64;       // wave loads per step-scalar array
         static_assert(NH * 64 == JB * LTL_K, "a batch's step scalars fill whole wave loads");
         double* wb = sh.wb + wave * PF_WB;
