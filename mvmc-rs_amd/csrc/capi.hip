@@ -10,6 +10,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -177,27 +178,42 @@ long long cg_solve_dev(int na, int W, int NS, const double* pack, const double* 
     HIPCHK(hipMemcpyAsync(wk.d, g, sizeof(double) * na, hipMemcpyDeviceToDevice, st));
   hipLaunchKernelGGL(k_cg_init, dim3(1), blk, 0, st, na, wk.r, wk.d, diag, pfac, diag_scale ? 1 : 0, wk.sc, wk.ctl);
   HIPCHK(hipGetLastError());
+  // one refresh period of iterations (kernel launches only)
+  auto issue_period = [&]() {
+    for (int j = 0; j < CG_PERIOD; ++j) {
+      hipLaunchKernelGGL(k_cg_head, dim3(1), dim3(64), 0, st, j, max_it, threshold, wk.sc, wk.ctl);
+      cg_apply_dev(na, rows, pack, mean, diag, wk.d, nullptr, wk.q, inv_w, shift, wk, st);
+      hipLaunchKernelGGL(k_cg_alpha, dim3(1), blk, 0, st, j, na, wk.d, wk.q, wk.sc, wk.ctl);
+      const int refresh = (j == CG_PERIOD - 1);  // (iter + 1) % 20 == 0 with periods starting at multiples of 20
+      hipLaunchKernelGGL(k_cg_update, dim3(nblk(na)), blk, 0, st, na, wk.sc, wk.ctl, wk.d, wk.x, wk.q, wk.r,
+                         refresh);
+      if (refresh) cg_apply_dev(na, rows, pack, mean, diag, wk.x, wk.g, wk.r, inv_w, shift, wk, st);
+      hipLaunchKernelGGL(k_cg_beta, dim3(1), blk, 0, st, j, na, wk.r, wk.q, diag, pfac, diag_scale ? 1 : 0, wk.sc,
+                         wk.ctl);
+      hipLaunchKernelGGL(k_cg_dir, dim3(nblk(na)), blk, 0, st, na, wk.sc, wk.ctl, diag_scale ? wk.q : wk.r, wk.d);
+    }
+    hipLaunchKernelGGL(k_cg_advance, dim3(1), dim3(64), 0, st, wk.ctl, CG_PERIOD);
+  };
+  // MVMC_CG_GRAPH=0 issues the same launches without a graph (for tools that
+  // do not follow graph launches)
+  static const bool use_graph = [] {
+    const char* e = std::getenv("MVMC_CG_GRAPH");
+    return !(e && e[0] == '0');
+  }();
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  for (int j = 0; j < CG_PERIOD; ++j) {
-    hipLaunchKernelGGL(k_cg_head, dim3(1), dim3(64), 0, st, j, max_it, threshold, wk.sc, wk.ctl);
-    cg_apply_dev(na, rows, pack, mean, diag, wk.d, nullptr, wk.q, inv_w, shift, wk, st);
-    hipLaunchKernelGGL(k_cg_alpha, dim3(1), blk, 0, st, j, na, wk.d, wk.q, wk.sc, wk.ctl);
-    const int refresh = (j == CG_PERIOD - 1);  // (iter + 1) % 20 == 0 with periods starting at multiples of 20
-    hipLaunchKernelGGL(k_cg_update, dim3(nblk(na)), blk, 0, st, na, wk.sc, wk.ctl, wk.d, wk.x, wk.q, wk.r, refresh);
-    if (refresh) cg_apply_dev(na, rows, pack, mean, diag, wk.x, wk.g, wk.r, inv_w, shift, wk, st);
-    hipLaunchKernelGGL(k_cg_beta, dim3(1), blk, 0, st, j, na, wk.r, wk.q, diag, pfac, diag_scale ? 1 : 0, wk.sc,
-                       wk.ctl);
-    hipLaunchKernelGGL(k_cg_dir, dim3(nblk(na)), blk, 0, st, na, wk.sc, wk.ctl, diag_scale ? wk.q : wk.r, wk.d);
+  if (use_graph) {
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    issue_period();
+    HIPCHK(hipStreamEndCapture(st, &graph));
+    HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   }
-  hipLaunchKernelGGL(k_cg_advance, dim3(1), dim3(64), 0, st, wk.ctl, CG_PERIOD);
-  HIPCHK(hipStreamEndCapture(st, &graph));
-  HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   long long status = 0;
   bool done = false;
   for (int it0 = 0; it0 <= max_it; it0 += CG_PERIOD) {
-    HIPCHK(hipGraphLaunch(exec, st));
+    if (use_graph) HIPCHK(hipGraphLaunch(exec, st));
+    else issue_period();
+    HIPCHK(hipGetLastError());
     int ctl[4];
     HIPCHK(hipMemcpyAsync(ctl, wk.ctl, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -207,8 +223,8 @@ long long cg_solve_dev(int na, int W, int NS, const double* pack, const double* 
       break;
     }
   }
-  (void)hipGraphExecDestroy(exec);
-  (void)hipGraphDestroy(graph);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
   if (!done) {
     if (own) (void)hipStreamDestroy(own);
     throw Fail(MVMC_E_BACKEND, "SR-CG control flag not set at the iteration limit");

@@ -28,6 +28,12 @@ def log(msg):
 
 @pytest.fixture(scope="module")
 def M():
+    # torch and libmvmc_hip.so each link a HIP runtime (same sonames); in a
+    # process that uses both, torch brings its runtime up first and the
+    # library binds to it, whichever test runs first
+    import torch
+    torch.zeros(1, device="cuda")
+    torch.cuda.synchronize()
     import mvmc_hip
     return mvmc_hip
 
