@@ -368,3 +368,32 @@ def test_uneven_lattices_side_by_side(M, tmp_path):
         for a, b in zip(alone[k], out[k]):
             assert np.array_equal(a, b), k
         ens[k].close()
+
+
+@pytest.mark.parametrize("name,W,warmup", [("square8", 2, 4), ("square16", 1, 2)])
+def test_benchmark_lattices_match_oracle(M, tmp_path, name, W, warmup):
+    """The benchmark's own lattices (8x8: Nsize 64; 16x16 t-t'-U at 1/8
+    doping: Nsize 224; 8 QP planes each, sublattice orbitals, Gutzwiller and
+    Jastrow), one SR-CG step against the oracle with the device's correctly
+    rounded exp/log: the zvo row, every parameter after the update and every
+    walker's saved configuration and SFMT state bit-exact."""
+    import lattice
+    log(f"{name} W={W} vs oracle")
+    info = lattice.write_workload(name, str(tmp_path / name), sr_steps=1, warmup=warmup)
+    O.set_math(1)
+    try:
+        run = O.Run(info["namelist"], info["initial"], walkers=W)
+        ens = M.Ensemble(info["namelist"], info["initial"], walkers=W)
+        run.step(do_sr=True)
+        row = ens.opt_step()
+        ro = run.energies()[-1]
+        assert np.array_equal(row, ro), (row, ro)
+        for w in range(W):
+            sg, so = ens.walker(w), run.walker(w)
+            assert np.array_equal(sg["ele_idx"], so["ele_idx"]), w
+            assert np.array_equal(sg["rng"], so["rng"]) and sg["rng_idx"] == so["rng_idx"], w
+        pg, sg_ = ens.params()
+        po, so_ = run.params()
+        assert np.array_equal(pg, po) and np.array_equal(sg_, so_)
+    finally:
+        O.set_math(0)
