@@ -123,7 +123,8 @@ inline int nblk(long long n, int b = 256) { return (int)((n + b - 1) / b); }
 
 // out = S x (or g - S x when g is given) with the sampled operator
 // (sr_cg.rs:952-1004) in the blocked reduction order.
-void cg_apply_dev(int na, int rows, const double* pack, const double* mean, const double* diag, const double* xv,
+template <class P>
+void cg_apply_dev(int na, int rows, const P* pack, const double* mean, const double* diag, const double* xv,
                   const double* g, double* out, double inv_w, double shift, CgWork& wk, hipStream_t st) {
   const int nch = (rows + CG_RCH - 1) / CG_RCH;
   const int nsup = (nch + CG_SCH - 1) / CG_SCH;
@@ -131,15 +132,16 @@ void cg_apply_dev(int na, int rows, const double* pack, const double* mean, cons
   const dim3 grid(nch + 1), blk(256);
   double* coef = wk.sc + 5;
   if (cols <= 4)
-    hipLaunchKernelGGL(k_cg_chunk<4>, grid, blk, 0, st, pack, rows, na, xv, mean, wk.cpart, coef, wk.ctl);
+    hipLaunchKernelGGL((k_cg_chunk<4, P>), grid, blk, 0, st, pack, rows, na, xv, mean, wk.cpart, coef, wk.ctl);
   else if (cols <= 8)
-    hipLaunchKernelGGL(k_cg_chunk<8>, grid, blk, 0, st, pack, rows, na, xv, mean, wk.cpart, coef, wk.ctl);
+    hipLaunchKernelGGL((k_cg_chunk<8, P>), grid, blk, 0, st, pack, rows, na, xv, mean, wk.cpart, coef, wk.ctl);
   else if (cols <= 16)
-    hipLaunchKernelGGL(k_cg_chunk<16>, grid, blk, 0, st, pack, rows, na, xv, mean, wk.cpart, coef, wk.ctl);
+    hipLaunchKernelGGL((k_cg_chunk<16, P>), grid, blk, 0, st, pack, rows, na, xv, mean, wk.cpart, coef, wk.ctl);
   else if (cols <= CG_CMAX)
-    hipLaunchKernelGGL(k_cg_chunk<CG_CMAX>, grid, blk, 0, st, pack, rows, na, xv, mean, wk.cpart, coef, wk.ctl);
+    hipLaunchKernelGGL((k_cg_chunk<CG_CMAX, P>), grid, blk, 0, st, pack, rows, na, xv, mean, wk.cpart, coef,
+                       wk.ctl);
   else
-    hipLaunchKernelGGL(k_cg_chunk_wide, grid, blk, 0, st, pack, rows, na, xv, mean, wk.cpart, coef, wk.ctl);
+    hipLaunchKernelGGL(k_cg_chunk_wide<P>, grid, blk, 0, st, pack, rows, na, xv, mean, wk.cpart, coef, wk.ctl);
   if (nsup > 0)
     hipLaunchKernelGGL(k_cg_fold1, dim3(nblk(na), nsup), blk, 0, st, wk.cpart, nch, na, wk.spart, wk.ctl);
   hipLaunchKernelGGL(k_cg_fold2, dim3(nblk(na)), blk, 0, st, wk.spart, nsup, na, xv, mean, diag, coef, inv_w, shift,
@@ -155,7 +157,8 @@ void cg_apply_dev(int na, int rows, const double* pack, const double* mean, cons
 // period.  The graph holds kernel nodes only.
 constexpr int CG_PERIOD = 20;
 
-long long cg_solve_dev(int na, int W, int NS, const double* pack, const double* mean, const double* diag,
+template <class P>
+long long cg_solve_dev(int na, int W, int NS, const P* pack, const double* mean, const double* diag,
                        const double* g, double inv_w, double shift, double tol, int max_it, bool diag_scale,
                        double* x_out, CgWork& wk, hipStream_t st_in) {
   const int rows = W * NS;
@@ -290,8 +293,8 @@ struct mvmc_ens {
   double *d_ho = nullptr, *d_mean = nullptr, *d_diag = nullptr;
   double* d_gscr = nullptr;
   int gstride = 1;
-  double* d_pack = nullptr;
-  size_t pack_cap = 0;
+  void* d_pack = nullptr;  // packed active samples: double, or float for the precision form's CG
+  size_t pack_cap = 0;      // bytes
   int* d_map = nullptr;
   double *d_cmean = nullptr, *d_cdiag = nullptr, *d_cg = nullptr, *d_cx = nullptr;
   CgWork cg;
@@ -749,7 +752,7 @@ int direct_sr(mvmc_ens* E, const std::vector<int>& map, int na, size_t rows, dou
   // the full Gram by DGEMM (P P^T): twice the flops of DSYRK, but DGEMM is
   // the matrix-core kernel the library tunes (measured ~9x faster here)
   BLASCHK(rocblas_dgemm(E->blas, rocblas_operation_none, rocblas_operation_transpose, na, na, (rocblas_int)rows,
-                        &inv_w, E->d_pack, na, E->d_pack, na, &zero, E->d_S, na));
+                        &inv_w, (const double*)E->d_pack, na, (const double*)E->d_pack, na, &zero, E->d_S, na));
   hipLaunchKernelGGL(k_sr_assemble, dim3(nblk((long long)nn)), dim3(256), 0, st, na, E->d_cmean,
                      1.0 + m.mp.dsr_opt_sta_del, E->d_S);
   HIPCHK(hipGetLastError());
@@ -832,24 +835,38 @@ int sr_impl(mvmc_ens* E, long long* info_out, bool force_direct = false) {
   HIPCHK(hipMemcpyAsync(E->d_cdiag, diag.data(), sizeof(double) * na, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(E->d_cg, g.data(), sizeof(double) * na, hipMemcpyHostToDevice, st));
   const size_t rows = (size_t)v.W * v.nsample;
-  if (E->pack_cap < rows * na) {
+  // the precision form streams the CG operand in FP32 (accumulated in FP64);
+  // direct SR and the FP64 path pack doubles
+  const bool direct = m.mp.nsrcg == 0 || force_direct;
+  const bool f32 = E->precision == 1 && !direct;
+  const size_t pbytes = rows * na * (f32 ? sizeof(float) : sizeof(double));
+  if (E->pack_cap < pbytes) {
     if (E->d_pack) (void)hipFree(E->d_pack);
     E->d_pack = nullptr;
-    HIPCHK(hipMalloc(&E->d_pack, sizeof(double) * rows * na));
-    E->pack_cap = rows * na;
+    HIPCHK(hipMalloc(&E->d_pack, pbytes));
+    E->pack_cap = pbytes;
   }
-  hipLaunchKernelGGL(k_cg_pack, dim3(nblk((long long)(rows * na))), dim3(256), 0, st, v, E->d_map, na, E->d_pack);
+  if (f32)
+    hipLaunchKernelGGL(k_cg_pack<float>, dim3(nblk((long long)(rows * na))), dim3(256), 0, st, v, E->d_map, na,
+                       (float*)E->d_pack);
+  else
+    hipLaunchKernelGGL(k_cg_pack<double>, dim3(nblk((long long)(rows * na))), dim3(256), 0, st, v, E->d_map, na,
+                       (double*)E->d_pack);
   HIPCHK(hipGetLastError());
   const double inv_w = 1.0 / E->wc;
-  if (m.mp.nsrcg == 0 || force_direct) return direct_sr(E, map, na, rows, inv_w, t0, info_out);
+  if (direct) return direct_sr(E, map, na, rows, inv_w, t0, info_out);
   const int max_it = m.mp.nsr_opt_cg_max_iter > 0 ? m.mp.nsr_opt_cg_max_iter : na;
   hipEvent_t c0, c1;
   HIPCHK(hipEventCreate(&c0));
   HIPCHK(hipEventCreate(&c1));
   HIPCHK(hipEventRecord(c0, st));
-  long long status = cg_solve_dev(na, v.W, v.nsample, E->d_pack, E->d_cmean, E->d_cdiag, E->d_cg, inv_w,
-                                  m.mp.dsr_opt_sta_del, m.mp.dsr_opt_cg_tol, max_it, m.mp.use_diag_scale != 0,
-                                  E->d_cx, E->cg, st);
+  auto solve = [&](bool dscale) {
+    return f32 ? cg_solve_dev(na, v.W, v.nsample, (const float*)E->d_pack, E->d_cmean, E->d_cdiag, E->d_cg, inv_w,
+                              m.mp.dsr_opt_sta_del, m.mp.dsr_opt_cg_tol, max_it, dscale, E->d_cx, E->cg, st)
+               : cg_solve_dev(na, v.W, v.nsample, (const double*)E->d_pack, E->d_cmean, E->d_cdiag, E->d_cg, inv_w,
+                              m.mp.dsr_opt_sta_del, m.mp.dsr_opt_cg_tol, max_it, dscale, E->d_cx, E->cg, st);
+  };
+  long long status = solve(m.mp.use_diag_scale != 0);
   HIPCHK(hipEventRecord(c1, st));
   HIPCHK(hipEventSynchronize(c1));
   {
@@ -865,9 +882,7 @@ int sr_impl(mvmc_ens* E, long long* info_out, bool force_direct = false) {
   long long info = status;
   if (status < 0) {
     if (m.mp.nsrcg_fallback) {
-      long long st2 = cg_solve_dev(na, v.W, v.nsample, E->d_pack, E->d_cmean, E->d_cdiag, E->d_cg, inv_w,
-                                   m.mp.dsr_opt_sta_del, m.mp.dsr_opt_cg_tol, max_it, m.mp.use_diag_scale == 0,
-                                   E->d_cx, E->cg, st);
+      long long st2 = solve(m.mp.use_diag_scale == 0);
       info = -((-status - 1) + 1);
       if (st2 < 0 && m.mp.nsrcg_abort_on_fail) throw Fail(MVMC_E_NUMERIC, "SR-CG fallback failed");
     } else {

@@ -1693,13 +1693,14 @@ __global__ void k_reduce_ops(EnsView v, double* ho, double* mean, double* diag) 
 //         c_i = sum_r O_ri y_r (ascending r); chunks folded left within
 //         superchunks of CG_SCH chunks, superchunks folded left.
 // ---------------------------------------------------------------------------
-__global__ void k_cg_pack(EnsView v, const int* map, int na, double* pack) {
+template <class P>
+__global__ void k_cg_pack(EnsView v, const int* map, int na, P* pack) {
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t total = (size_t)v.W * v.nsample * na;
   if (e >= total) return;
   const size_t ws = e / na;
   const int si = (int)(e % na);
-  pack[e] = v.o_store[ws * v.size + map[si] + 1];
+  pack[e] = static_cast<P>(v.o_store[ws * v.size + map[si] + 1]);
 }
 
 // ctl: 0 done, 1..2 status (int64), 3 first iteration of the current period
@@ -1731,17 +1732,19 @@ __device__ double block_tree256(double p, double* red) {
   return r;
 }
 
-__device__ inline double block_bdot(const double* a, const double* b, int n, double* red) {
+template <class TA>
+__device__ inline double block_bdot(const TA* a, const double* b, int n, double* red) {
   double p = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) p += a[i] * b[i];
+  for (int i = threadIdx.x; i < n; i += 256) p += (double)a[i] * b[i];
   return block_tree256(p, red);
 }
 
 // Chunk partials cpart[b][i] = sum_{r in chunk b} O_ri y_r with y_r formed
 // from the same registers; block nch computes coef = bdot(mean, x).
-// C = columns per lane (na <= 256*C).
-template <int C>
-__global__ void __launch_bounds__(256) k_cg_chunk(const double* pack, int rows, int na, const double* x,
+// C = columns per lane (na <= 256*C).  P: element type of the packed samples
+// (double; float in the precision form, accumulated in FP64).
+template <int C, class P>
+__global__ void __launch_bounds__(256) k_cg_chunk(const P* pack, int rows, int na, const double* x,
                                                   const double* mean, double* cpart, double* coef, const int* ctl) {
   if (ctl[CTL_DONE]) return;
   __shared__ double red[256];
@@ -1762,13 +1765,15 @@ __global__ void __launch_bounds__(256) k_cg_chunk(const double* pack, int rows, 
   }
   const int r1 = min(rows, (b + 1) * CG_RCH);
   for (int r = b * CG_RCH; r < r1; ++r) {
-    const double* row = pack + (size_t)r * na;
+    const P* row = pack + (size_t)r * na;
+    // unconditional loads (index clamped in bounds; lanes past na are masked
+    // in the sums below) so all C loads are in flight before the first use
+    P raw[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) raw[k] = row[min(tid + 256 * k, na - 1)];
     double o[C];
 #pragma unroll
-    for (int k = 0; k < C; ++k) {
-      const int i = tid + 256 * k;
-      o[k] = i < na ? row[i] : 0.0;
-    }
+    for (int k = 0; k < C; ++k) o[k] = (double)raw[k];
     double p = 0.0;
 #pragma unroll
     for (int k = 0; k < C; ++k)
@@ -1786,7 +1791,8 @@ __global__ void __launch_bounds__(256) k_cg_chunk(const double* pack, int rows, 
 }
 
 // Same order for na > 256*CG_CMAX: the chunk accumulators live in cpart.
-__global__ void __launch_bounds__(256) k_cg_chunk_wide(const double* pack, int rows, int na, const double* x,
+template <class P>
+__global__ void __launch_bounds__(256) k_cg_chunk_wide(const P* pack, int rows, int na, const double* x,
                                                        const double* mean, double* cpart, double* coef,
                                                        const int* ctl) {
   if (ctl[CTL_DONE]) return;
@@ -1802,11 +1808,11 @@ __global__ void __launch_bounds__(256) k_cg_chunk_wide(const double* pack, int r
   double* cp = cpart + (size_t)b * na;
   const int r0 = b * CG_RCH, r1 = min(rows, (b + 1) * CG_RCH);
   for (int r = r0; r < r1; ++r) {
-    const double* row = pack + (size_t)r * na;
+    const P* row = pack + (size_t)r * na;
     const double y = block_bdot(row, x, na, red);
     for (int i = tid; i < na; i += 256) {
       double a = (r == r0) ? 0.0 : cp[i];
-      a += row[i] * y;
+      a += (double)row[i] * y;
       cp[i] = a;
     }
   }

@@ -215,31 +215,52 @@ def test_sr_cg_solver_c_abi(M, name, variant):
 
 @pytest.mark.parametrize("namelist,W", [(HUB, 8), (HEIS, 4)])
 def test_mixed_precision_form(M, namelist, W):
-    """Precision form (1): the sampler's resident inverse stored in FP32.
-    Against the FP64 path on the same seeds: the Metropolis decisions do not
-    flip on these chains, so every configuration is identical, and the
-    measurement (a from-scratch FP64 rebuild per sample) and SR give
-    bit-identical zvo rows; the FP32 inverse after a sampling call holds
-    FP32 values within 1e-5 (relative to the plane's largest element) of
-    the FP64 inverse (the drift of at most Nsite rounded rank-2 updates)."""
+    """Precision form (1): the sampler's resident inverse and the SR-CG
+    operand stored in FP32.  Against the FP64 path on the same seeds: the
+    first step's Metropolis decisions do not flip on these chains, so every
+    configuration and (the measurement being FP64) the first zvo row are
+    identical; the SR update, formed from FP32 operands, stays within 1e-3 of
+    the FP64 update (relative to its largest component; the FP32 rounding of
+    O amplified by the conditioning of S); two more steps stay within 1e-4
+    relative in energy; the FP32 inverse after a sampling call holds FP32
+    values within 1e-3 (relative to the plane's largest element) of the FP64
+    inverse: the drift of at most Nsite FP32 rank-2 updates, each dividing by
+    its hop's ratio (measured up to 3e-4 here; a misapplied update is O(1))."""
     log(f"mixed precision {os.path.basename(os.path.dirname(namelist))} W={W}")
     e64 = M.Ensemble(namelist, walkers=W)
     e32 = M.Ensemble(namelist, walkers=W)
     e32.set_precision("mixed")
-    for step in range(3):
+    p0 = np.concatenate(e64.params())
+    r64, r32 = e64.opt_step(), e32.opt_step()
+    for w in range(W):
+        assert np.array_equal(e64.walker(w)["ele_idx"], e32.walker(w)["ele_idx"]), w
+    assert np.array_equal(r64, r32)
+    d64 = np.concatenate(e64.params()) - p0
+    d32 = np.concatenate(e32.params()) - p0
+    rel = np.abs(d32 - d64).max() / np.abs(d64).max()
+    log(f"  SR update difference {rel:.2e} (relative to the largest FP64 component)")
+    assert rel <= 1e-3
+    for step in range(2):
         r64, r32 = e64.opt_step(), e32.opt_step()
-        for w in range(W):
-            assert np.array_equal(e64.walker(w)["ele_idx"], e32.walker(w)["ele_idx"]), (step, w)
-        assert np.array_equal(r64, r32), step
+        assert abs(r32[0] - r64[0]) <= 1e-4 * abs(r64[0]), (step, r32[0], r64[0])
     e64.sample()
     e32.sample()
     for w in range(W):
         t64, t32 = e64.tables(w)[0], e32.tables(w)[0]
         assert np.array_equal(t32, t32.astype(np.float32).astype(np.float64))
-        scale = np.abs(t64).max()
-        assert np.abs(t32 - t64).max() <= 1e-5 * scale, (w, np.abs(t32 - t64).max() / scale)
     with pytest.raises(M.InvalidShape):
         e32.set_precision(7)
+    # the inverse drift, on identical parameters and configurations
+    f64 = M.Ensemble(namelist, walkers=W)
+    f32 = M.Ensemble(namelist, walkers=W)
+    f32.set_precision("mixed")
+    f64.sample()
+    f32.sample()
+    for w in range(W):
+        assert np.array_equal(f64.walker(w)["ele_idx"], f32.walker(w)["ele_idx"]), w
+        t64, t32 = f64.tables(w)[0], f32.tables(w)[0]
+        scale = np.abs(t64).max()
+        assert np.abs(t32 - t64).max() <= 1e-3 * scale, (w, np.abs(t32 - t64).max() / scale)
 
 
 def test_derivative_form_vjp_and_finite_differences(M):
