@@ -1739,12 +1739,37 @@ __device__ inline double block_bdot(const TA* a, const double* b, int n, double*
   return block_tree256(p, red);
 }
 
+// Row of n elements read through a raw buffer resource (stride 0, range
+// n * sizeof(P) bytes, gfx9 dword3 0x00020000): an element offset past the
+// range returns 0 without a memory access.
+template <class P>
+struct RowBuf;
+template <>
+struct RowBuf<double> {
+  __amdgpu_buffer_rsrc_t rs;
+  __device__ RowBuf(const double* p, int n)
+      : rs(__builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, n * (int)sizeof(double), 0x00020000)) {}
+  __device__ double load(int i) const {
+    const auto u = __builtin_amdgcn_raw_buffer_load_b64(rs, i * (int)sizeof(double), 0, 0);
+    return __builtin_bit_cast(double, u);
+  }
+};
+template <>
+struct RowBuf<float> {
+  __amdgpu_buffer_rsrc_t rs;
+  __device__ RowBuf(const float* p, int n)
+      : rs(__builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, n * (int)sizeof(float), 0x00020000)) {}
+  __device__ float load(int i) const {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, i * (int)sizeof(float), 0, 0));
+  }
+};
+
 // Chunk partials cpart[b][i] = sum_{r in chunk b} O_ri y_r with y_r formed
 // from the same registers; block nch computes coef = bdot(mean, x).
 // C = columns per lane (na <= 256*C).  P: element type of the packed samples
 // (double; float in the precision form, accumulated in FP64).
 template <int C, class P>
-__global__ void __launch_bounds__(256) k_cg_chunk(const P* pack, int rows, int na, const double* x,
+__global__ void __launch_bounds__(256, 2) k_cg_chunk(const P* pack, int rows, int na, const double* x,
                                                   const double* mean, double* cpart, double* coef, const int* ctl) {
   if (ctl[CTL_DONE]) return;
   __shared__ double red[256];
@@ -1765,12 +1790,12 @@ __global__ void __launch_bounds__(256) k_cg_chunk(const P* pack, int rows, int n
   }
   const int r1 = min(rows, (b + 1) * CG_RCH);
   for (int r = b * CG_RCH; r < r1; ++r) {
-    const P* row = pack + (size_t)r * na;
-    // unconditional loads (index clamped in bounds; lanes past na are masked
-    // in the sums below) so all C loads are in flight before the first use
+    // one buffer resource per row: element i >= na reads 0 in hardware, so
+    // the C loads carry no branch and are all in flight before the first use
+    const RowBuf<P> row(pack + (size_t)r * na, na);
     P raw[C];
 #pragma unroll
-    for (int k = 0; k < C; ++k) raw[k] = row[min(tid + 256 * k, na - 1)];
+    for (int k = 0; k < C; ++k) raw[k] = row.load(tid + 256 * k);
     double o[C];
 #pragma unroll
     for (int k = 0; k < C; ++k) o[k] = (double)raw[k];
