@@ -40,6 +40,21 @@ int mvmc_pfinv_f64(const double* planes, int n, int count, double* pf, double* i
  * overwritten with the true inverse. */
 int mvmc_pfinv_f64_inplace(double* planes, int n, int count, double* pf, int* status, void* stream);
 
+/* (a) Batched Pfaffian + inverse of complex skew-symmetric planes.
+ * Replaces mvmc_gpu::BatchedEngine::run_c64 (crates/mvmc-gpu/src/lib.rs:109-115;
+ * CPU reference crates/mvmc-gpu/src/scalar.rs:146-170): planes [n,n,count]
+ * column-major Complex64 (re, im interleaved; 16-byte aligned), n even,
+ * 2 <= n <= 512; pf[2*count] (re, im), inv [n,n,count] complex = the TRUE
+ * inverse, status[count] as mvmc_pfinv_f64 (zero-pivot planes: pf = 0 and a
+ * zero inverse; non-finite planes: the computed pf and a zero inverse).
+ * order 0: zsktf2 (crates/pfapack/src/ltl.rs:39-43, the run_c64 order);
+ * order 1: zsktf2_c_compat (ltl.rs:54-59, the order of the production
+ * calc_m_all_complex, crates/mvmc-core/src/pfaffian.rs:383-398).
+ * inv == planes factorises in place.  Device pointers; asynchronous on
+ * `stream` (NULL = default stream). */
+int mvmc_pfinv_c128(const double* planes, int n, int count, int order, double* pf, double* inv, int* status,
+                    void* stream);
+
 /* Walker ensemble: W independent chains (= W ungrouped C ranks, walker w
  * seeded RndSeed + w; crates/mvmc-core/src/multichain.rs:88-96) of one
  * Expert-mode model, resident in HBM.  Covers the SamplingEngine/RealPfStage

@@ -22,6 +22,7 @@
 #include "../../include/mvmc_hip.h"
 #include "../host/model.h"
 #include "ensemble.hip"
+#include "pfinv_c128_dev.h"
 
 using namespace mvmc_hip;
 
@@ -936,6 +937,34 @@ int mvmc_pfinv_f64(const double* planes, int n, int count, double* pf, double* i
 
 int mvmc_pfinv_f64_inplace(double* planes, int n, int count, double* pf, int* status, void* stream) {
   return mvmc_pfinv_f64(planes, n, count, pf, planes, status, stream);
+}
+
+int mvmc_pfinv_c128(const double* planes, int n, int count, int order, double* pf, double* inv, int* status,
+                    void* stream) {
+  return guarded([&] {
+    require_gpu();
+    if (n < 2 || (n % 2) != 0 || n > PFC_NMAX || count < 0)
+      throw Fail(MVMC_E_INVALID_SHAPE, "n must be even, 2 <= n <= 512");
+    if (order != 0 && order != 1) throw Fail(MVMC_E_INVALID_SHAPE, "order must be 0 (zsktf2) or 1 (C order)");
+    if (count == 0) return;
+    if (!planes || !pf || !inv || !status) throw Fail(MVMC_E_INVALID_SHAPE, "null buffer");
+    if (((uintptr_t)planes | (uintptr_t)pf | (uintptr_t)inv) % 16 != 0)
+      throw Fail(MVMC_E_INVALID_SHAPE, "complex buffers must be 16-byte aligned");
+    hipStream_t st = (hipStream_t)stream;
+    const size_t nn = (size_t)n * n;
+    if ((const double*)inv != planes)
+      HIPCHK(hipMemcpyAsync(inv, planes, sizeof(cz) * nn * count, hipMemcpyDeviceToDevice, st));
+    int dev = 0, cus = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int grid = std::min(count, 2 * std::max(cus, 1));  // planes stride over a resident grid
+    cz* work = nullptr;
+    HIPCHK(hipMallocAsync((void**)&work, sizeof(cz) * 2 * nn * grid, st));
+    hipLaunchKernelGGL(k_pfinv_c128, dim3(grid), dim3(256), 0, st, (cz*)inv, work, n, count, order, (cz*)pf,
+                       status);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipFreeAsync(work, st));
+  });
 }
 
 int mvmc_ens_create(const char* namelist, const char* initial_def, int walkers, long long seed_override,

@@ -1998,7 +1998,20 @@ __global__ void __launch_bounds__(256) k_pfinv_batched(double* planes, double* w
   for (int p = threadIdx.x; p < cnt; p += blockDim.x) {
     const int info = sh.info[p];
     status[first + p] = info;
-    if (info != 0) pf[first + p] = 0.0;
+    if (info > 0) {
+      pf[first + p] = 0.0;
+    } else if (info < 0) {
+      // non-finite Pfaffian: reported with its value (crates/mvmc-gpu/src/lib.rs:14-15),
+      // recomputed as wg_pfinv forms it from the LTL form the plane still holds
+      const double* a = pl + (size_t)p * n * n;
+      double v = 1.0;
+      for (int i = 0; i + 1 < n; i += 2) v *= a[i + (size_t)(i + 1) * n];
+      bool neg = false;
+      for (int k = 0; k < n; ++k)
+        if (sh.P(p, k) != k + 1) neg = !neg;
+      const double s = neg ? (0.0 - 1.0) : 1.0;
+      pf[first + p] = s * v;
+    }
   }
   __syncthreads();
   for (int p = 0; p < cnt; ++p)

@@ -42,6 +42,8 @@ def lib():
         L.mvmc_last_error.restype = ctypes.c_char_p
         L.mvmc_pfinv_f64.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_void_p, ctypes.c_void_p]
+        L.mvmc_pfinv_c128.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.mvmc_ens_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_longlong,
                                       ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
         for name in ["mvmc_ens_destroy", "mvmc_ens_sample", "mvmc_ens_measure", "mvmc_ens_status"]:
@@ -61,7 +63,7 @@ EXPORTED_SYMBOLS = [
     "mvmc_ens_measure", "mvmc_ens_sr_cg", "mvmc_ens_opt_step", "mvmc_ens_get_energy", "mvmc_ens_get_operands",
     "mvmc_ens_get_o_store", "mvmc_ens_get_walker", "mvmc_ens_get_tables", "mvmc_ens_get_config", "mvmc_ens_get_cg",
     "mvmc_ens_timings", "mvmc_ens_phase_clocks", "mvmc_ens_status", "mvmc_sr_cg_solve", "mvmc_ens_set_precision",
-    "mvmc_ens_vjp", "mvmc_ens_log_psi", "mvmc_ens_sr_direct",
+    "mvmc_ens_vjp", "mvmc_ens_log_psi", "mvmc_ens_sr_direct", "mvmc_pfinv_c128",
 ]
 
 
@@ -88,6 +90,17 @@ def pfaffian_inverse_batched(planes_dev, pf_dev, inv_dev, status_dev, n, count, 
     _check(lib().mvmc_pfinv_f64(ctypes.c_void_p(addr(planes_dev)), n, count, ctypes.c_void_p(addr(pf_dev)),
                                 ctypes.c_void_p(addr(inv_dev)), ctypes.c_void_p(addr(status_dev)),
                                 ctypes.c_void_p(stream or 0)))
+
+
+def pfaffian_inverse_batched_c128(planes_dev, pf_dev, inv_dev, status_dev, n, count, order=0, stream=None):
+    """Complex form (BatchedEngine::run_c64): complex128 device tensors (or raw
+    addresses), planes [count][n][n] column-major; order 0 = zsktf2 (the
+    batched reference), 1 = the C-order LTL of the production sampler."""
+    def addr(x):
+        return x if isinstance(x, int) else x.data_ptr()
+    _check(lib().mvmc_pfinv_c128(ctypes.c_void_p(addr(planes_dev)), n, count, int(order),
+                                 ctypes.c_void_p(addr(pf_dev)), ctypes.c_void_p(addr(inv_dev)),
+                                 ctypes.c_void_p(addr(status_dev)), ctypes.c_void_p(stream or 0)))
 
 
 class Ensemble:

@@ -26,6 +26,7 @@
 #include "../mvmc-rs_amd/csrc/ddmath.h"
 #include "../mvmc-rs_amd/host/model.h"
 #include "pfapack.h"
+#include "pfapack_c.h"
 #include "sfmt.h"
 
 using mvmc::Model;
@@ -1166,10 +1167,52 @@ void oracle_pfinv(const double* planes, int n, int count, double* pf, double* in
       continue;
     }
     double v = utu2pfa(a, n, piv.data());
-    if (!std::isfinite(v)) { status[p] = -1; pf[p] = 0.0; std::memset(a, 0, sizeof(double) * n * n); continue; }
+    // a non-finite Pfaffian is reported with its computed value
+    // (crates/mvmc-gpu/src/lib.rs:14-15, scalar.rs:84-87)
+    if (!std::isfinite(v)) { status[p] = -1; pf[p] = v; std::memset(a, 0, sizeof(double) * n * n); continue; }
     pf[p] = v;
     utu2inv(a, n, piv.data());
     status[p] = 0;
+  }
+}
+
+// BatchedEngine::run_c64 (crates/mvmc-gpu/src/scalar.rs:146-170): complex
+// planes (re, im interleaved) -> pf, true inverse, status.  order 0: zsktf2
+// (the run_c64 reference); order 1: zsktf2_c_compat (production calc_m_all).
+void oracle_pfinv_c128(const double* planes, int n, int count, int order, double* pf, double* inv, int* status) {
+  std::vector<int> piv(n);
+  for (int p = 0; p < count; ++p) {
+    zc* a = reinterpret_cast<zc*>(inv) + (size_t)p * n * n;
+    std::memcpy(a, planes + (size_t)p * n * n * 2, sizeof(zc) * n * n);
+    const int info = zsktf2(a, n, piv.data(), order);
+    if (info != 0) {
+      status[p] = info;
+      pf[2 * p] = 0.0;
+      pf[2 * p + 1] = 0.0;
+      std::memset(a, 0, sizeof(zc) * n * n);
+      continue;
+    }
+    const zc v = zutu2pfa(a, n, piv.data());
+    pf[2 * p] = v.re;
+    pf[2 * p + 1] = v.im;
+    if (!(std::isfinite(v.re) && std::isfinite(v.im))) {
+      status[p] = -1;
+      std::memset(a, 0, sizeof(zc) * n * n);
+      continue;
+    }
+    zutu2inv(a, n, piv.data());
+    status[p] = 0;
+  }
+}
+
+// Julia's complex division / reciprocal, elementwise over interleaved arrays
+// (which 0: z / w, 1: 1 / w).
+void oracle_julia_complex(int which, const double* z, const double* w, double* out, long long count) {
+  for (long long i = 0; i < count; ++i) {
+    const zc wi{w[2 * i], w[2 * i + 1]};
+    const zc r = which == 0 ? julia_divide(zc{z[2 * i], z[2 * i + 1]}, wi) : julia_reciprocal(wi);
+    out[2 * i] = r.re;
+    out[2 * i + 1] = r.im;
   }
 }
 

@@ -44,6 +44,8 @@ def lib():
         L.oracle_cg_apply.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 5 + \
             [ctypes.c_double, ctypes.c_double]
         L.oracle_measure_config.restype = ctypes.c_double
+        L.oracle_pfinv_c128.restype = None
+        L.oracle_julia_complex.restype = None
         _lib = L
     return _lib
 
@@ -72,6 +74,28 @@ def pfinv(planes):
     status = np.zeros(count, dtype=np.int32)
     lib().oracle_pfinv(ptr(planes), ctypes.c_int(n), ctypes.c_int(count), ptr(pf), ptr(inv), ptr(status))
     return pf, inv, status
+
+
+def pfinv_c128(planes, order=0):
+    """planes: (count, n, n) complex128 column-major planes stored as [p][c][r];
+    order 0: zsktf2 (the batched reference), 1: the C-order production LTL."""
+    planes = np.ascontiguousarray(planes, dtype=np.complex128)
+    count, n, _ = planes.shape
+    pf = np.zeros(count, dtype=np.complex128)
+    inv = np.zeros_like(planes)
+    status = np.zeros(count, dtype=np.int32)
+    lib().oracle_pfinv_c128(ptr(planes), ctypes.c_int(n), ctypes.c_int(count), ctypes.c_int(order), ptr(pf),
+                            ptr(inv), ptr(status))
+    return pf, inv, status
+
+
+def julia_complex(which, z, w):
+    """which 0: Julia's z / w; 1: Julia's 1 / w (elementwise, complex128)."""
+    w = np.ascontiguousarray(w, dtype=np.complex128)
+    z = np.ascontiguousarray(z if z is not None else np.zeros_like(w), dtype=np.complex128)
+    out = np.zeros_like(w)
+    lib().oracle_julia_complex(ctypes.c_int(which), ptr(z), ptr(w), ptr(out), ctypes.c_longlong(w.size))
+    return out
 
 
 def pfaffian(plane_colmajor, n):

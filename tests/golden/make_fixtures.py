@@ -13,6 +13,7 @@ Everything written here is data held by the reference's tests:
     (overrides from tests/fixtures/mpi_matrix_179/cells.txt)
   * SR-CG iterates                   tests/fixtures/sr_cg/c_refresh/real{,_diag_scale}.txt
   * PfaPack input matrices           tests/fixtures/pfapack/real_n*.orig.txt
+                                     tests/fixtures/pfapack/complex_n*.orig.txt
   * two-hop bilinear bits            tests/fixtures/pfaffian_cg/two_hop_bilinear{,_c}.txt
 """
 import json
@@ -143,6 +144,29 @@ def pfapack():
     np.savez_compressed(os.path.join(OUT, "pfapack_real_inputs.npz"), **arrays)
 
 
+def pfapack_complex():
+    """Complex skew-symmetric inputs (n <= 128) of the reference's identity
+    tests, as complex128 column-major matrices (value = re + i im per line)."""
+    arrays = {}
+    for name in sorted(os.listdir(os.path.join(FIX, "pfapack"))):
+        if not name.startswith("complex_"):
+            continue
+        n = int(re.search(r"_n(\d+)_", name).group(1))
+        if n > 128:
+            continue
+        vals = []
+        for line in open(os.path.join(FIX, "pfapack", name)):
+            parts = line.split()
+            if len(parts) != 2 or line.startswith("#") or not parts[0][0] in "+-0123456789":
+                continue
+            try:
+                vals.append(complex(float(parts[0]), float(parts[1])))
+            except ValueError:
+                continue
+        arrays[name.replace(".orig.txt", "")] = np.array(vals, dtype=np.complex128).reshape(n, n, order="F")
+    np.savez_compressed(os.path.join(OUT, "pfapack_complex_inputs.npz"), **arrays)
+
+
 def bilinear():
     lines = [l for l in open(os.path.join(FIX, "pfaffian_cg/two_hop_bilinear.txt")) if l.strip() and not l.startswith("#")]
     expected = [l.split() for l in open(os.path.join(FIX, "pfaffian_cg/two_hop_bilinear_c.txt")) if not l.startswith("#")]
@@ -165,5 +189,6 @@ if __name__ == "__main__":
     heisenberg()
     sr_cg()
     pfapack()
+    pfapack_complex()
     bilinear()
     print("fixtures written to", OUT)

@@ -106,6 +106,100 @@ def test_pfinv_rejects_odd_n(M):
                                    torch.zeros(1, dtype=torch.int32, device="cuda"), 3, 1)
 
 
+def test_pfinv_nonfinite_plane_reports_value(M):
+    """A plane whose Pfaffian overflows: status -1, the computed (infinite)
+    Pfaffian reported as the oracle forms it, zero inverse; its neighbour is
+    unaffected (crates/mvmc-gpu/src/lib.rs:12-15)."""
+    import torch
+    rng = np.random.default_rng(12)
+    big = _random_skew(rng, 8) * 1e200
+    ok = _random_skew(rng, 8)
+    planes = np.stack([big.T.copy(), ok.T.copy()])
+    pf_o, inv_o, st_o = O.pfinv(planes)
+    assert st_o[0] == -1 and np.isinf(pf_o[0]) and st_o[1] == 0
+    dp = torch.tensor(planes, device="cuda")
+    pf = torch.zeros(2, dtype=torch.float64, device="cuda")
+    inv = torch.zeros_like(dp)
+    st = torch.zeros(2, dtype=torch.int32, device="cuda")
+    M.pfaffian_inverse_batched(dp, pf, inv, st, 8, 2)
+    torch.cuda.synchronize()
+    assert np.array_equal(st.cpu().numpy(), st_o)
+    assert np.array_equal(pf.cpu().numpy(), pf_o)
+    assert np.array_equal(inv.cpu().numpy(), inv_o)
+
+
+def _random_skew_c(rng, n):
+    a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    return a - a.T
+
+
+def _run_c128(M, planes, n, order):
+    import torch
+    dp = torch.tensor(planes, dtype=torch.complex128, device="cuda")
+    count = planes.shape[0]
+    pf = torch.zeros(count, dtype=torch.complex128, device="cuda")
+    inv = torch.zeros_like(dp)
+    st = torch.zeros(count, dtype=torch.int32, device="cuda")
+    M.pfaffian_inverse_batched_c128(dp, pf, inv, st, n, count, order)
+    torch.cuda.synchronize()
+    return pf.cpu().numpy(), inv.cpu().numpy(), st.cpu().numpy()
+
+
+@pytest.mark.parametrize("order", [0, 1])
+def test_pfinv_c128_bit_exact_vs_oracle(M, order):
+    """Complex planes (BatchedEngine::run_c64): the reference's complex inputs
+    and random planes on both sides of the 64-column product switch (n = 64,
+    66), at the 12x12 flux lattice's n = 144 and above 256 rows (n = 258):
+    Pfaffians, inverses and statuses bit-identical to the oracle, for the
+    zsktf2 order (0) and the C order of the production sampler (1)."""
+    rng = np.random.default_rng(31 + order)
+    data = np.load(os.path.join(GOLDEN, "pfapack_complex_inputs.npz"))
+    cases = [data[k] for k in data.files] + [_random_skew_c(rng, 2) for _ in range(3)] + \
+            [_random_skew_c(rng, 10) for _ in range(5)] + [_random_skew_c(rng, 64)] + \
+            [_random_skew_c(rng, 66) for _ in range(2)] + [_random_skew_c(rng, 144) for _ in range(3)] + \
+            [_random_skew_c(rng, 258)]
+    by_n = {}
+    for A in cases:
+        by_n.setdefault(A.shape[0], []).append(A)
+    for n, mats in sorted(by_n.items()):
+        log(f"pfinv_c128 order={order} n={n} count={len(mats)}")
+        planes = np.stack([m.T.copy() for m in mats])
+        pf_o, inv_o, st_o = O.pfinv_c128(planes, order)
+        pf, inv, st = _run_c128(M, planes, n, order)
+        assert np.array_equal(st, st_o), n
+        assert (st == 0).all(), n
+        assert np.array_equal(pf, pf_o), n
+        assert np.array_equal(inv, inv_o), n
+
+
+def test_pfinv_c128_statuses_and_shapes(M):
+    """Zero plane and zero-column plane: positive INFO, pf = 0, zero inverse;
+    a non-finite Pfaffian: status -1 with its value; neighbours unaffected;
+    odd n and unknown orders rejected."""
+    import torch
+    rng = np.random.default_rng(13)
+    z = np.zeros((4, 4, 4), dtype=np.complex128)
+    z[1][0, 1], z[1][1, 0], z[1][2, 3], z[1][3, 2] = 1 + 1j, -1 - 1j, 2j, -2j
+    z[2] = (_random_skew_c(rng, 4) * 1e200).T
+    z[3] = _random_skew_c(rng, 4).T
+    for order in (0, 1):
+        pf_o, inv_o, st_o = O.pfinv_c128(z, order)
+        pf, inv, st = _run_c128(M, z, 4, order)
+        assert st_o[0] > 0 and st_o[2] == -1 and st_o[3] == 0
+        assert np.array_equal(st, st_o)
+        # inf - inf in the overflowing product: NaN payloads are not compared
+        assert not np.isfinite(pf[2]) and np.array_equal(pf, pf_o, equal_nan=True)
+        assert np.array_equal(inv, inv_o)
+    p = torch.zeros((1, 3, 3), dtype=torch.complex128, device="cuda")
+    out = torch.zeros(1, dtype=torch.complex128, device="cuda")
+    sti = torch.zeros(1, dtype=torch.int32, device="cuda")
+    with pytest.raises(M.InvalidShape):
+        M.pfaffian_inverse_batched_c128(p, out, p, sti, 3, 1)
+    p4 = torch.zeros((1, 4, 4), dtype=torch.complex128, device="cuda")
+    with pytest.raises(M.InvalidShape):
+        M.pfaffian_inverse_batched_c128(p4, out, p4, sti, 4, 1, order=2)
+
+
 def test_hubbard_step1_operands(M):
     log("hubbard step-1 sample+measure")
     O.set_math(1)
