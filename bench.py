@@ -113,6 +113,44 @@ def mixed_form(info, walkers, args, rows64, total64):
                           "Pfaffians, periodic rebuilds, measurement, SR in FP64"}
 
 
+def complex_pfinv(n=144, count=8192, cpu_planes=16):
+    """Batched complex Pfaffian + inverse (mvmc_pfinv_c128, the run_c64 seam)
+    at the 12x12 flux lattice's plane size: device time of one call over
+    `count` random skew planes already in HBM (HIP events on the library's
+    stream), flops counted as 8 n^3 per plane (LTL, trtri and M^T C at 8 real
+    flops per complex multiply-add), beside the oracle on `cpu_planes` planes
+    on one host core."""
+    import torch
+    import mvmc_hip
+    import oracle_py
+    g = torch.Generator(device="cuda").manual_seed(5)
+    a = torch.randn(count, n, n, dtype=torch.complex128, device="cuda", generator=g)
+    a = (a - a.transpose(1, 2)).contiguous()
+    pf = torch.zeros(count, dtype=torch.complex128, device="cuda")
+    inv = torch.empty_like(a)
+    st = torch.zeros(count, dtype=torch.int32, device="cuda")
+    res = {"n": n, "planes": count}
+    for order, name in ((0, "zsktf2"), (1, "c_order")):
+        mvmc_hip.pfaffian_inverse_batched_c128(a, pf, inv, st, n, count, order)   # warm-up
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        mvmc_hip.pfaffian_inverse_batched_c128(a, pf, inv, st, n, count, order)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1)
+        assert int((st != 0).sum()) == 0
+        res[name] = {"ms": ms, "planes_per_s": count / (ms / 1e3),
+                     "achieved_TFLOPs": 8.0 * n ** 3 * count / (ms / 1e3) / 1e12,
+                     "frac_fp64": 8.0 * n ** 3 * count / (ms / 1e3) / 1e12 / 78.6}
+    host = a[:cpu_planes].cpu().numpy()
+    del a, inv
+    t = time.perf_counter()
+    oracle_py.pfinv_c128(host, 0)
+    dt = time.perf_counter() - t
+    res["cpu_oracle"] = {"planes_per_s": cpu_planes / dt, "cores": 1, "sample": f"{cpu_planes} planes, zsktf2 order"}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +165,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-direct", action="store_true", help="skip the direct-SR measurement")
+    ap.add_argument("--no-c128", action="store_true", help="skip the complex batched Pfaffian measurement")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the precision-form run (sampler inverse in FP32) reported beside the FP64 line")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -138,6 +177,11 @@ def main():
               f"by construction but no multi-GPU path is built", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe_mib > 0 or not args.no_c128:
+        # torch ships its own HIP runtime under the same sonames: it has to
+        # bring the device up before this library's first HIP call
+        import torch
+        torch.cuda.init()
     import lattice
     import mvmc_hip
     if args.copy_probe_mib > 0:
@@ -248,6 +292,8 @@ def main():
             "active_parameters": na_, "rows": rows_, "ms": td["cg_ms"],
             "achieved_TFLOPs": fl / (td["cg_ms"] / 1e3) / 1e12,
             "frac_fp64_mfma": fl / (td["cg_ms"] / 1e3) / 1e12 / 78.6}
+    if not args.no_c128:
+        sec["k_pfinv_c128 (a, complex planes of the 12x12 flux lattice)"] = complex_pfinv()
     out["secondary_rooflines"] = sec
     out["autotuned"] = {"rebuild_variant": ["3 workgroups/CU, 8-column close batches",
                                             "2 workgroups/CU, 16-column close batches",
