@@ -109,8 +109,9 @@ def mixed_form(info, walkers, args, rows64, total64):
             "sampling_s_per_step": (tm1["sample_s"] - tm0["sample_s"]) / args.steps,
             "energy_per_site": [r[0] / info["nsite"] for r in rows], "abs_dE_per_site_vs_fp64": de,
             "fp64_stat_err_per_site": se,
-            "arithmetic": "sampler inverse stored in FP32, its rank-2 updates in FP32; ratios, vec1 sums, "
-                          "Pfaffians, periodic rebuilds, measurement, SR in FP64"}
+            "arithmetic": "sampler inverse stored in FP32, its rank-2 updates in FP32; SR-CG samples stored in "
+                          "FP32, accumulated in FP64; ratios, vec1 sums, Pfaffians, periodic rebuilds, "
+                          "measurement, CG scalars and vectors in FP64"}
 
 
 def complex_pfinv(n=144, count=8192, cpu_planes=16):
@@ -166,6 +167,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-direct", action="store_true", help="skip the direct-SR measurement")
     ap.add_argument("--no-c128", action="store_true", help="skip the complex batched Pfaffian measurement")
+    ap.add_argument("--c128-only", action="store_true",
+                    help="run only the complex batched Pfaffian measurement and print its JSON (profiling aid; "
+                         "not the benchmark line)")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the precision-form run (sampler inverse in FP32) reported beside the FP64 line")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -177,6 +181,11 @@ def main():
               f"by construction but no multi-GPU path is built", file=sys.stderr)
         sys.exit(2)
 
+    if args.c128_only:
+        import torch
+        torch.cuda.init()
+        print(json.dumps({"k_pfinv_c128": complex_pfinv()}), flush=True)
+        return
     if args.copy_probe_mib > 0 or not args.no_c128:
         # torch ships its own HIP runtime under the same sonames: it has to
         # bring the device up before this library's first HIP call
