@@ -957,7 +957,7 @@ int mvmc_pfinv_c128(const double* planes, int n, int count, int order, double* p
     int dev = 0, cus = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const int grid = std::min(count, 2 * std::max(cus, 1));  // planes stride over a resident grid
+    const int grid = std::min(count, 4 * std::max(cus, 1));  // planes stride over a resident grid
     cz* work = nullptr;
     HIPCHK(hipMallocAsync((void**)&work, sizeof(cz) * 2 * nn * grid, st));
     hipLaunchKernelGGL(k_pfinv_c128, dim3(grid), dim3(256), 0, st, (cz*)inv, work, n, count, order, (cz*)pf,

@@ -36,6 +36,7 @@
 namespace mvmc_hip {
 
 constexpr int PFC_NMAX = 512;  // largest plane side (LDS vectors below)
+constexpr int PFC_U = 4;       // elements a lane loads before it stores in the rank-2 update
 
 struct alignas(16) cz {
   double re, im;
@@ -146,10 +147,9 @@ struct PfcShared {
   int piv[PFC_NMAX];
   int lab_c[PFC_NMAX];  // final column j holds column lab_c[j] (sequential swaps composed)
   int lab_r[PFC_NMAX];
-  cz vt[PFC_NMAX];
-  cz rneg[PFC_NMAX];  // Julia 1 / (-vt[i])
-  cz rpos[PFC_NMAX];  // Julia 1 / vt[i]
-  cz tcol[PFC_NMAX];
+  // four complex vectors of PFC_NMAX: during the LTL x, y, t1, t2 of the
+  // current step; during the inverse vt, 1/(-vt), 1/vt and a trtri column
+  cz buf[4 * PFC_NMAX];
   int kp, zero_col, info;
 };
 
@@ -159,6 +159,14 @@ __device__ void wg_pfinv_c128(cz* a, cz* mt, cz* wk, int n, int order, PfcShared
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;
   auto A = [&](int i, int j) -> cz& { return a[i + (size_t)j * n]; };
+  cz* const cx = sh.buf;                 // LTL step: column k0
+  cz* const cy = sh.buf + PFC_NMAX;      //           column kk0
+  cz* const t1v = sh.buf + 2 * PFC_NMAX; //           alpha y_j
+  cz* const t2v = sh.buf + 3 * PFC_NMAX; //           alpha x_j
+  cz* const vtv = sh.buf;                // inverse: vt
+  cz* const rneg = sh.buf + PFC_NMAX;    //          Julia 1 / (-vt[i])
+  cz* const rpos = sh.buf + 2 * PFC_NMAX;//          Julia 1 / vt[i]
+  cz* const tcol = sh.buf + 3 * PFC_NMAX;//          trtri column
   for (int i = tid; i < n; i += nthr) sh.piv[i] = i + 1;
   if (tid == 0) sh.info = 0;
   __syncthreads();
@@ -243,22 +251,48 @@ __device__ void wg_pfinv_c128(cz* a, cz* mt, cz* wk, int n, int order, PfcShared
         const double den = pv.re * pv.re + pv.im * pv.im;
         alpha = cz{pv.re / den, -pv.im / den};
       }
-      const int tot = kk0 * kk0;
-      for (int e = tid; e < tot; e += nthr) {
-        const int j = e / kk0, i = e - j * kk0;
-        if (i < j) {
-          const cz t1 = cz_mul(alpha, A(j, kk0));
-          const cz t2 = cz_mul(alpha, A(j, k0));
-          const cz p = cz_mul(A(i, k0), t1);
-          const cz q = cz_mul(A(i, kk0), t2);
-          A(i, j) = cz_add(A(i, j), cz_sub(p, q));
-        } else if (i == j) {
-          A(j, j) = cz{0.0, 0.0};
-        }
+      // x = A(0..kk0-1, k0), y = A(0..kk0-1, kk0) and the per-column
+      // t1 = alpha y_j, t2 = alpha x_j into LDS: the update below then reads
+      // and writes only its own elements
+      for (int t = tid; t < kk0; t += nthr) {
+        const cz x = A(t, k0), y = A(t, kk0);
+        cx[t] = x;
+        cy[t] = y;
+        t1v[t] = cz_mul(alpha, y);
+        t2v[t] = cz_mul(alpha, x);
       }
       __syncthreads();
+      // strict upper triangle i < j < kk0 enumerated without gaps: column j
+      // holds j elements starting at j(j-1)/2; a lane loads PFC_U elements
+      // before it stores any.  The column scaling of k0 writes elements the
+      // update neither reads nor writes.
+      const int tot = kk0 * (kk0 - 1) / 2;
+      for (int e0 = tid; e0 < tot; e0 += nthr * PFC_U) {
+        int ii[PFC_U], jj[PFC_U];
+        cz c[PFC_U];
+#pragma unroll
+        for (int u = 0; u < PFC_U; ++u) {
+          const int e = e0 + u * nthr;
+          int j = (int)((1.0 + sqrt(1.0 + 8.0 * (double)e)) * 0.5);
+          while (j > 1 && j * (j - 1) / 2 > e) --j;
+          while ((j + 1) * j / 2 <= e) ++j;
+          ii[u] = e - j * (j - 1) / 2;
+          jj[u] = j;
+          if (e < tot) c[u] = A(ii[u], j);
+        }
+#pragma unroll
+        for (int u = 0; u < PFC_U; ++u) {
+          if (e0 + u * nthr < tot) {
+            const int i = ii[u], j = jj[u];
+            const cz p = cz_mul(cx[i], t1v[j]);
+            const cz q = cz_mul(cy[i], t2v[j]);
+            A(i, j) = cz_add(c[u], cz_sub(p, q));
+          }
+        }
+      }
       for (int t = tid; t < kk0; t += nthr) {
-        const cz v = A(t, k0);
+        A(t, t) = cz{0.0, 0.0};
+        const cz v = cx[t];
         A(t, k0) = order == 0 ? cz_mul(v, alpha)
                               : cz{fma(alpha.re, v.re, -(alpha.im * v.im)), fma(alpha.re, v.im, alpha.im * v.re)};
       }
@@ -297,10 +331,10 @@ __device__ void wg_pfinv_c128(cz* a, cz* mt, cz* wk, int n, int order, PfcShared
     for (int i = tid; i < j; i += nthr) {
       cz acc = A(i, j + 1);
       for (int k = i + 1; k < j; ++k) acc = cz_add(acc, cz_mul(A(i, k + 1), A(k, j + 1)));
-      sh.tcol[i] = cz_neg(acc);
+      tcol[i] = cz_neg(acc);
     }
     __syncthreads();
-    for (int i = tid; i < j; i += nthr) A(i, j + 1) = sh.tcol[i];
+    for (int i = tid; i < j; i += nthr) A(i, j + 1) = tcol[i];
     __syncthreads();
   }
   // (2) M = I except M(k,i) = A(k,i+1) for 1 <= i <= n-2, k < i, stored
@@ -314,9 +348,9 @@ __device__ void wg_pfinv_c128(cz* a, cz* mt, cz* wk, int n, int order, PfcShared
   }
   for (int i = tid; i < n - 1; i += nthr) {
     const cz v = cz_neg(A(i, i + 1));
-    sh.vt[i] = v;
-    sh.rneg[i] = julia_recip(cz_neg(v));
-    sh.rpos[i] = julia_recip(v);
+    vtv[i] = v;
+    rneg[i] = julia_recip(cz_neg(v));
+    rpos[i] = julia_recip(v);
   }
   if (tid == 0) {
     for (int j = 0; j < n; ++j) {
@@ -339,13 +373,13 @@ __device__ void wg_pfinv_c128(cz* a, cz* mt, cz* wk, int n, int order, PfcShared
   // (3) skew-tridiagonal solve T C = M, one column per lane, C into A
   auto Mx = [&](int r, int j) -> cz { return mt[j + (size_t)r * n]; };  // M(r, j)
   for (int j = tid; j < n; j += nthr) {
-    A(1, j) = cz_mul(Mx(0, j), sh.rneg[0]);
+    A(1, j) = cz_mul(Mx(0, j), rneg[0]);
     for (int ij = 2; ij < n; ij += 2)
-      A(ij + 1, j) = cz_mul(cz_sub(Mx(ij, j), cz_mul(A(ij - 1, j), sh.vt[ij - 1])), sh.rneg[ij]);
-    A(n - 2, j) = cz_mul(Mx(n - 1, j), sh.rpos[n - 2]);
+      A(ij + 1, j) = cz_mul(cz_sub(Mx(ij, j), cz_mul(A(ij - 1, j), vtv[ij - 1])), rneg[ij]);
+    A(n - 2, j) = cz_mul(Mx(n - 1, j), rpos[n - 2]);
     if (n >= 4)
       for (int i = n - 3; i >= 1; i -= 2)
-        A(i - 1, j) = cz_mul(cz_add(Mx(i, j), cz_mul(A(i + 1, j), sh.vt[i])), sh.rpos[i - 1]);
+        A(i - 1, j) = cz_mul(cz_add(Mx(i, j), cz_mul(A(i + 1, j), vtv[i])), rpos[i - 1]);
   }
   __syncthreads();
   // (4) product W = M^T C: n > 64, the strict upper triangle of M^T C (then
