@@ -172,6 +172,30 @@ def test_pfinv_c128_bit_exact_vs_oracle(M, order):
         assert np.array_equal(inv, inv_o), n
 
 
+def test_pfinv_c128_persistent_grid_in_place(M):
+    """More planes than the resident grid holds (each workgroup loops over
+    several planes), factorised in place (inv == planes), with a zero plane
+    inside the batch: every plane bit-identical to the oracle."""
+    import torch
+    rng = np.random.default_rng(17)
+    count, n = 1500, 8
+    mats = [_random_skew_c(rng, n) for _ in range(count)]
+    mats[777] = np.zeros((n, n), dtype=np.complex128)
+    planes = np.stack([m.T.copy() for m in mats])
+    for order in (0, 1):
+        log(f"pfinv_c128 in place order={order} count={count}")
+        pf_o, inv_o, st_o = O.pfinv_c128(planes, order)
+        dp = torch.tensor(planes, dtype=torch.complex128, device="cuda")
+        pf = torch.zeros(count, dtype=torch.complex128, device="cuda")
+        st = torch.zeros(count, dtype=torch.int32, device="cuda")
+        M.pfaffian_inverse_batched_c128(dp, pf, dp, st, n, count, order)
+        torch.cuda.synchronize()
+        assert st_o[777] > 0 and (np.delete(st_o, 777) == 0).all()
+        assert np.array_equal(st.cpu().numpy(), st_o)
+        assert np.array_equal(pf.cpu().numpy(), pf_o)
+        assert np.array_equal(dp.cpu().numpy(), inv_o)
+
+
 def test_pfinv_c128_statuses_and_shapes(M):
     """Zero plane and zero-column plane: positive INFO, pf = 0, zero inverse;
     a non-finite Pfaffian: status -1 with its value; neighbours unaffected;
