@@ -509,6 +509,36 @@ def test_uneven_lattices_side_by_side(M, tmp_path):
         ens[k].close()
 
 
+def test_walkers_beyond_resident_grid_match_single_walker_oracle(M, tmp_path):
+    """16x16 benchmark lattice with more than twice as many walkers as the
+    sampler's resident grid holds, so walkers wait for slots, are handed to
+    the rebuild kernel and resume in later launches: after the first
+    sampling phase (thermalisation + one sample, no SR yet) every checked
+    walker, from each scheduling round, has the configuration, SFMT state and
+    counters of a one-walker oracle run seeded RndSeed + w (walkers are
+    independent chains until the first SR update, multichain.rs:88)."""
+    import lattice
+    info = lattice.write_workload("square16", str(tmp_path / "sq16"), sr_steps=1, warmup=2)
+    W = 1100
+    ens = M.Ensemble(info["namelist"], info["initial"], walkers=W)
+    assert W > 2 * ens.grid, ens.grid
+    log(f"square16 W={W} grid={ens.grid}: sampling")
+    ens.sample()
+    O.set_math(1)
+    try:
+        for w in (0, ens.grid - 1, ens.grid, 2 * ens.grid + 1, W - 1):
+            log(f"  walker {w} vs one-walker oracle")
+            run = O.Run(info["namelist"], info["initial"], walkers=1, seed=11272 + w)
+            run.sample()
+            sg, so = ens.walker(w), run.walker(0)
+            assert np.array_equal(sg["ele_idx"], so["ele_idx"]), w
+            assert np.array_equal(sg["rng"], so["rng"]) and sg["rng_idx"] == so["rng_idx"], w
+            assert list(sg["counter"]) == list(so["counter"][:4]), (w, sg["counter"], so["counter"][:4])
+    finally:
+        O.set_math(0)
+        ens.close()
+
+
 @pytest.mark.parametrize("name,W,warmup", [("square8", 2, 4), ("square16", 1, 2)])
 def test_benchmark_lattices_match_oracle(M, tmp_path, name, W, warmup):
     """The benchmark's own lattices (8x8: Nsize 64; 16x16 t-t'-U at 1/8
