@@ -101,6 +101,19 @@ int mvmc_ens_sample(mvmc_ens* e);
  * (crates/mvmc-core/src/run.rs:2937-3616, 1863-1903; average.rs:18-63). */
 int mvmc_ens_measure(mvmc_ens* e);
 
+/* (c)+(3) on caller-supplied configurations: MeasurementStages::measure_sample
+ * batched over samples (crates/mvmc-core/src/stage_backend.rs:418-438, the
+ * per-sample work of run.rs:2937-3616).  ele_idx [count][n] host array in the
+ * reference's EleIdx layout (electron mi + s*Ne -> site), count <= walkers x
+ * NVMCSample; outputs (host) e_loc[count] (0 for an invalid sample),
+ * o[count][1 + NPara] = [1, projection counts, Slater derivatives] (may be
+ * NULL), valid[count] (0 when calc_m_all fails or the overlap vanishes; may be
+ * NULL).  Uses the current parameters.  Overwrites the ensemble's saved
+ * samples and inverse tables: call mvmc_ens_sample before the next
+ * mvmc_ens_measure.  Out-of-range sites or doubly occupied spin-sites return
+ * MVMC_E_INPUT. */
+int mvmc_ens_measure_configs(mvmc_ens* e, int count, const int* ele_idx, double* e_loc, double* o, int* valid);
+
 /* SR-CG solve on the reduced operands, parameter update and gauge sync
  * (crates/mvmc-core/src/sr_cg.rs:201-484, sync.rs:18-104).  cg_info gets the C
  * status (iterations or -(iter+1)). */

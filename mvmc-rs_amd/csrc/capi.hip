@@ -712,6 +712,91 @@ void measure_impl(mvmc_ens* E) {
   E->t_measure += now_s() - t0;
 }
 
+// Measurement of caller-supplied configurations: MeasurementStages::
+// measure_sample batched over samples (crates/mvmc-core/src/stage_backend.rs:
+// 418-438; per sample the work of run.rs:2937-3616).  Configuration c goes to
+// the saved-sample slot of walker c mod W in layer c / W; every layer is
+// rebuilt (k_meas_rebuild) and measured (k_measure) by the kernels of
+// mvmc_ens_measure with the current parameters.  EleCfg / EleNum and the
+// projection counts are derived here (projection.rs:179-277).  The saved
+// samples and the walkers' inverse tables are overwritten.
+void measure_configs_impl(mvmc_ens* E, int count, const int* ele_idx, double* e_loc, double* o, int* valid) {
+  EnsView& v = E->v;
+  if (count < 0 || (long long)count > (long long)v.W * v.nsample)
+    throw Fail(MVMC_E_INVALID_SHAPE, "count must be between 0 and walkers x NVMCSample");
+  if (count == 0) return;
+  if (!ele_idx || !e_loc) throw Fail(MVMC_E_INVALID_SHAPE, "null buffer");
+  const mvmc::Model& m = E->m;
+  const int n = v.n, ne = v.ne, ns = v.ns, n2 = v.n2, np = v.nproj, size = v.size, nsamp = v.nsample;
+  const int npc = np > 0 ? np : 1;
+  E->upload_tables();
+  const int W0 = v.W;
+  struct RestoreW {
+    EnsView& view;
+    int w;
+    ~RestoreW() { view.W = w; }
+  } restore{v, W0};
+  hipStream_t st = E->st;
+  for (int s = 0; (long long)s * W0 < count; ++s) {
+    const int base = s * W0, wl = std::min(W0, count - base);
+    std::vector<int> idx((size_t)wl * n), cfg((size_t)wl * n2, -1), num((size_t)wl * n2, 0), pc((size_t)wl * npc, 0);
+    for (int w = 0; w < wl; ++w) {
+      const int* e = ele_idx + (size_t)(base + w) * n;
+      int* cf = cfg.data() + (size_t)w * n2;
+      int* nm = num.data() + (size_t)w * n2;
+      for (int mi = 0; mi < n; ++mi) {
+        const int ri = e[mi];
+        if (ri < 0 || ri >= ns)
+          throw Fail(MVMC_E_INPUT, "configuration " + std::to_string(base + w) + ": site index out of range");
+        const int slot = ri + (mi / ne) * ns;
+        if (cf[slot] != -1)
+          throw Fail(MVMC_E_INPUT, "configuration " + std::to_string(base + w) + ": two electrons of one spin on a site");
+        cf[slot] = mi % ne;
+        nm[slot] = 1;
+        idx[(size_t)w * n + mi] = ri;
+      }
+      int* cnt = pc.data() + (size_t)w * npc;
+      for (int ri = 0; ri < ns; ++ri) {
+        const int g = m.gutz_idx[ri];
+        if (g >= 0 && g < np) cnt[g] += nm[ri] * nm[ri + ns];
+      }
+      if (np > m.n_gutz)
+        for (int ri = 0; ri < ns; ++ri) {
+          const int xi = nm[ri] + nm[ri + ns] - 1;
+          if (xi == 0) continue;
+          for (int rj = ri + 1; rj < ns; ++rj) {
+            const int j = m.jast_idx[(size_t)ri * ns + rj];
+            if (j < 0) continue;
+            cnt[m.n_gutz + j] += xi * (nm[rj] + nm[rj + ns] - 1);
+          }
+        }
+    }
+    auto put = [&](int* dst, const std::vector<int>& src, int width) {
+      HIPCHK(hipMemcpy2DAsync(dst + (size_t)s * width, sizeof(int) * nsamp * width, src.data(), sizeof(int) * width,
+                              sizeof(int) * width, wl, hipMemcpyHostToDevice, st));
+    };
+    put(v.s_idx, idx, n);
+    put(v.s_cfg, cfg, n2);
+    put(v.s_num, num, n2);
+    if (np > 0) put(v.s_proj, pc, np);
+    v.W = wl;
+    launch_meas_rebuild(E, s);
+    hipLaunchKernelGGL(k_measure, dim3(E->grid_meas), dim3(256), E->lds_meas, st, v, E->d_gscr, E->gstride, 1, s);
+    HIPCHK(hipGetLastError());
+    v.W = W0;
+    HIPCHK(hipMemcpy2DAsync(e_loc + base, sizeof(double), v.e_loc + s, sizeof(double) * nsamp, sizeof(double), wl,
+                            hipMemcpyDeviceToHost, st));
+    if (o)
+      HIPCHK(hipMemcpy2DAsync(o + (size_t)base * size, sizeof(double) * size, v.o_store + (size_t)s * size,
+                              sizeof(double) * size * nsamp, sizeof(double) * size, wl, hipMemcpyDeviceToHost, st));
+    if (valid)
+      HIPCHK(hipMemcpy2DAsync(valid + base, sizeof(int), v.s_valid + s, sizeof(int) * nsamp, sizeof(int), wl,
+                              hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  E->check_status();
+}
+
 // Parameter update of an SR step (sr.rs:408-421, sync.rs): p[map[si]] += x[si].
 void apply_sr_update(mvmc_ens* E, const std::vector<int>& map, const std::vector<double>& x) {
   mvmc::Model& m = E->m;
@@ -1008,6 +1093,9 @@ int mvmc_ens_set_params(mvmc_ens* e, const double* proj, const double* slater) {
 
 int mvmc_ens_sample(mvmc_ens* e) { return guarded([&] { sample_impl(e); }); }
 int mvmc_ens_measure(mvmc_ens* e) { return guarded([&] { measure_impl(e); }); }
+int mvmc_ens_measure_configs(mvmc_ens* e, int count, const int* ele_idx, double* e_loc, double* o, int* valid) {
+  return guarded([&] { measure_configs_impl(e, count, ele_idx, e_loc, o, valid); });
+}
 int mvmc_ens_sr_cg(mvmc_ens* e, long long* info) {
   long long dummy = 0;
   return guarded([&] { sr_impl(e, info ? info : &dummy); });

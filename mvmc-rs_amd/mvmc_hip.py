@@ -64,6 +64,7 @@ EXPORTED_SYMBOLS = [
     "mvmc_ens_get_o_store", "mvmc_ens_get_walker", "mvmc_ens_get_tables", "mvmc_ens_get_config", "mvmc_ens_get_cg",
     "mvmc_ens_timings", "mvmc_ens_phase_clocks", "mvmc_ens_status", "mvmc_sr_cg_solve", "mvmc_ens_set_precision",
     "mvmc_ens_vjp", "mvmc_ens_log_psi", "mvmc_ens_sr_direct", "mvmc_pfinv_c128",
+    "mvmc_ens_measure_configs",
 ]
 
 
@@ -131,6 +132,17 @@ class Ensemble:
 
     def measure(self):
         _check(lib().mvmc_ens_measure(self.h))
+
+    def measure_configs(self, ele_idx):
+        """Local energy, O row and validity of each configuration (rows of
+        ele_idx, EleIdx layout) with the current parameters."""
+        cfg = np.ascontiguousarray(ele_idx, dtype=np.int32).reshape(-1, self.n)
+        count = cfg.shape[0]
+        e = np.zeros(count)
+        o = np.zeros((count, self.size))
+        ok = np.zeros(count, dtype=np.int32)
+        _check(lib().mvmc_ens_measure_configs(self.h, ctypes.c_int(count), _p(cfg), _p(e), _p(o), _p(ok)))
+        return e, o, ok
 
     def sr_cg(self):
         info = ctypes.c_longlong(0)

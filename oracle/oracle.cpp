@@ -1421,4 +1421,28 @@ double oracle_measure_config(void* h, const int* ele_idx, const int* ele_cfg, co
   return e;
 }
 
+// Measurement of one configuration given only its EleIdx: EleCfg, EleNum and
+// the projection counts derived as make_initial_sample leaves them
+// (projection.rs:179-277).  Returns the local energy (0 when invalid).
+double oracle_measure_idx(void* h, const int* ele_idx, double* o_out, int* valid) {
+  Run& R = *(Run*)h;
+  const Model& m = R.m;
+  const int ns = m.nsite(), ne = m.ne(), n = m.nsize();
+  std::vector<int> cfg(2 * ns, -1), num(2 * ns, 0), cnt(m.n_proj() > 0 ? m.n_proj() : 1, 0);
+  for (int mi = 0; mi < n; ++mi) {
+    const int si = mi / ne;
+    cfg[ele_idx[mi] + si * ns] = mi % ne;
+  }
+  for (int rsi = 0; rsi < 2 * ns; ++rsi) num[rsi] = cfg[rsi] < 0 ? 0 : 1;
+  make_proj_cnt(m, num.data(), cnt.data());
+  double ip = 0.0;
+  int status = 0;
+  std::vector<double> o(m.sr_opt_size(), 0.0);
+  const double e = oracle_measure_config(h, ele_idx, cfg.data(), num.data(), cnt.data(), o.data(), &ip, &status);
+  const bool ok = status == 0 && std::fabs(ip) >= 1.0e-100;
+  if (o_out) std::copy(o.begin(), o.end(), o_out);
+  *valid = ok ? 1 : 0;
+  return ok ? e : 0.0;
+}
+
 }  // extern "C"

@@ -44,6 +44,7 @@ def lib():
         L.oracle_cg_apply.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 5 + \
             [ctypes.c_double, ctypes.c_double]
         L.oracle_measure_config.restype = ctypes.c_double
+        L.oracle_measure_idx.restype = ctypes.c_double
         L.oracle_pfinv_c128.restype = None
         L.oracle_julia_complex.restype = None
         _lib = L
@@ -175,6 +176,15 @@ class Run:
         o = np.zeros(self.size * self.nsample)
         lib().oracle_get_o_store(self.h, ctypes.c_int(walker), ptr(o))
         return o.reshape(self.nsample, self.size)
+
+    def measure_idx(self, ele_idx):
+        """(local energy, O row, valid) of one configuration given as EleIdx,
+        with the run's current parameters."""
+        idx = np.ascontiguousarray(ele_idx, dtype=np.int32)
+        o = np.zeros(self.size)
+        ok = ctypes.c_int(0)
+        e = lib().oracle_measure_idx(self.h, ptr(idx), ptr(o), ctypes.byref(ok))
+        return e, o, ok.value
 
     def walker(self, w):
         idx = np.zeros(self.nsample * self.n, dtype=np.int32)

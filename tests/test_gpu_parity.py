@@ -509,6 +509,54 @@ def test_uneven_lattices_side_by_side(M, tmp_path):
         ens[k].close()
 
 
+def test_measure_caller_configurations(M, tmp_path):
+    """MeasurementStages::measure_sample on caller-supplied configurations
+    (mvmc_ens_measure_configs): the 8x8 benchmark lattice's sampled
+    configurations and random placements, over two saved-sample layers;
+    local energy, O row and validity bit-identical to the oracle measuring
+    the same EleIdx, and the sampled configurations' O rows identical to the
+    ensemble's own measurement; malformed configurations rejected."""
+    import lattice
+    info = lattice.write_workload("square8", str(tmp_path / "sq8"), sr_steps=1, warmup=2, nvmc_sample=2)
+    W = 12
+    ens = M.Ensemble(info["namelist"], info["initial"], walkers=W)
+    log(f"square8 W={W}: sample + measure")
+    ens.sample()
+    ens.measure()
+    own = [ens.o_store(w) for w in range(W)]                      # [nsample][size] per walker
+    sampled = [ens.walker(w)["ele_idx"][s] for w in range(W) for s in range(ens.nsample)]
+    rng = np.random.default_rng(23)
+    ns, ne = ens.nsite, ens.ne
+    rand = [np.concatenate([rng.choice(ns, ne, replace=False), rng.choice(ns, ne, replace=False)])
+            for _ in range(6)]
+    configs = np.array(sampled[:14] + rand, dtype=np.int32)       # 20 > W: two layers
+    e, o, ok = ens.measure_configs(configs)
+    for c in range(14):
+        w, s = divmod(c, ens.nsample)
+        assert np.array_equal(o[c], own[w][s]), c
+    O.set_math(1)
+    try:
+        run = O.Run(info["namelist"], info["initial"], walkers=1)
+        for c, cfg in enumerate(configs):
+            eo, oo, oko = run.measure_idx(cfg)
+            assert ok[c] == oko, c
+            assert e[c] == eo, (c, e[c], eo)
+            assert np.array_equal(o[c], oo), c
+    finally:
+        O.set_math(0)
+    bad = configs[:1].copy()
+    bad[0, 0] = ns
+    with pytest.raises(M.BackendError):
+        ens.measure_configs(bad)
+    bad = configs[:1].copy()
+    bad[0, 1] = bad[0, 0]
+    with pytest.raises(M.BackendError):
+        ens.measure_configs(bad)
+    with pytest.raises(M.InvalidShape):
+        ens.measure_configs(np.zeros((W * ens.nsample + 1, ens.n), dtype=np.int32))
+    ens.close()
+
+
 def test_walkers_beyond_resident_grid_match_single_walker_oracle(M, tmp_path):
     """16x16 benchmark lattice with more than twice as many walkers as the
     sampler's resident grid holds, so walkers wait for slots, are handed to
