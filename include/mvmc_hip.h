@@ -183,6 +183,14 @@ int mvmc_ens_phase_clocks(mvmc_ens* e, long long* clocks32);
  * 3 rebuild failure, 4 non-finite, 5 CG failure). */
 int mvmc_ens_status(mvmc_ens* e);
 
+/* SrStages::cg_local_product (crates/mvmc-core/src/sr_backend.rs:518-598),
+ * one product of the sampled operator on device operands:
+ * z = inv_w O^T (O x) - (mean . x) mean + shift diag * x (sr_cg.rs:952-1004),
+ * O = samples packed [walkers*nsample][na], in the blocked order of the
+ * solver below (row dots, 32-row chunks, 16-chunk superchunks folded left). */
+int mvmc_sr_cg_apply(int na, int walkers, int nsample, const double* pack, const double* mean, const double* diag,
+                     const double* x, double inv_w, double shift, double* z, void* stream);
+
 /* SrStages::cg_step on caller-provided device operands: samples packed
  * [nrows][na] in walker order (nrows = W*nsample), mean/diag/g [na];
  * solves (S + shift diag) x = g with S = O^T O*inv_w - mean mean^T

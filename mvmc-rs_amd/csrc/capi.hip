@@ -1260,6 +1260,21 @@ int mvmc_ens_status(mvmc_ens* e) {
   return s;
 }
 
+int mvmc_sr_cg_apply(int na, int walkers, int nsample, const double* pack, const double* mean, const double* diag,
+                     const double* x, double inv_w, double shift, double* z, void* stream) {
+  static CgWork wk;
+  return guarded([&] {
+    require_gpu();
+    if (na < 1 || walkers < 1 || nsample < 0) throw Fail(MVMC_E_INVALID_SHAPE, "bad CG dimensions");
+    if ((!pack && nsample > 0) || !mean || !diag || !x || !z) throw Fail(MVMC_E_INVALID_SHAPE, "null buffer");
+    hipStream_t st = (hipStream_t)stream;
+    wk.ensure(na, walkers * nsample);
+    HIPCHK(hipMemsetAsync(wk.ctl, 0, sizeof(int) * 4, st));
+    cg_apply_dev(na, walkers * nsample, pack, mean, diag, x, (const double*)nullptr, z, inv_w, shift, wk, st);
+    HIPCHK(hipGetLastError());
+  });
+}
+
 int mvmc_sr_cg_solve(int na, int walkers, int nsample, const double* pack, const double* mean, const double* diag,
                      const double* g, double inv_w, double shift, double tol, int max_it, int variant, double* x,
                      long long* status, void* stream) {

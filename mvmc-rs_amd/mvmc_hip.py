@@ -53,6 +53,8 @@ def lib():
         L.mvmc_sr_cg_solve.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 4 + \
             [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
              ctypes.POINTER(ctypes.c_longlong), ctypes.c_void_p]
+        L.mvmc_sr_cg_apply.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 4 + \
+            [ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -64,7 +66,7 @@ EXPORTED_SYMBOLS = [
     "mvmc_ens_get_o_store", "mvmc_ens_get_walker", "mvmc_ens_get_tables", "mvmc_ens_get_config", "mvmc_ens_get_cg",
     "mvmc_ens_timings", "mvmc_ens_phase_clocks", "mvmc_ens_status", "mvmc_sr_cg_solve", "mvmc_ens_set_precision",
     "mvmc_ens_vjp", "mvmc_ens_log_psi", "mvmc_ens_sr_direct", "mvmc_pfinv_c128",
-    "mvmc_ens_measure_configs",
+    "mvmc_ens_measure_configs", "mvmc_sr_cg_apply",
 ]
 
 
@@ -255,6 +257,15 @@ class Ensemble:
 
     def status(self):
         return int(lib().mvmc_ens_status(self.h))
+
+
+def sr_cg_apply(pack_dev, mean_dev, diag_dev, x_dev, z_dev, na, walkers, nsample, inv_w, shift, stream=None):
+    """SrStages::cg_local_product: z = S x of the sampled operator on device
+    operands (torch CUDA tensors)."""
+    _check(lib().mvmc_sr_cg_apply(na, walkers, nsample, ctypes.c_void_p(pack_dev.data_ptr()),
+                                  ctypes.c_void_p(mean_dev.data_ptr()), ctypes.c_void_p(diag_dev.data_ptr()),
+                                  ctypes.c_void_p(x_dev.data_ptr()), inv_w, shift,
+                                  ctypes.c_void_p(z_dev.data_ptr()), ctypes.c_void_p(stream or 0)))
 
 
 def sr_cg_solve(pack_dev, mean_dev, diag_dev, g_dev, x_dev, na, walkers, nsample, inv_w, shift, tol, max_it,

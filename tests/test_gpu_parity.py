@@ -509,6 +509,28 @@ def test_uneven_lattices_side_by_side(M, tmp_path):
         ens[k].close()
 
 
+@pytest.mark.parametrize("na,rows", [(1, 1), (300, 1100), (2500, 37), (9000, 70), (7784, 600)])
+def test_sr_cg_local_product(M, na, rows):
+    """SrStages::cg_local_product (mvmc_sr_cg_apply): z = S x of the sampled
+    operator bit-identical to the oracle's restatement, for one column, for
+    several 32-row chunks and 16-chunk superchunks, for every columns-per-lane
+    width up to the wide path (na > 8192), and at the benchmark's 7784 active
+    parameters."""
+    import torch
+    rng = np.random.default_rng(na + rows)
+    O_ = rng.standard_normal((rows, na))
+    mean = O_.mean(axis=0)
+    diag = (O_ * O_).mean(axis=0) - mean * mean
+    x = rng.standard_normal(na)
+    inv_w, shift = 1.0 / rows, 1e-3
+    z_o = O.cg_apply(mean, diag, O_, x, inv_w, shift)
+    dev = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+    z = torch.zeros(na, dtype=torch.float64, device="cuda")
+    M.sr_cg_apply(dev(O_), dev(mean), dev(diag), dev(x), z, na, rows, 1, inv_w, shift)
+    torch.cuda.synchronize()
+    assert np.array_equal(z.cpu().numpy(), z_o)
+
+
 def test_measure_caller_configurations(M, tmp_path):
     """MeasurementStages::measure_sample on caller-supplied configurations
     (mvmc_ens_measure_configs): the 8x8 benchmark lattice's sampled
