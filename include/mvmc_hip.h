@@ -187,9 +187,24 @@ int mvmc_ens_status(mvmc_ens* e);
  * one product of the sampled operator on device operands:
  * z = inv_w O^T (O x) - (mean . x) mean + shift diag * x (sr_cg.rs:952-1004),
  * O = samples packed [walkers*nsample][na], in the blocked order of the
- * solver below (row dots, 32-row chunks, 16-chunk superchunks folded left). */
+ * solver below (row dots, 32-row chunks, 16-chunk superchunks folded left).
+ * mean = 0, inv_w = 1, shift = 0 give cg_local_product's bare O^T (O x). */
 int mvmc_sr_cg_apply(int na, int walkers, int nsample, const double* pack, const double* mean, const double* diag,
                      const double* x, double inv_w, double shift, double* z, void* stream);
+
+/* SrStages::gram_real (sr_backend.rs:219-226, C order c_order_gram_real
+ * sr_backend.rs:305-333): out = O O^T, the full n x n column-major Gram of
+ * the [n, samples] column-major store, one DGEMM('N','T') (rocBLAS on the
+ * matrix cores).  Device pointers, asynchronous on `stream`. */
+int mvmc_sr_gram_real(int n, int samples, const double* store, double* out, void* stream);
+
+/* SrStages::cholesky_solve (sr_backend.rs:243-247, crates/mvmc-core/src/sr.rs:535):
+ * solves the symmetric positive-definite S x = rhs in place (rhs becomes x,
+ * S is overwritten by its factor) with rocSOLVER DPOTRF/DPOTRS.  *info: 0 ok,
+ * k > 0 the factorisation's info (leading minor k not positive definite),
+ * -1 non-finite solution (the reference's Err(())).  Device pointers for s and
+ * rhs; synchronises `stream`. */
+int mvmc_sr_cholesky_solve(int n, double* s, double* rhs, int* info, void* stream);
 
 /* SrStages::cg_step on caller-provided device operands: samples packed
  * [nrows][na] in walker order (nrows = W*nsample), mean/diag/g [na];

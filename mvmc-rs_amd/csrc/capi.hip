@@ -1260,6 +1260,63 @@ int mvmc_ens_status(mvmc_ens* e) {
   return s;
 }
 
+// One rocBLAS handle for the standalone SR stages, created on first use.
+rocblas_handle sr_blas() {
+  static std::mutex mu;
+  static rocblas_handle h = nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!h) BLASCHK(rocblas_create_handle(&h));
+  return h;
+}
+
+int mvmc_sr_gram_real(int n, int samples, const double* store, double* out, void* stream) {
+  return guarded([&] {
+    require_gpu();
+    if (n < 0 || samples < 0) throw Fail(MVMC_E_INVALID_SHAPE, "bad Gram dimensions");
+    if (n == 0) return;
+    if (!out || (!store && samples > 0)) throw Fail(MVMC_E_INVALID_SHAPE, "null buffer");
+    rocblas_handle h = sr_blas();
+    BLASCHK(rocblas_set_stream(h, (hipStream_t)stream));
+    const double one = 1.0, zero = 0.0;
+    BLASCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, n, n, samples, &one, store, n,
+                          store, n, &zero, out, n));
+  });
+}
+
+int mvmc_sr_cholesky_solve(int n, double* s, double* rhs, int* info, void* stream) {
+  return guarded([&] {
+    require_gpu();
+    if (n < 0) throw Fail(MVMC_E_INVALID_SHAPE, "bad dimension");
+    if (!info) throw Fail(MVMC_E_INVALID_SHAPE, "null info");
+    *info = 0;
+    if (n == 0) return;
+    if (!s || !rhs) throw Fail(MVMC_E_INVALID_SHAPE, "null buffer");
+    hipStream_t st = (hipStream_t)stream;
+    rocblas_handle h = sr_blas();
+    BLASCHK(rocblas_set_stream(h, st));
+    int* d_info = nullptr;
+    HIPCHK(hipMallocAsync((void**)&d_info, sizeof(int), st));
+    BLASCHK(rocsolver_dpotrf(h, rocblas_fill_upper, n, s, n, d_info));
+    int fi = 0;
+    HIPCHK(hipMemcpyAsync(&fi, d_info, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipFreeAsync(d_info, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (fi != 0) {
+      *info = fi;
+      return;
+    }
+    BLASCHK(rocsolver_dpotrs(h, rocblas_fill_upper, n, 1, s, n, rhs, n));
+    std::vector<double> x(n);
+    HIPCHK(hipMemcpyAsync(x.data(), rhs, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (double v : x)
+      if (!std::isfinite(v)) {
+        *info = -1;
+        return;
+      }
+  });
+}
+
 int mvmc_sr_cg_apply(int na, int walkers, int nsample, const double* pack, const double* mean, const double* diag,
                      const double* x, double inv_w, double shift, double* z, void* stream) {
   static CgWork wk;

@@ -55,6 +55,9 @@ def lib():
              ctypes.POINTER(ctypes.c_longlong), ctypes.c_void_p]
         L.mvmc_sr_cg_apply.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 4 + \
             [ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
+        L.mvmc_sr_gram_real.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.mvmc_sr_cholesky_solve.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.POINTER(ctypes.c_int), ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -66,7 +69,7 @@ EXPORTED_SYMBOLS = [
     "mvmc_ens_get_o_store", "mvmc_ens_get_walker", "mvmc_ens_get_tables", "mvmc_ens_get_config", "mvmc_ens_get_cg",
     "mvmc_ens_timings", "mvmc_ens_phase_clocks", "mvmc_ens_status", "mvmc_sr_cg_solve", "mvmc_ens_set_precision",
     "mvmc_ens_vjp", "mvmc_ens_log_psi", "mvmc_ens_sr_direct", "mvmc_pfinv_c128",
-    "mvmc_ens_measure_configs", "mvmc_sr_cg_apply",
+    "mvmc_ens_measure_configs", "mvmc_sr_cg_apply", "mvmc_sr_gram_real", "mvmc_sr_cholesky_solve",
 ]
 
 
@@ -266,6 +269,22 @@ def sr_cg_apply(pack_dev, mean_dev, diag_dev, x_dev, z_dev, na, walkers, nsample
                                   ctypes.c_void_p(mean_dev.data_ptr()), ctypes.c_void_p(diag_dev.data_ptr()),
                                   ctypes.c_void_p(x_dev.data_ptr()), inv_w, shift,
                                   ctypes.c_void_p(z_dev.data_ptr()), ctypes.c_void_p(stream or 0)))
+
+
+def sr_gram_real(store_dev, out_dev, n, samples, stream=None):
+    """SrStages::gram_real: out = O O^T of the [n, samples] column-major store
+    (torch CUDA tensors)."""
+    _check(lib().mvmc_sr_gram_real(n, samples, ctypes.c_void_p(store_dev.data_ptr()),
+                                   ctypes.c_void_p(out_dev.data_ptr()), ctypes.c_void_p(stream or 0)))
+
+
+def sr_cholesky_solve(s_dev, rhs_dev, n, stream=None):
+    """SrStages::cholesky_solve: rhs becomes the solution of S x = rhs; returns
+    info (0 ok, k > 0 not positive definite, -1 non-finite solution)."""
+    info = ctypes.c_int(0)
+    _check(lib().mvmc_sr_cholesky_solve(n, ctypes.c_void_p(s_dev.data_ptr()), ctypes.c_void_p(rhs_dev.data_ptr()),
+                                        ctypes.byref(info), ctypes.c_void_p(stream or 0)))
+    return info.value
 
 
 def sr_cg_solve(pack_dev, mean_dev, diag_dev, g_dev, x_dev, na, walkers, nsample, inv_w, shift, tol, max_it,

@@ -531,6 +531,42 @@ def test_sr_cg_local_product(M, na, rows):
     assert np.array_equal(z.cpu().numpy(), z_o)
 
 
+def test_sr_gram_and_cholesky_stages(M):
+    """SrStages::gram_real and cholesky_solve (the direct-SR stages): the Gram
+    of a [n, samples] store against exact rational sums to DGEMM rounding
+    (bit-level parity with OpenBLAS's summation order is unpinned), the
+    Cholesky solve against a dense host solve within cond(S) 1e-14; an
+    indefinite matrix reports the factorisation's info."""
+    import torch
+    from fractions import Fraction
+    rng = np.random.default_rng(77)
+    n, samples = 96, 300
+    store = rng.standard_normal((samples, n))          # rows = samples: column-major [n, samples]
+    dev = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+    out = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    M.sr_gram_real(dev(store), out, n, samples)
+    torch.cuda.synchronize()
+    G = out.cpu().numpy().T                            # column-major -> row-major
+    ref = store.T @ store
+    assert np.array_equal(G, G.T)
+    for i, j in [(0, 0), (5, 17), (95, 3), (40, 40)]:
+        exact = float(sum(Fraction(float(a)) * Fraction(float(b)) for a, b in zip(store[:, i], store[:, j])))
+        assert abs(G[i, j] - exact) <= samples * 4 * np.finfo(float).eps * np.abs(store[:, i] * store[:, j]).sum()
+    assert np.abs(G - ref).max() <= 1e-12 * np.abs(ref).max()
+    S = ref / samples + 1e-3 * np.eye(n)
+    b = rng.standard_normal(n)
+    s_dev, rhs = dev(S.T), dev(b)
+    info = M.sr_cholesky_solve(s_dev, rhs, n)
+    torch.cuda.synchronize()
+    assert info == 0
+    x = rhs.cpu().numpy()
+    want = np.linalg.solve(S, b)
+    assert np.abs(x - want).max() <= np.linalg.cond(S) * 1e-14 * np.abs(want).max()
+    bad = -np.eye(4)
+    rhs4 = dev(np.ones(4))
+    assert M.sr_cholesky_solve(dev(bad), rhs4, 4) > 0
+
+
 def test_measure_caller_configurations(M, tmp_path):
     """MeasurementStages::measure_sample on caller-supplied configurations
     (mvmc_ens_measure_configs): the 8x8 benchmark lattice's sampled
