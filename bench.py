@@ -45,16 +45,16 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def measured_traffic(workload, walkers):
-    """HBM bytes per k_block launch from the two PMC passes (FETCH_SIZE, WRITE_SIZE,
-    each scaled by a 1 GiB copy) recorded under profiles/ for this workload; None
-    when no such record exists for this configuration."""
+def measured_traffic(workload, walkers, steps, warmup):
+    """HBM bytes per k_block launch of the last timed step from the two PMC passes
+    (FETCH_SIZE, WRITE_SIZE, each scaled by a 1 GiB copy) recorded under profiles/
+    for this workload and schedule; None when no record matches this run."""
     path = os.path.join(ROOT, "profiles", f"k_block_{workload}_traffic.json")
     try:
         rec = json.load(open(path))
     except (OSError, ValueError):
         return None
-    if rec.get("walkers") != walkers:
+    if (rec.get("walkers"), rec.get("steps"), rec.get("warmup")) != (walkers, steps, warmup):
         return None
     return rec.get("traffic_bytes_per_launch")
 
@@ -262,7 +262,7 @@ def main():
         "acceptance": accs / max(props, 1),
         "energy_per_site": [r[0] / info["nsite"] for r in rows],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload, walkers),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload, walkers, args.steps, args.warmup),
                      "kernel": "k_block", "avg_launch_ms": avg_launch_s * 1e3, "launches": launches,
                      "algorithmic_bytes_per_launch": bytes_total / max(launches, 1)},
     }
