@@ -615,24 +615,25 @@ def test_measure_caller_configurations(M, tmp_path):
     ens.close()
 
 
-def test_walkers_beyond_resident_grid_match_single_walker_oracle(M, tmp_path):
+@pytest.mark.parametrize("W,warmup", [(1100, 2), (16384, 0)])
+def test_walkers_beyond_resident_grid_match_single_walker_oracle(M, tmp_path, W, warmup):
     """16x16 benchmark lattice with more than twice as many walkers as the
-    sampler's resident grid holds, so walkers wait for slots, are handed to
-    the rebuild kernel and resume in later launches: after the first
-    sampling phase (thermalisation + one sample, no SR yet) every checked
-    walker, from each scheduling round, has the configuration, SFMT state and
-    counters of a one-walker oracle run seeded RndSeed + w (walkers are
-    independent chains until the first SR update, multichain.rs:88)."""
+    sampler's resident grid holds (and at the benchmark's full 16384
+    walkers), so walkers wait for slots, are handed to the rebuild kernel and
+    resume in later launches: after the first sampling phase (thermalisation
+    + one sample, no SR yet) every checked walker, from each scheduling
+    round, has the configuration, SFMT state and counters of a one-walker
+    oracle run seeded RndSeed + w (walkers are independent chains until the
+    first SR update, multichain.rs:88)."""
     import lattice
-    info = lattice.write_workload("square16", str(tmp_path / "sq16"), sr_steps=1, warmup=2)
-    W = 1100
+    info = lattice.write_workload("square16", str(tmp_path / "sq16"), sr_steps=1, warmup=warmup)
     ens = M.Ensemble(info["namelist"], info["initial"], walkers=W)
     assert W > 2 * ens.grid, ens.grid
-    log(f"square16 W={W} grid={ens.grid}: sampling")
+    log(f"square16 W={W} grid={ens.grid} NVMCWarmUp={warmup}: sampling")
     ens.sample()
     O.set_math(1)
     try:
-        for w in (0, ens.grid - 1, ens.grid, 2 * ens.grid + 1, W - 1):
+        for w in sorted({0, ens.grid - 1, ens.grid, 2 * ens.grid + 1, W // 2 + 3, W - 1}):
             log(f"  walker {w} vs one-walker oracle")
             run = O.Run(info["namelist"], info["initial"], walkers=1, seed=11272 + w)
             run.sample()
