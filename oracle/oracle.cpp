@@ -412,12 +412,57 @@ static bool metropolis(Walker& W, double lpd, double lip_new, double lip_old) {
   return w > u;
 }
 
+// makeCandidate_hopping (candidate.rs:137-227; C vmcmake.c): draw-until-
+// itinerant source electron (cap Nsite^2 + 1 draws, then reject if no
+// itinerant electron exists), then draw-until-empty destination with the C
+// order "draw, test the cap, test occupancy".  Returns true on rejection; the
+// last drawn mi, s, ri, rj are reported either way.  RNG: gen32 / mod / real2.
+template <class RNG>
+static bool hop_candidate(int ns, int ne, const int* loc_spn, const int* ele_idx, const int* ele_cfg, RNG& rng,
+                          int& mi, int& s, int& ri, int& rj) {
+  auto is_local = [&](int r) { return r < 0 || r >= ns ? true : loc_spn[r] == 1; };
+  const long long icnt_max = (long long)ns * ns;
+  long long icnt_e = 0;
+  mi = s = ri = rj = 0;
+  for (;;) {
+    if (icnt_e == icnt_max + 1) {
+      bool any = false;
+      for (int k = 0; k < 2 * ne; ++k) any = any || !is_local(ele_idx[k] < 0 ? 0 : ele_idx[k]);
+      if (!any) return true;
+    }
+    ++icnt_e;
+    mi = (int)rng.mod((uint32_t)ne);
+    s = rng.real2() < 0.5 ? 0 : 1;
+    ri = ele_idx[mi + s * ne];
+    if (!is_local(ri)) break;
+  }
+  long long icnt = 0;
+  for (;;) {
+    rj = (int)rng.mod((uint32_t)ns);
+    if (icnt > icnt_max) return true;
+    ++icnt;
+    if (ele_cfg[rj + s * ns] == -1 && !is_local(rj)) break;
+  }
+  return false;
+}
+
+// Raw-word tape with SFMT's draw conversions (mod: word % n, real2: word 2^-32).
+struct WordTape {
+  const uint32_t* w;
+  int n, pos = 0;
+  uint32_t gen32() {
+    if (pos >= n) throw std::runtime_error("word tape exhausted");
+    return w[pos++];
+  }
+  double real2() { return (double)gen32() * (1.0 / 4294967296.0); }
+  uint32_t mod(uint32_t k) { return gen32() % k; }
+};
+
 static void run_block(const Run& R, Walker& W, int n_in, double& log_ip_old, int& n_accept) {
   const Model& m = R.m;
   const int ns = m.nsite(), ne = m.ne(), nqp = m.nqp(), np = m.n_proj();
   vector<double> pf_new(nqp);
   vector<int> cnew(np), mid(np);
-  auto is_local = [&](int r) { return r < 0 || r >= ns ? true : m.loc_spn[r] == 1; };
   for (int step = 0; step < n_in; ++step) {
     int type;  // 0 hop, 1 exchange, 3 none
     switch (m.mp.nex_update_path) {
@@ -428,31 +473,8 @@ static void run_block(const Run& R, Walker& W, int n_in, double& log_ip_old, int
     }
     if (type == 0) {
       W.counter[0]++;
-      const long long icnt_max = (long long)ns * ns;
       int mi, s, ri, rj;
-      bool reject = false;
-      long long icnt_e = 0;
-      for (;;) {
-        if (icnt_e == icnt_max + 1) {
-          bool any = false;
-          for (int k = 0; k < 2 * ne; ++k) any = any || !is_local(W.ele_idx[k] < 0 ? 0 : W.ele_idx[k]);
-          if (!any) { reject = true; break; }
-        }
-        ++icnt_e;
-        mi = (int)W.rng.mod((uint32_t)ne);
-        s = W.rng.real2() < 0.5 ? 0 : 1;
-        ri = W.ele_idx[mi + s * ne];
-        if (!is_local(ri)) break;
-      }
-      if (reject) continue;
-      long long icnt = 0;
-      for (;;) {
-        rj = (int)W.rng.mod((uint32_t)ns);
-        if (icnt > icnt_max) { reject = true; break; }
-        ++icnt;
-        if (W.ele_cfg[rj + s * ns] == -1 && !is_local(rj)) break;
-      }
-      if (reject) continue;
+      if (hop_candidate(ns, ne, m.loc_spn.data(), W.ele_idx.data(), W.ele_cfg.data(), W.rng, mi, s, ri, rj)) continue;
       update_ele_config(m, mi, ri, rj, s, W.ele_idx.data(), W.ele_cfg.data(), W.ele_num.data());
       std::copy(W.proj_cnt.begin(), W.proj_cnt.end(), cnew.begin());
       update_proj_cnt(m, ri, rj, cnew.data(), W.ele_num.data());
@@ -1213,6 +1235,26 @@ void oracle_julia_complex(int which, const double* z, const double* w, double* o
     const zc r = which == 0 ? julia_divide(zc{z[2 * i], z[2 * i + 1]}, wi) : julia_reciprocal(wi);
     out[2 * i] = r.re;
     out[2 * i + 1] = r.im;
+  }
+}
+
+// Hopping candidate on a raw-word tape (the C retry-cap fixture of
+// tests/fixtures/candidate): out8 = mi ri rj s t reject candidate_words
+// words_after_metropolis (the Metropolis decision draws one more word when
+// the candidate is not rejected; the tapes hold only the candidate's words).
+// Returns -1 if the tape ran out.
+int oracle_hop_candidate_tape(int ns, int ne, const int* loc_spn, const int* ele_idx, const int* ele_cfg,
+                              const uint32_t* words, int nwords, int* out8) {
+  WordTape tape{words, nwords};
+  try {
+    int mi, s, ri, rj;
+    const bool rej = hop_candidate(ns, ne, loc_spn, ele_idx, ele_cfg, tape, mi, s, ri, rj);
+    const int used = tape.pos;
+    const int vals[8] = {mi, ri, rj, s, s, rej ? 1 : 0, used, used + (rej ? 0 : 1)};
+    std::memcpy(out8, vals, sizeof(vals));
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
   }
 }
 

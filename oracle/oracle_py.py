@@ -99,6 +99,18 @@ def julia_complex(which, z, w):
     return out
 
 
+def hop_candidate_tape(nsite, nelec, loc_spn, ele_idx, ele_cfg, words):
+    """Hopping candidate on a raw-word tape: [mi, ri, rj, s, t, reject,
+    candidate_words, words_after_metropolis], or None if the tape ran out."""
+    arr = lambda a, t: np.ascontiguousarray(a, dtype=t)
+    ls, ei, ec = arr(loc_spn, np.int32), arr(ele_idx, np.int32), arr(ele_cfg, np.int32)
+    w = arr(words, np.uint32)
+    out = np.zeros(8, dtype=np.int32)
+    rc = lib().oracle_hop_candidate_tape(ctypes.c_int(nsite), ctypes.c_int(nelec), ptr(ls), ptr(ei), ptr(ec), ptr(w),
+                                         ctypes.c_int(w.size), ptr(out))
+    return None if rc != 0 else [int(x) for x in out]
+
+
 def pfaffian(plane_colmajor, n):
     a = np.ascontiguousarray(plane_colmajor, dtype=np.float64)
     pf = ctypes.c_double(0.0)

@@ -15,6 +15,7 @@ Everything written here is data held by the reference's tests:
   * PfaPack input matrices           tests/fixtures/pfapack/real_n*.orig.txt
                                      tests/fixtures/pfapack/complex_n*.orig.txt
   * two-hop bilinear bits            tests/fixtures/pfaffian_cg/two_hop_bilinear{,_c}.txt
+  * hopping candidate retry tapes    tests/fixtures/candidate/hopping_retry_cap_559.txt
 """
 import json
 import os
@@ -167,6 +168,13 @@ def pfapack_complex():
     np.savez_compressed(os.path.join(OUT, "pfapack_complex_inputs.npz"), **arrays)
 
 
+def candidate():
+    """C retry-cap tapes of the hopping candidate (data file, copied as is)."""
+    shutil.copy(os.path.join(FIX, "candidate", "hopping_retry_cap_559.txt"),
+                os.path.join(OUT, "candidate_hopping_retry_cap.txt"))
+    os.chmod(os.path.join(OUT, "candidate_hopping_retry_cap.txt"), 0o644)
+
+
 def bilinear():
     lines = [l for l in open(os.path.join(FIX, "pfaffian_cg/two_hop_bilinear.txt")) if l.strip() and not l.startswith("#")]
     expected = [l.split() for l in open(os.path.join(FIX, "pfaffian_cg/two_hop_bilinear_c.txt")) if not l.startswith("#")]
@@ -190,5 +198,6 @@ if __name__ == "__main__":
     sr_cg()
     pfapack()
     pfapack_complex()
+    candidate()
     bilinear()
     print("fixtures written to", OUT)

@@ -114,3 +114,35 @@ def test_sr_cg_iterates_match_c(name, diag_scale):
             b = 8.0 * (n + samples) * limit * eps
             for got, want in ((x, x_ref), (r, r_ref), (d, d_ref)):
                 assert close(got, want, b, b), limit
+
+
+def _parse_candidate_cases(path):
+    cases, cur = [], None
+    for line in open(path):
+        parts = line.split()
+        if not parts:
+            continue
+        if parts[0] == "case":
+            cur = {"name": parts[1], "kind": parts[2]}
+        elif parts[0] == "end":
+            cases.append(cur)
+            cur = None
+        elif cur is not None:
+            cur[parts[0]] = [int(x) for x in parts[1:]]
+    return cases
+
+
+def test_hopping_candidate_retry_cap_matches_c_tapes():
+    """makeCandidate_hopping on the C retry-cap tapes
+    (tests/fixtures/candidate/hopping_retry_cap_559.txt, C vmcmake.c verbatim on
+    raw words): source and destination draws, the Nsite^2 cap tested after each
+    destination draw, the source loop's itinerant-electron check, and the words
+    consumed before and after the Metropolis draw.  The FSZ / local-spin-flip
+    cases of the same file belong to the FSZ sampler, which is not built."""
+    cases = _parse_candidate_cases(os.path.join(GOLDEN, "candidate_hopping_retry_cap.txt"))
+    hop = [c for c in cases if c["kind"] == "hopping"]
+    assert len(hop) >= 7
+    for c in hop:
+        ns, ne = c["nsite"][0], c["nelec"][0]
+        got = O.hop_candidate_tape(ns, ne, c["loc_spn"], c["ele_idx"], c["ele_cfg"], c["words"])
+        assert got == c["result"], (c["name"], got, c["result"])
