@@ -404,12 +404,20 @@ static void make_initial_sample_validated(const Run& R, Walker& W) {
 
 static double log_ip_of(const Model& m, const double* pf) { return o_log(std::fabs(calc_ip(m, pf))); }
 
-static bool metropolis(Walker& W, double lpd, double lip_new, double lip_old) {
+// metropolis_weight / metropolis_decision (metropolis.rs:42-83): the weight
+// w = exp(2((dproj + 0) + lip_new - lip_old)), a non-finite weight becomes -1,
+// accept iff w > real2 (one word).
+template <class RNG>
+static bool metropolis_rng(RNG& rng, double lpd, double lip_new, double lip_old, double* w_out = nullptr) {
   const double ex = 2.0 * (((lpd + 0.0) + lip_new) - lip_old);
   double w = o_exp(ex);
   if (!std::isfinite(w)) w = -1.0;
-  const double u = W.rng.real2();
+  if (w_out) *w_out = w;
+  const double u = rng.real2();
   return w > u;
+}
+static bool metropolis(Walker& W, double lpd, double lip_new, double lip_old) {
+  return metropolis_rng(W.rng, lpd, lip_new, lip_old);
 }
 
 // makeCandidate_hopping (candidate.rs:137-227; C vmcmake.c): draw-until-
@@ -1256,6 +1264,12 @@ int oracle_hop_candidate_tape(int ns, int ne, const int* loc_spn, const int* ele
   } catch (const std::exception&) {
     return -1;
   }
+}
+
+// Metropolis decision with the uniform draw given as a raw word.
+int oracle_metropolis_word(double lpd, double lip_new, double lip_old, uint32_t word, double* w_out) {
+  WordTape tape{&word, 1};
+  return metropolis_rng(tape, lpd, lip_new, lip_old, w_out) ? 1 : 0;
 }
 
 // LTL only (for Pfaffian bit checks): returns info, writes Pf.

@@ -46,6 +46,8 @@ def lib():
         L.oracle_measure_config.restype = ctypes.c_double
         L.oracle_measure_idx.restype = ctypes.c_double
         L.oracle_pfinv_c128.restype = None
+        L.oracle_metropolis_word.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint32,
+                                             ctypes.POINTER(ctypes.c_double)]
         L.oracle_julia_complex.restype = None
         _lib = L
     return _lib
@@ -109,6 +111,13 @@ def hop_candidate_tape(nsite, nelec, loc_spn, ele_idx, ele_cfg, words):
     rc = lib().oracle_hop_candidate_tape(ctypes.c_int(nsite), ctypes.c_int(nelec), ptr(ls), ptr(ei), ptr(ec), ptr(w),
                                          ctypes.c_int(w.size), ptr(out))
     return None if rc != 0 else [int(x) for x in out]
+
+
+def metropolis_word(lpd, lip_new, lip_old, word):
+    """(accept, weight) of the Metropolis decision with uniform draw word * 2^-32."""
+    w = ctypes.c_double(0.0)
+    acc = lib().oracle_metropolis_word(lpd, lip_new, lip_old, word, ctypes.byref(w))
+    return bool(acc), w.value
 
 
 def pfaffian(plane_colmajor, n):

@@ -179,3 +179,22 @@ def test_correctly_rounded_math_agrees_with_libm_nearly_always():
     ys = np.exp(rng.uniform(-200, 200, 200000))
     mism_l = sum(1 for y in ys if L.oracle_cr_log(float(y)) != math.log(float(y)))
     assert mism_l / ys.size < 1e-4
+
+
+def test_metropolis_closed_forms():
+    """metropolis_weight / metropolis_decision (metropolis.rs:42-83,92-128):
+    w = exp(2((dproj + 0) + lip_new - lip_old)); a non-finite weight becomes
+    -1; accept iff w > real2 with real2(word) = word 2^-32, so
+    real2(u32::MAX) = 1 - 2^-32 still accepts w = 1."""
+    acc, w = O.metropolis_word(0.1, 0.2, 0.05, 0)
+    assert w == math.exp(2.0 * (((0.1 + 0.0) + 0.2) - 0.05)) and acc
+    acc, w = O.metropolis_word(0.0, 0.3, 0.3, 0xFFFFFFFF)
+    assert w == 1.0 and acc                      # 1 > 1 - 2^-32
+    acc, w = O.metropolis_word(0.5 * math.log(0.5), 0.0, 0.0, 1 << 31)
+    assert w == math.exp(math.log(0.5)) and acc == (w > 0.5)
+    acc, w = O.metropolis_word(1000.0, 0.0, 0.0, 0)
+    assert w == -1.0 and not acc                 # overflow: weight -1, rejected even at u = 0
+    acc, w = O.metropolis_word(0.0, float("nan"), 0.0, 0)
+    assert w == -1.0 and not acc
+    acc, w = O.metropolis_word(0.0, -1000.0, 0.0, 0)
+    assert w == 0.0 and not acc                  # underflow to 0 is finite: 0 > 0 fails
