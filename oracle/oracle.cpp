@@ -1266,6 +1266,25 @@ int oracle_hop_candidate_tape(int ns, int ne, const int* loc_spn, const int* ele
   }
 }
 
+// Overlap of one configuration from scratch: ip = sum_q w_q Pf_q (full
+// calc_m_all, no ratio formula) and its projection counts.  Returns the
+// calc_m_all status.
+int oracle_overlap_idx(void* h, const int* ele_idx, double* ip_out, int* cnt_out) {
+  Run& R = *(Run*)h;
+  refresh_tables(R);
+  const Model& m = R.m;
+  const int ns = m.nsite(), ne = m.ne(), n = m.nsize();
+  std::vector<int> cfg(2 * ns, -1), num(2 * ns, 0), cnt(m.n_proj() > 0 ? m.n_proj() : 1, 0);
+  for (int mi = 0; mi < n; ++mi) cfg[ele_idx[mi] + (mi / ne) * ns] = mi % ne;
+  for (int rsi = 0; rsi < 2 * ns; ++rsi) num[rsi] = cfg[rsi] < 0 ? 0 : 1;
+  make_proj_cnt(m, num.data(), cnt.data());
+  std::vector<double> inv((size_t)m.nqp() * n * n), pf(m.nqp());
+  const int st = calc_m_all(R, ele_idx, inv.data(), pf.data(), false);
+  *ip_out = st ? 0.0 : calc_ip(m, pf.data());
+  for (int i = 0; i < m.n_proj(); ++i) cnt_out[i] = cnt[i];
+  return st;
+}
+
 // Metropolis decision with the uniform draw given as a raw word.
 int oracle_metropolis_word(double lpd, double lip_new, double lip_old, uint32_t word, double* w_out) {
   WordTape tape{&word, 1};

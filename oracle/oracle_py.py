@@ -207,6 +207,15 @@ class Run:
         e = lib().oracle_measure_idx(self.h, ptr(idx), ptr(o), ctypes.byref(ok))
         return e, o, ok.value
 
+    def overlap_idx(self, ele_idx):
+        """(status, ip = sum_q w_q Pf_q from full Pfaffians, projection counts) of
+        one configuration with the run's current parameters."""
+        idx = np.ascontiguousarray(ele_idx, dtype=np.int32)
+        ip = ctypes.c_double(0.0)
+        cnt = np.zeros(max(self.nproj, 1), dtype=np.int32)
+        st = lib().oracle_overlap_idx(self.h, ptr(idx), ctypes.byref(ip), ptr(cnt))
+        return st, ip.value, cnt[:self.nproj].copy()
+
     def walker(self, w):
         idx = np.zeros(self.nsample * self.n, dtype=np.int32)
         cnt = np.zeros(10, dtype=np.int32)

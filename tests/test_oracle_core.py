@@ -198,3 +198,65 @@ def test_metropolis_closed_forms():
     assert w == -1.0 and not acc
     acc, w = O.metropolis_word(0.0, -1000.0, 0.0, 0)
     assert w == 0.0 and not acc                  # underflow to 0 is finite: 0 > 0 fails
+
+
+def _read_def_rows(path, ncols):
+    rows = []
+    for line in open(path):
+        parts = line.split()
+        if len(parts) != ncols:
+            continue
+        try:
+            rows.append([float(x) for x in parts])
+        except ValueError:
+            continue
+    return rows
+
+
+def test_local_energy_matches_brute_force_overlaps():
+    """Local energy of the 6-site Hubbard chain (U = 4, 24 transfer terms, 8 QP
+    planes) against brute force: E_L(x) = U sum n_up n_dn - sum_k t_k
+    psi(x'_k) / psi(x), every psi from full Pfaffians of the moved
+    configuration (the electron keeps its label) times exp(sum_p v_p n_p(x)),
+    with the Green-ratio formula of the oracle nowhere in the check
+    (observables.rs:483-552,1638-1755)."""
+    inputs = os.path.join(GOLDEN, "inputs", "hubbard_chain")
+    run = O.Run(os.path.join(inputs, "namelist.def"), walkers=1)
+    proj, _ = run.params()
+    trans = [(int(a), int(b), int(c), int(d), t) for a, b, c, d, t, _ in
+             _read_def_rows(os.path.join(inputs, "trans.def"), 6)]
+    coul = {int(r): u for r, u in _read_def_rows(os.path.join(inputs, "coulombintra.def"), 2)}
+    assert len(trans) == 24 and len(coul) == 6
+    ns, ne = run.nsite, run.ne
+    rng = np.random.default_rng(8)
+
+    def psi(idx):
+        st, ip, cnt = run.overlap_idx(idx)
+        assert st == 0
+        return ip * math.exp(float(np.dot(proj, cnt)))
+
+    checked = 0
+    for _ in range(25):
+        idx = np.concatenate([rng.choice(ns, ne, replace=False), rng.choice(ns, ne, replace=False)]).astype(np.int32)
+        e_fast, _, ok = run.measure_idx(idx)
+        if not ok:
+            continue
+        occ = np.zeros((2, ns), dtype=int)
+        for mi, r in enumerate(idx):
+            occ[mi // ne, r] = 1
+        e = sum(u * occ[0, r] * occ[1, r] for r, u in coul.items())
+        p0 = psi(idx)
+        for ri, s1, rj, s2, t in trans:
+            if s1 == s2 and ri == rj:
+                g = occ[s2, rj]
+            elif s1 != s2 or occ[s1, ri] == 1 or occ[s2, rj] == 0:
+                g = 0.0
+            else:
+                moved = idx.copy()
+                mi = [k for k in range(s2 * ne, (s2 + 1) * ne) if idx[k] == rj][0]
+                moved[mi] = ri
+                g = psi(moved) / p0
+            e -= t * g
+        assert abs(e_fast - e) <= 1e-11 * max(1.0, abs(e)), (idx, e_fast, e)
+        checked += 1
+    assert checked >= 20
