@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Write the committed golden fixtures of tests/golden from the reference
-checkout's own fixture data (run once, in a container where the reference is
-mounted; the GPU box never runs this).
+checkout's own fixture data (run once where the reference checkout is
+mounted; the tests only read the files written here).
 
 Everything written here is data held by the reference's tests:
   * SFMT words of the C library      crates/sfmt19937/tests/golden_vs_c.rs:40-160
