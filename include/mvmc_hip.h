@@ -131,6 +131,16 @@ int mvmc_ens_sr_direct(mvmc_ens* e, int* skipped);
  * (Etot, Im Etot, Etot2, (Etot2-Etot^2)/Etot^2, Sztot, Sztot2). */
 int mvmc_ens_opt_step(mvmc_ens* e, double* energy_row6);
 
+/* Run-loop parameter output: output_opt_data (crates/mvmc-core/src/io.rs:561-727).
+ * mvmc_ens_opt_step records (E, E^2, parameters after the SR update) for the
+ * steps [NSROptItrStep - NSROptItrSmp, NSROptItrStep) of the run
+ * (run.rs:1258-1262,1457-1460); this writes {CParaFileHead}_opt.dat into
+ * out_dir (NULL or "" = current directory) in C's "% .18e": one snapshot as
+ * (value, 0) pairs, more as (mean, 0, sample standard deviation) triples, plus
+ * the {head}_{gutzwiller,jastrow,orbital}_opt.dat blocks of the means.
+ * MVMC_E_INPUT when the window is incomplete or a file cannot be written. */
+int mvmc_ens_write_opt_data(mvmc_ens* e, const char* out_dir);
+
 /* Results for parity checks (host arrays). */
 int mvmc_ens_get_energy(mvmc_ens* e, double* wc_etot_etot2_sz_sz2);
 int mvmc_ens_get_operands(mvmc_ens* e, double* oo_2size, double* ho_size);

@@ -271,6 +271,13 @@ T* dupload(std::vector<void*>& list, const std::vector<T>& h) {
 }  // namespace
 
 struct mvmc_ens {
+  // optimisation window of the last NSROptItrSmp steps (store_opt_data,
+  // crates/mvmc-core/src/io.rs:21-48): energy, energy^2 and the parameters
+  // after the step's SR update
+  int opt_steps = 0;
+  std::vector<std::vector<double>> opt_win;
+  std::vector<double> opt_win_e, opt_win_e2;
+  std::vector<char> opt_win_set;
   mvmc::Model m;
   EnsView v{};
   std::vector<void*> allocs;
@@ -1122,6 +1129,99 @@ int mvmc_ens_opt_step(mvmc_ens* e, double* row) {
     long long info = 0;
     int rc = sr_impl(e, &info);
     if (rc != 0) throw Fail(MVMC_E_NUMERIC, "SR step skipped (non-finite update)");
+    // store_opt_data for the window [NSROptItrStep - NSROptItrSmp, NSROptItrStep)
+    // (crates/mvmc-core/src/run.rs:1258-1262,1457-1460)
+    const mvmc::Model& m = e->m;
+    const int step = e->opt_steps++;
+    const int len = m.mp.nsr_opt_itr_smp, start = std::max(m.mp.nsr_opt_itr_step, 0) - len;
+    if (len > 0 && step >= start && step - start < len) {
+      const size_t k = (size_t)(step - start);
+      if (e->opt_win.size() < (size_t)len) {
+        e->opt_win.resize(len);
+        e->opt_win_e.resize(len);
+        e->opt_win_e2.resize(len);
+        e->opt_win_set.assign(len, 0);
+      }
+      std::vector<double> p(m.proj);
+      p.insert(p.end(), m.slater.begin(), m.slater.end());
+      e->opt_win[k] = p;
+      e->opt_win_e[k] = e->etot;
+      e->opt_win_e2[k] = e->etot2;
+      e->opt_win_set[k] = 1;
+    }
+  });
+}
+
+// output_opt_data (crates/mvmc-core/src/io.rs:561-727): {head}_opt.dat from the
+// stored window in C's "% .18e" -- one snapshot: (value, 0) pairs; more:
+// (mean.re, mean.im, sample standard deviation) triples, mean summed in
+// window order; then {head}_{gutzwiller,jastrow,orbital}_opt.dat blocks of
+// the means.
+int mvmc_ens_write_opt_data(mvmc_ens* e, const char* out_dir) {
+  return guarded([&] {
+    const mvmc::Model& m = e->m;
+    const size_t len = e->opt_win.size();
+    bool complete = len > 0;
+    for (char c : e->opt_win_set) complete = complete && c;
+    if (!complete) throw Fail(MVMC_E_INPUT, "incomplete optimization window");
+    const std::string dir = out_dir && *out_dir ? std::string(out_dir) + "/" : std::string();
+    const std::string head = m.mp.c_para_file_head.empty() ? "zqp" : m.mp.c_para_file_head;
+    const size_t width = e->opt_win[0].size();
+    auto value = [&](size_t w, size_t idx) {
+      return idx == 0 ? e->opt_win_e[w] : idx == 1 ? e->opt_win_e2[w] : e->opt_win[w][idx - 2];
+    };
+    auto cfmt = [](double x) {
+      char b[64];
+      std::snprintf(b, sizeof b, "% .18e", x);
+      return std::string(b);
+    };
+    const std::string path = dir + head + "_opt.dat";
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) throw Fail(MVMC_E_INPUT, "cannot write " + path);
+    std::vector<double> mean(width + 2);
+    if (len == 1) {
+      for (size_t idx = 0; idx < width + 2; ++idx)
+        std::fprintf(f, "%s %s ", cfmt(value(0, idx)).c_str(), cfmt(0.0).c_str());
+    } else {
+      for (size_t idx = 0; idx < width + 2; ++idx) {
+        double s = 0.0;
+        for (size_t w = 0; w < len; ++w) s += value(w, idx);
+        const double mu = s / (double)len;
+        double var = 0.0;
+        for (size_t w = 0; w < len; ++w) {
+          const double d = value(w, idx) - mu;
+          var += d * d;
+        }
+        mean[idx] = mu;
+        std::fprintf(f, "%s %s %s ", cfmt(mu).c_str(), cfmt(0.0).c_str(),
+                     cfmt(std::sqrt(var / ((double)len - 1.0))).c_str());
+      }
+    }
+    std::fprintf(f, "\n");
+    std::fclose(f);
+    if (len == 1) return;
+    struct Block {
+      const char* suffix;
+      const char* label;
+      size_t count;
+    };
+    const Block blocks[3] = {{"gutzwiller", "NGutzwillerIdx", (size_t)m.n_gutz},
+                             {"jastrow", "NJastrowIdx", (size_t)m.n_jast},
+                             {"orbital", "NOrbitalIdx", (size_t)m.n_slater}};
+    size_t off = 2;
+    for (const Block& b : blocks) {
+      if (b.count > 0) {
+        const std::string bp = dir + head + "_" + b.suffix + "_opt.dat";
+        FILE* g = std::fopen(bp.c_str(), "w");
+        if (!g) throw Fail(MVMC_E_INPUT, "cannot write " + bp);
+        const char* sep = "======================";
+        std::fprintf(g, "%s\n%s  %zu\n%s\n%s\n%s\n", sep, b.label, b.count, sep, sep, sep);
+        for (size_t i = 0; i < b.count; ++i)
+          std::fprintf(g, "%zu %s %s \n", i, cfmt(mean[off + i]).c_str(), cfmt(0.0).c_str());
+        std::fclose(g);
+      }
+      off += b.count;
+    }
   });
 }
 

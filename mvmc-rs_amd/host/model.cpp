@@ -94,6 +94,10 @@ double to_d(const std::string& s, const std::string& path) {
 void parse_modpara(const std::string& path, ModPara& p) {
   auto lines = read_lines(path);
   // The first eight lines are a fixed banner (file heads); keywords follow.
+  for (size_t i = 0; i < 8 && i < lines.size(); ++i) {
+    auto t = tokens(lines[i]);
+    if (t.size() >= 2 && lower(t[0]) == "cparafilehead") p.c_para_file_head = t[1];
+  }
   for (size_t i = 8; i < lines.size(); ++i) {
     const std::string& l = lines[i];
     if (l.empty() || l[0] == '-') continue;

@@ -46,6 +46,7 @@ struct ModPara {
   int nmp_trans = 0;
   int nsr_opt_itr_step = 1000;
   int nsr_opt_itr_smp = 100;
+  std::string c_para_file_head = "zqp";  // CParaFileHead of the banner (parameter output files)
   double dsr_opt_red_cut = 0.001;
   double dsr_opt_sta_del = 0.02;
   double dsr_opt_step_dt = 0.02;

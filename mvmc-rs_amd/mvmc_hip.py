@@ -55,6 +55,7 @@ def lib():
              ctypes.POINTER(ctypes.c_longlong), ctypes.c_void_p]
         L.mvmc_sr_cg_apply.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 4 + \
             [ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
+        L.mvmc_ens_write_opt_data.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
         L.mvmc_sr_gram_real.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.mvmc_sr_cholesky_solve.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                              ctypes.POINTER(ctypes.c_int), ctypes.c_void_p]
@@ -70,6 +71,7 @@ EXPORTED_SYMBOLS = [
     "mvmc_ens_timings", "mvmc_ens_phase_clocks", "mvmc_ens_status", "mvmc_sr_cg_solve", "mvmc_ens_set_precision",
     "mvmc_ens_vjp", "mvmc_ens_log_psi", "mvmc_ens_sr_direct", "mvmc_pfinv_c128",
     "mvmc_ens_measure_configs", "mvmc_sr_cg_apply", "mvmc_sr_gram_real", "mvmc_sr_cholesky_solve",
+    "mvmc_ens_write_opt_data",
 ]
 
 
@@ -137,6 +139,10 @@ class Ensemble:
 
     def measure(self):
         _check(lib().mvmc_ens_measure(self.h))
+
+    def write_opt_data(self, out_dir):
+        """Write {CParaFileHead}_opt.dat (+ block files) of the recorded window."""
+        _check(lib().mvmc_ens_write_opt_data(self.h, str(out_dir).encode()))
 
     def measure_configs(self, ele_idx):
         """Local energy, O row and validity of each configuration (rows of

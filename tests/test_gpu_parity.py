@@ -5,6 +5,7 @@ order, -ffp-contract=off); floating-point operands bit-exact against the
 oracle run with the same correctly rounded exp/log, and within the
 reference's own tolerances (1e-14 + 1e-13 |b|) against the C records."""
 import json
+import math
 import os
 import struct
 import sys
@@ -565,6 +566,84 @@ def test_sr_gram_and_cholesky_stages(M):
     bad = -np.eye(4)
     rhs4 = dev(np.ones(4))
     assert M.sr_cholesky_solve(dev(bad), rhs4, 4) > 0
+
+
+@pytest.mark.parametrize("window", [2, 1])
+def test_opt_data_output(M, tmp_path, window):
+    """zqp_opt.dat after a 4-step run with a window of the last `window`
+    steps (output_opt_data, io.rs:561-727): the file text equals the
+    reference rules applied to the recorded energies and post-update
+    parameters (C "% .18e"; pairs for one snapshot, mean / 0 / standard
+    deviation triples and the Gutzwiller, Jastrow and orbital block files for
+    more), and reading it back as an initial.def gives the window means to
+    the rounding of the gauge fixing applied on creation."""
+    import shutil
+    src = os.path.join(GOLDEN, "inputs", "hubbard_chain")
+    d = tmp_path / "in"
+    shutil.copytree(src, d)
+    mp = (d / "modpara.def").read_text().splitlines()
+    out = []
+    for line in mp:
+        t = line.split()
+        if t and t[0] == "NSROptItrStep":
+            line = "NSROptItrStep 4"
+        elif t and t[0] == "NSROptItrSmp":
+            line = f"NSROptItrSmp {window}"
+        out.append(line)
+    (d / "modpara.def").write_text("\n".join(out) + "\n")
+    ens = M.Ensemble(str(d / "namelist.def"), walkers=2)
+    rows, pars = [], []
+    for _ in range(4):
+        rows.append(ens.opt_step())
+        p_, s_ = ens.params()
+        pars.append(np.concatenate([p_, s_]))
+    ens.write_opt_data(tmp_path)
+    f = lambda x: "% .18e" % x
+    win = list(range(4 - window, 4))
+    vals = [[rows[k][0], rows[k][2]] + list(pars[k]) for k in win]
+    width = len(vals[0])
+    if window == 1:
+        want = "".join(f"{f(vals[0][i])} {f(0.0)} " for i in range(width)) + "\n"
+        means = vals[0]
+    else:
+        means = []
+        want = ""
+        for i in range(width):
+            s_ = 0.0
+            for v in vals:
+                s_ += v[i]
+            mu = s_ / len(vals)
+            var = 0.0
+            for v in vals:
+                var += (v[i] - mu) * (v[i] - mu)
+            means.append(mu)
+            want += f"{f(mu)} {f(0.0)} {f(math.sqrt(var / (len(vals) - 1.0)))} "
+        want += "\n"
+    got = (tmp_path / "zqp_opt.dat").read_text()
+    assert got == want
+    blocks = sorted(p.name for p in tmp_path.glob("zqp_*_opt.dat"))
+    if window == 1:
+        assert blocks == []
+    else:
+        assert "zqp_orbital_opt.dat" in blocks
+        orb = (tmp_path / "zqp_orbital_opt.dat").read_text().splitlines()
+        assert orb[1].split()[0] == "NOrbitalIdx" and int(orb[1].split()[1]) == ens.nslater
+        first = orb[5].split()
+        assert first[0] == "0" and first[1] == f(means[2 + ens.nproj]).strip()
+    if window == 1:
+        # C's one-snapshot form holds (value, 0) pairs, not initial.def triples
+        with pytest.raises(M.BackendError):
+            M.Ensemble(str(d / "namelist.def"), str(tmp_path / "zqp_opt.dat"), walkers=1)
+        ens.close()
+        return
+    # read back as an initial.def: the means, up to the rounding of the gauge
+    # fixing that creation re-applies (sync_modified_parameter)
+    back = M.Ensemble(str(d / "namelist.def"), str(tmp_path / "zqp_opt.dat"), walkers=1)
+    p_, s_ = back.params()
+    m_ = np.array(means[2:])
+    assert np.allclose(np.concatenate([p_, s_]), m_, rtol=1e-12, atol=1e-14 * np.abs(m_).max())
+    back.close()
+    ens.close()
 
 
 def test_measure_caller_configurations(M, tmp_path):
