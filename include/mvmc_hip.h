@@ -141,6 +141,16 @@ int mvmc_ens_opt_step(mvmc_ens* e, double* energy_row6);
  * MVMC_E_INPUT when the window is incomplete or a file cannot be written. */
 int mvmc_ens_write_opt_data(mvmc_ens* e, const char* out_dir);
 
+/* Per-step output of the run loop: output_data (crates/mvmc-core/src/io.rs:
+ * 142-232, text form).  With a directory set (NULL switches it off; ""
+ * = current directory), every mvmc_ens_opt_step appends, after its
+ * measurement and before its SR update (run.rs:1376-1385), the zvo_out row
+ * "E 0  E^2 var Sz Sz^2" (C "% .18e", Sz columns "%.18e"; var by Julia's
+ * complex division) to {CDataFileHead}_out.dat and the energies plus every
+ * parameter ("re im 0.0" triples) to {CDataFileHead}_var.dat; the first step
+ * of the handle truncates both. */
+int mvmc_ens_set_output_dir(mvmc_ens* e, const char* out_dir);
+
 /* Results for parity checks (host arrays). */
 int mvmc_ens_get_energy(mvmc_ens* e, double* wc_etot_etot2_sz_sz2);
 int mvmc_ens_get_operands(mvmc_ens* e, double* oo_2size, double* ho_size);

@@ -271,6 +271,9 @@ T* dupload(std::vector<void*>& list, const std::vector<T>& h) {
 }  // namespace
 
 struct mvmc_ens {
+  // per-step output (output_data): directory, or empty for none
+  std::string out_dir;
+  bool out_enabled = false;
   // optimisation window of the last NSROptItrSmp steps (store_opt_data,
   // crates/mvmc-core/src/io.rs:21-48): energy, energy^2 and the parameters
   // after the step's SR update
@@ -804,6 +807,87 @@ void measure_configs_impl(mvmc_ens* E, int count, const int* ele_idx, double* e_
   E->check_status();
 }
 
+// Julia's ComplexF64 division (crates/pfapack/src/julia_complex.rs:9-77) on
+// the host, for the zvo_out variance.
+double julia_div_part_h(double a, double b, double c, double d, double r, double t) {
+  if (r != 0.0) {
+    const double br = b * r;
+    return br != 0.0 ? (a + br) * t : a * t + (b * t) * r;
+  }
+  return (a + d * (b / c)) * t;
+}
+void julia_div_h(double a, double b, double c, double d, double& re, double& im) {
+  const double ab = std::fabs(a) >= std::fabs(b) ? std::fabs(a) : std::fabs(b);
+  const double cdm = std::fabs(c) >= std::fabs(d) ? std::fabs(c) : std::fabs(d);
+  if (std::isinf(c) || std::isinf(d)) {
+    auto sgn = [](double x) { return (x == 0.0 || std::isnan(x)) ? x : std::copysign(1.0, x); };
+    if (std::isfinite(a) && std::isfinite(b)) {
+      re = 0.0 * sgn(a) * sgn(c);
+      im = -0.0 * sgn(b) * sgn(d);
+    } else {
+      re = im = NAN;
+    }
+    return;
+  }
+  const double halfov = 0x1.fffffffffffffp+1022, twouneps = 0x1p-969, bs = 0x1p+105;
+  double s = 1.0;
+  const bool scaled = ab >= halfov || ab <= twouneps || cdm >= halfov || cdm <= twouneps;
+  if (scaled) {
+    if (ab >= halfov) { a *= 0.5; b *= 0.5; s *= 2.0; }
+    else if (ab <= twouneps) { a *= bs; b *= bs; s /= bs; }
+    if (cdm >= halfov) { c *= 0.5; d *= 0.5; s *= 0.5; }
+    else if (cdm <= twouneps) { c *= bs; d *= bs; s *= bs; }
+  }
+  bool swapped = false;
+  if (!(std::fabs(d) <= std::fabs(c))) {
+    std::swap(a, b);
+    std::swap(c, d);
+    swapped = true;
+  }
+  const double r = d / c, t = 1.0 / (c + d * r);
+  re = julia_div_part_h(a, b, c, d, r, t);
+  im = julia_div_part_h(b, -a, c, d, r, t);
+  if (swapped) im = -im;
+  if (scaled) { re *= s; im *= s; }
+}
+
+std::string cfmt18(double x, bool space) {
+  char b[64];
+  std::snprintf(b, sizeof b, space ? "% .18e" : "%.18e", x);
+  return b;
+}
+
+// output_data (crates/mvmc-core/src/io.rs:142-232), text form: appends the
+// step's {head}_out.dat row and {head}_var.dat line (step 0 truncates both);
+// called after the measurement and before the SR update (run.rs:1376-1385).
+void write_step_output(mvmc_ens* E, int step) {
+  const mvmc::Model& m = E->m;
+  const std::string dir = E->out_dir.empty() ? std::string() : E->out_dir + "/";
+  const std::string head = m.mp.c_data_file_head.empty() ? "zvo" : m.mp.c_data_file_head;
+  const double er = E->etot, ei = 0.0;
+  // (etot2 - etot*etot) / (etot*etot) in complex arithmetic, Julia's division
+  const double sq_re = er * er - ei * ei, sq_im = er * ei + ei * er;
+  double vre = 0.0, vim = 0.0;
+  if (std::hypot(er, ei) > 1.0e-14) julia_div_h(E->etot2 - sq_re, 0.0 - sq_im, sq_re, sq_im, vre, vim);
+  const char* mode = step == 0 ? "w" : "a";
+  const std::string op = dir + head + "_out.dat";
+  FILE* f = std::fopen(op.c_str(), mode);
+  if (!f) throw Fail(MVMC_E_INPUT, "cannot write " + op);
+  std::fprintf(f, "%s %s  %s %s %s %s\n", cfmt18(er, true).c_str(), cfmt18(ei, true).c_str(),
+               cfmt18(E->etot2, true).c_str(), cfmt18(vre, true).c_str(), cfmt18(E->sztot, false).c_str(),
+               cfmt18(E->sztot2, false).c_str());
+  std::fclose(f);
+  const std::string vp = dir + head + "_var.dat";
+  FILE* g = std::fopen(vp.c_str(), mode);
+  if (!g) throw Fail(MVMC_E_INPUT, "cannot write " + vp);
+  std::fprintf(g, "%s %s 0.0 %s %s 0.0 ", cfmt18(er, true).c_str(), cfmt18(ei, true).c_str(),
+               cfmt18(E->etot2, true).c_str(), cfmt18(0.0, true).c_str());
+  for (double x : m.proj) std::fprintf(g, "%s %s 0.0 ", cfmt18(x, true).c_str(), cfmt18(0.0, true).c_str());
+  for (double x : m.slater) std::fprintf(g, "%s %s 0.0 ", cfmt18(x, true).c_str(), cfmt18(0.0, true).c_str());
+  std::fprintf(g, "\n");
+  std::fclose(g);
+}
+
 // Parameter update of an SR step (sr.rs:408-421, sync.rs): p[map[si]] += x[si].
 void apply_sr_update(mvmc_ens* E, const std::vector<int>& map, const std::vector<double>& x) {
   mvmc::Model& m = E->m;
@@ -1126,6 +1210,7 @@ int mvmc_ens_opt_step(mvmc_ens* e, double* row) {
       const double r6[6] = {e->etot, 0.0, e->etot2, var, e->sztot, e->sztot2};
       std::memcpy(row, r6, sizeof(r6));
     }
+    if (e->out_enabled) write_step_output(e, e->opt_steps);
     long long info = 0;
     int rc = sr_impl(e, &info);
     if (rc != 0) throw Fail(MVMC_E_NUMERIC, "SR step skipped (non-finite update)");
@@ -1157,6 +1242,13 @@ int mvmc_ens_opt_step(mvmc_ens* e, double* row) {
 // (mean.re, mean.im, sample standard deviation) triples, mean summed in
 // window order; then {head}_{gutzwiller,jastrow,orbital}_opt.dat blocks of
 // the means.
+int mvmc_ens_set_output_dir(mvmc_ens* e, const char* out_dir) {
+  return guarded([&] {
+    e->out_enabled = out_dir != nullptr;
+    e->out_dir = out_dir ? out_dir : "";
+  });
+}
+
 int mvmc_ens_write_opt_data(mvmc_ens* e, const char* out_dir) {
   return guarded([&] {
     const mvmc::Model& m = e->m;

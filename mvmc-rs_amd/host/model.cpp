@@ -97,6 +97,7 @@ void parse_modpara(const std::string& path, ModPara& p) {
   for (size_t i = 0; i < 8 && i < lines.size(); ++i) {
     auto t = tokens(lines[i]);
     if (t.size() >= 2 && lower(t[0]) == "cparafilehead") p.c_para_file_head = t[1];
+    if (t.size() >= 2 && lower(t[0]) == "cdatafilehead") p.c_data_file_head = t[1];
   }
   for (size_t i = 8; i < lines.size(); ++i) {
     const std::string& l = lines[i];

@@ -47,6 +47,7 @@ struct ModPara {
   int nsr_opt_itr_step = 1000;
   int nsr_opt_itr_smp = 100;
   std::string c_para_file_head = "zqp";  // CParaFileHead of the banner (parameter output files)
+  std::string c_data_file_head = "zvo";  // CDataFileHead of the banner (per-step output files)
   double dsr_opt_red_cut = 0.001;
   double dsr_opt_sta_del = 0.02;
   double dsr_opt_step_dt = 0.02;

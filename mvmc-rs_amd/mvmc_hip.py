@@ -56,6 +56,7 @@ def lib():
         L.mvmc_sr_cg_apply.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 4 + \
             [ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
         L.mvmc_ens_write_opt_data.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+        L.mvmc_ens_set_output_dir.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
         L.mvmc_sr_gram_real.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.mvmc_sr_cholesky_solve.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                              ctypes.POINTER(ctypes.c_int), ctypes.c_void_p]
@@ -71,7 +72,7 @@ EXPORTED_SYMBOLS = [
     "mvmc_ens_timings", "mvmc_ens_phase_clocks", "mvmc_ens_status", "mvmc_sr_cg_solve", "mvmc_ens_set_precision",
     "mvmc_ens_vjp", "mvmc_ens_log_psi", "mvmc_ens_sr_direct", "mvmc_pfinv_c128",
     "mvmc_ens_measure_configs", "mvmc_sr_cg_apply", "mvmc_sr_gram_real", "mvmc_sr_cholesky_solve",
-    "mvmc_ens_write_opt_data",
+    "mvmc_ens_write_opt_data", "mvmc_ens_set_output_dir",
 ]
 
 
@@ -139,6 +140,10 @@ class Ensemble:
 
     def measure(self):
         _check(lib().mvmc_ens_measure(self.h))
+
+    def set_output_dir(self, out_dir):
+        """Per-step zvo_out / zvo_var output into out_dir (None: off)."""
+        _check(lib().mvmc_ens_set_output_dir(self.h, None if out_dir is None else str(out_dir).encode()))
 
     def write_opt_data(self, out_dir):
         """Write {CParaFileHead}_opt.dat (+ block files) of the recorded window."""

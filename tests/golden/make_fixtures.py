@@ -16,6 +16,7 @@ Everything written here is data held by the reference's tests:
                                      tests/fixtures/pfapack/complex_n*.orig.txt
   * two-hop bilinear bits            tests/fixtures/pfaffian_cg/two_hop_bilinear{,_c}.txt
   * hopping candidate retry tapes    tests/fixtures/candidate/hopping_retry_cap_559.txt
+  * zvo_out row layout               tests/fixtures/c_integration_reference/hubbard_chain_real/zvo_out_first10.dat
 """
 import json
 import os
@@ -175,6 +176,14 @@ def candidate():
     os.chmod(os.path.join(OUT, "candidate_hopping_retry_cap.txt"), 0o644)
 
 
+def zvo_rows():
+    """C 1.4.0 zvo_out.dat rows of the real Hubbard chain (data file, as is):
+    the byte layout of the per-step output row."""
+    shutil.copy(os.path.join(FIX, "c_integration_reference", "hubbard_chain_real", "zvo_out_first10.dat"),
+                os.path.join(OUT, "zvo_out_c_rows.dat"))
+    os.chmod(os.path.join(OUT, "zvo_out_c_rows.dat"), 0o644)
+
+
 def bilinear():
     lines = [l for l in open(os.path.join(FIX, "pfaffian_cg/two_hop_bilinear.txt")) if l.strip() and not l.startswith("#")]
     expected = [l.split() for l in open(os.path.join(FIX, "pfaffian_cg/two_hop_bilinear_c.txt")) if not l.startswith("#")]
@@ -199,5 +208,6 @@ if __name__ == "__main__":
     pfapack()
     pfapack_complex()
     candidate()
+    zvo_rows()
     bilinear()
     print("fixtures written to", OUT)

@@ -646,6 +646,39 @@ def test_opt_data_output(M, tmp_path, window):
     ens.close()
 
 
+def test_step_output_files(M, tmp_path):
+    """Per-step output (output_data, io.rs:142-232): three optimisation steps
+    of the Hubbard chain append zvo_out rows byte-identical to the C layout
+    (the layout pinned on the C 1.4.0 rows in test_oracle_fixtures) with the
+    variance by Julia's division, and zvo_var lines carrying the energies and
+    the parameters each step sampled with (before its SR update)."""
+    namelist = os.path.join(GOLDEN, "inputs", "hubbard_chain", "namelist.def")
+    ens = M.Ensemble(namelist, walkers=2)
+    ens.set_output_dir(tmp_path)
+    f = lambda x: "% .18e" % x
+    rows, pre = [], []
+    for _ in range(3):
+        p_, s_ = ens.params()
+        pre.append(np.concatenate([p_, s_]))
+        rows.append(ens.opt_step())
+    out = (tmp_path / "zvo_out.dat").read_text().splitlines()
+    var = (tmp_path / "zvo_var.dat").read_text().splitlines()
+    assert len(out) == 3 and len(var) == 3
+    for k, r in enumerate(rows):
+        e, e2 = r[0], r[2]
+        a, c = e2 - e * e, e * e
+        v = (a + 0.0 * (0.0 / c)) * (1.0 / (c + 0.0 * (0.0 / c)))    # Julia's (a + 0i)/(c + 0i)
+        want = f"{f(e)} {f(0.0)}  {f(e2)} {f(v)} {'%.18e' % r[4]} {'%.18e' % r[5]}"
+        assert out[k] == want, (k, out[k], want)
+        fields = var[k].split()
+        assert len(fields) == 3 * (2 + len(pre[k]))
+        assert float(fields[0]) == e and float(fields[3]) == e2
+        got_p = np.array([float(x) for x in fields[6::3]])
+        assert np.array_equal(got_p, pre[k])
+    ens.set_output_dir(None)
+    ens.close()
+
+
 def test_measure_caller_configurations(M, tmp_path):
     """MeasurementStages::measure_sample on caller-supplied configurations
     (mvmc_ens_measure_configs): the 8x8 benchmark lattice's sampled

@@ -146,3 +146,20 @@ def test_hopping_candidate_retry_cap_matches_c_tapes():
         ns, ne = c["nsite"][0], c["nelec"][0]
         got = O.hop_candidate_tape(ns, ne, c["loc_spn"], c["ele_idx"], c["ele_cfg"], c["words"])
         assert got == c["result"], (c["name"], got, c["result"])
+
+
+def zvo_row_text(e, e2, var, sz, sz2):
+    """The zvo_out row layout (crates/mvmc-core/src/io.rs:166-179): C "% .18e"
+    with two spaces before E^2 and "%.18e" for the Sz columns."""
+    f = lambda x: "% .18e" % x
+    return f"{f(e)} {f(0.0)}  {f(e2)} {f(var)} {'%.18e' % sz} {'%.18e' % sz2}"
+
+
+def test_zvo_row_layout_matches_c_bytes():
+    """Every row of the C 1.4.0 zvo_out.dat fixture re-rendered from its own
+    values reproduces the C bytes (pins the row layout the library writes)."""
+    for line in open(os.path.join(GOLDEN, "zvo_out_c_rows.dat")):
+        if not line.strip():
+            continue
+        v = [float(x) for x in line.split()]
+        assert zvo_row_text(v[0], v[2], v[3], v[4], v[5]) == line.rstrip("\n")
