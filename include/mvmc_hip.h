@@ -203,6 +203,11 @@ int mvmc_ens_phase_clocks(mvmc_ens* e, long long* clocks32);
  * 3 rebuild failure, 4 non-finite, 5 CG failure). */
 int mvmc_ens_status(mvmc_ens* e);
 
+/* The standalone SR entry points below (mvmc_sr_cg_apply, mvmc_sr_gram_real,
+ * mvmc_sr_cholesky_solve, mvmc_sr_cg_solve) keep one device workspace or
+ * rocBLAS handle per process: calls that overlap from several host threads
+ * race; serialise them (the ensemble's own SR uses per-handle state). */
+
 /* SrStages::cg_local_product (crates/mvmc-core/src/sr_backend.rs:518-598),
  * one product of the sampled operator on device operands:
  * z = inv_w O^T (O x) - (mean . x) mean + shift diag * x (sr_cg.rs:952-1004),
