@@ -12,6 +12,12 @@
 //   * multi-walker sampler:  native-C per-rank counters / EleIdx / SFMT state
 //                            (tests/golden/mpi179_*.json, Heisenberg chain)
 //   * SR-CG iterates:        native-C iterates (tests/golden/sr_cg_real.json)
+//   * hopping candidate:     C retry-cap word tapes
+//                            (tests/golden/candidate_hopping_retry_cap.txt)
+//   * Metropolis decision:   closed forms (weight, non-finite, real2 bound)
+//   * local energy:          brute force, every Green ratio against full
+//                            Pfaffians of the moved configuration
+//   * complex Pfaffian:      see pfapack_c.h
 // Parity unpinned: the inverse against OpenBLAS dtrtri/dtrmm bits (residual
 // checks only), glibc exp/log bits on a different glibc version.
 #include <cmath>
