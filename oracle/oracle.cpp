@@ -11,7 +11,7 @@
 //                            (tests/golden/hubbard_cg_store0.json)
 //   * multi-walker sampler:  native-C per-rank counters / EleIdx / SFMT state
 //                            (tests/golden/mpi179_*.json, Heisenberg chain)
-//   * SR-CG iterates:        native-C iterates (tests/golden/sr_cg_real.json)
+//   * SR-CG iterates:        native-C iterates (tests/golden/sr_cg_real*.txt)
 //   * hopping candidate:     C retry-cap word tapes
 //                            (tests/golden/candidate_hopping_retry_cap.txt)
 //   * Metropolis decision:   closed forms (weight, non-finite, real2 bound)
