@@ -460,6 +460,59 @@ static bool hop_candidate(int ns, int ne, const int* loc_spn, const int* ele_idx
   return false;
 }
 
+// FSZ candidates (the sampler that uses them is not built; restated here so
+// their draw order is pinned ahead of it).  Lookups outside the arrays read
+// as "local site" and "not empty", as the reference's bounds-checked reads.
+// makeCandidate_hopping_fsz / _csz (candidate.rs:387-492): source drawn over
+// all n_size electrons until itinerant (no cap), spin ele_spn[mi]; then
+// destination site and, with 2Sz = -1, target spin; cap tested after each
+// draw, then occupancy.
+template <class RNG>
+static bool hop_candidate_fsz(int ns, int nsize, int two_sz, const int* loc_spn, const int* ele_idx,
+                              const int* ele_cfg, const int* ele_spn, RNG& rng, int& mi, int& s, int& t, int& ri,
+                              int& rj) {
+  auto loc = [&](int r) { return (r < 0 || r >= ns) ? 1 : loc_spn[r]; };
+  auto cfg = [&](int k) { return (k < 0 || k >= 2 * ns) ? 0 : ele_cfg[k]; };
+  for (;;) {
+    mi = (int)rng.mod((uint32_t)nsize);
+    s = std::min(std::max(ele_spn[mi], 0), 1);
+    ri = std::max(ele_idx[mi], 0);
+    if (loc(ri) == 0) break;
+  }
+  const long long icnt_max = (long long)ns * ns;
+  long long icnt = 0;
+  for (;;) {
+    rj = (int)rng.mod((uint32_t)ns);
+    t = two_sz == -1 ? (rng.real2() < 0.5 ? 0 : 1) : s;
+    if (icnt > icnt_max) return true;
+    ++icnt;
+    if (cfg(rj + t * ns) == -1 && loc(rj) == 0) break;
+  }
+  return false;
+}
+
+// makeCandidate_LocalSpinFlip_conduction (candidate.rs:494-566): draw an
+// electron, flip its spin; cap tested after each draw, then "itinerant site
+// with the flipped spin-site empty".  rj = ri.
+template <class RNG>
+static bool spin_flip_conduction(int ns, int nsize, const int* loc_spn, const int* ele_idx, const int* ele_cfg,
+                                 const int* ele_spn, RNG& rng, int& mi, int& s, int& t, int& ri) {
+  auto loc = [&](int r) { return (r < 0 || r >= ns) ? 1 : loc_spn[r]; };
+  auto cfg = [&](int k) { return (k < 0 || k >= 2 * ns) ? 0 : ele_cfg[k]; };
+  const long long icnt_max = (long long)ns * ns;
+  long long icnt = 0;
+  for (;;) {
+    mi = (int)rng.mod((uint32_t)nsize);
+    s = std::min(std::max(ele_spn[mi], 0), 1);
+    t = 1 - s;
+    ri = std::max(ele_idx[mi], 0);
+    if (icnt > icnt_max) return true;
+    ++icnt;
+    if (loc(ri) == 0 && cfg(ri + t * ns) == -1) break;
+  }
+  return false;
+}
+
 // Raw-word tape with SFMT's draw conversions (mod: word % n, real2: word 2^-32).
 struct WordTape {
   const uint32_t* w;
@@ -1289,6 +1342,31 @@ int oracle_overlap_idx(void* h, const int* ele_idx, double* ip_out, int* cnt_out
   *ip_out = st ? 0.0 : calc_ip(m, pf.data());
   for (int i = 0; i < m.n_proj(); ++i) cnt_out[i] = cnt[i];
   return st;
+}
+
+// FSZ candidates on a raw-word tape: kind 0 hopping_fsz (2Sz = -1), 1
+// hopping_csz, 2 LocalSpinFlip_conduction; out8 as oracle_hop_candidate_tape.
+int oracle_fsz_candidate_tape(int kind, int ns, int nsize, const int* loc_spn, const int* ele_idx,
+                              const int* ele_cfg, const int* ele_spn, const uint32_t* words, int nwords,
+                              int* out8) {
+  WordTape tape{words, nwords};
+  try {
+    int mi = 0, s = 0, t = 0, ri = 0, rj = 0;
+    bool rej;
+    if (kind == 2) {
+      rej = spin_flip_conduction(ns, nsize, loc_spn, ele_idx, ele_cfg, ele_spn, tape, mi, s, t, ri);
+      rj = ri;
+    } else {
+      rej = hop_candidate_fsz(ns, nsize, kind == 0 ? -1 : 0, loc_spn, ele_idx, ele_cfg, ele_spn, tape, mi, s, t,
+                              ri, rj);
+    }
+    const int used = tape.pos;
+    const int vals[8] = {mi, ri, rj, s, t, rej ? 1 : 0, used, used + (rej ? 0 : 1)};
+    std::memcpy(out8, vals, sizeof(vals));
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
+  }
 }
 
 // Metropolis decision with the uniform draw given as a raw word.

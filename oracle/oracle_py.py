@@ -113,6 +113,19 @@ def hop_candidate_tape(nsite, nelec, loc_spn, ele_idx, ele_cfg, words):
     return None if rc != 0 else [int(x) for x in out]
 
 
+def fsz_candidate_tape(kind, nsite, loc_spn, ele_idx, ele_cfg, ele_spn, words):
+    """FSZ candidate on a raw-word tape (kind 0 hopping_fsz, 1 hopping_csz,
+    2 local spin flip of a conduction electron): the 8 result numbers, or
+    None if the tape ran out."""
+    arr = lambda a, t: np.ascontiguousarray(a, dtype=t)
+    ls, ei, ec, es = (arr(x, np.int32) for x in (loc_spn, ele_idx, ele_cfg, ele_spn))
+    w = arr(words, np.uint32)
+    out = np.zeros(8, dtype=np.int32)
+    rc = lib().oracle_fsz_candidate_tape(ctypes.c_int(kind), ctypes.c_int(nsite), ctypes.c_int(ei.size), ptr(ls),
+                                         ptr(ei), ptr(ec), ptr(es), ptr(w), ctypes.c_int(w.size), ptr(out))
+    return None if rc != 0 else [int(x) for x in out]
+
+
 def metropolis_word(lpd, lip_new, lip_old, word):
     """(accept, weight) of the Metropolis decision with uniform draw word * 2^-32."""
     w = ctypes.c_double(0.0)

@@ -137,8 +137,7 @@ def test_hopping_candidate_retry_cap_matches_c_tapes():
     (tests/fixtures/candidate/hopping_retry_cap_559.txt, C vmcmake.c verbatim on
     raw words): source and destination draws, the Nsite^2 cap tested after each
     destination draw, the source loop's itinerant-electron check, and the words
-    consumed before and after the Metropolis draw.  The FSZ / local-spin-flip
-    cases of the same file belong to the FSZ sampler, which is not built."""
+    consumed before and after the Metropolis draw."""
     cases = _parse_candidate_cases(os.path.join(GOLDEN, "candidate_hopping_retry_cap.txt"))
     hop = [c for c in cases if c["kind"] == "hopping"]
     assert len(hop) >= 7
@@ -163,3 +162,19 @@ def test_zvo_row_layout_matches_c_bytes():
             continue
         v = [float(x) for x in line.split()]
         assert zvo_row_text(v[0], v[2], v[3], v[4], v[5]) == line.rstrip("\n")
+
+
+def test_fsz_candidates_match_c_tapes():
+    """The FSZ siblings on the same C tapes: makeCandidate_hopping_fsz
+    (target spin drawn, 2Sz = -1), makeCandidate_hopping_csz (spin kept) and
+    makeCandidate_LocalSpinFlip_conduction, at their final and last permitted
+    draws (candidate.rs:387-566).  The FSZ sampler that will use them is not
+    built; the loops are pinned ahead of it."""
+    cases = _parse_candidate_cases(os.path.join(GOLDEN, "candidate_hopping_retry_cap.txt"))
+    kinds = {"hopping_fsz": 0, "hopping_csz": 1, "conduction": 2}
+    fsz = [c for c in cases if c["kind"] in kinds]
+    assert len(fsz) == 6
+    for c in fsz:
+        got = O.fsz_candidate_tape(kinds[c["kind"]], c["nsite"][0], c["loc_spn"], c["ele_idx"], c["ele_cfg"],
+                                   c["ele_spn"], c["words"])
+        assert got == c["result"], (c["name"], got, c["result"])
