@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <filesystem>
 #include <map>
 #include <mutex>
 #include <set>
@@ -863,6 +864,10 @@ std::string cfmt18(double x, bool space) {
 void write_step_output(mvmc_ens* E, int step) {
   const mvmc::Model& m = E->m;
   const std::string dir = E->out_dir.empty() ? std::string() : E->out_dir + "/";
+  if (!E->out_dir.empty()) {  // output_path creates the directory (io.rs:127-140)
+    std::error_code ec;
+    std::filesystem::create_directories(E->out_dir, ec);
+  }
   const std::string head = m.mp.c_data_file_head.empty() ? "zvo" : m.mp.c_data_file_head;
   const double er = E->etot, ei = 0.0;
   // (etot2 - etot*etot) / (etot*etot) in complex arithmetic, Julia's division
@@ -1257,6 +1262,10 @@ int mvmc_ens_write_opt_data(mvmc_ens* e, const char* out_dir) {
     for (char c : e->opt_win_set) complete = complete && c;
     if (!complete) throw Fail(MVMC_E_INPUT, "incomplete optimization window");
     const std::string dir = out_dir && *out_dir ? std::string(out_dir) + "/" : std::string();
+    if (out_dir && *out_dir) {  // output_path creates the directory (io.rs:127-140)
+      std::error_code ec;
+      std::filesystem::create_directories(out_dir, ec);
+    }
     const std::string head = m.mp.c_para_file_head.empty() ? "zqp" : m.mp.c_para_file_head;
     const size_t width = e->opt_win[0].size();
     auto value = [&](size_t w, size_t idx) {

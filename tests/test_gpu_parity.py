@@ -654,15 +654,16 @@ def test_step_output_files(M, tmp_path):
     the parameters each step sampled with (before its SR update)."""
     namelist = os.path.join(GOLDEN, "inputs", "hubbard_chain", "namelist.def")
     ens = M.Ensemble(namelist, walkers=2)
-    ens.set_output_dir(tmp_path)
+    out_dir = tmp_path / "created" / "by" / "writer"   # missing directories are created
+    ens.set_output_dir(out_dir)
     f = lambda x: "% .18e" % x
     rows, pre = [], []
     for _ in range(3):
         p_, s_ = ens.params()
         pre.append(np.concatenate([p_, s_]))
         rows.append(ens.opt_step())
-    out = (tmp_path / "zvo_out.dat").read_text().splitlines()
-    var = (tmp_path / "zvo_var.dat").read_text().splitlines()
+    out = (out_dir / "zvo_out.dat").read_text().splitlines()
+    var = (out_dir / "zvo_var.dat").read_text().splitlines()
     assert len(out) == 3 and len(var) == 3
     for k, r in enumerate(rows):
         e, e2 = r[0], r[2]
