@@ -45,7 +45,7 @@ def build_library(defines=(), out=None):
 
 def build_oracle():
     os.makedirs(ORACLE_BUILD, exist_ok=True)
-    srcs = [os.path.join(ROOT, "oracle", "oracle.cpp"), os.path.join(HERE, "host", "model.cpp")]
+    srcs = [os.path.join(ROOT, "oracle", "oracle.cpp"), os.path.join(ROOT, "oracle", "expert.cpp")]
     tmp = ORACLE_LIB + ".tmp"
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
           "-o", tmp] + srcs + ["-lpthread"])

@@ -20,6 +20,7 @@
 //   * complex Pfaffian:      see pfapack_c.h
 // Parity unpinned: the inverse against OpenBLAS dtrtri/dtrmm bits (residual
 // checks only), glibc exp/log bits on a different glibc version.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -30,12 +31,11 @@
 #include <vector>
 
 #include "../mvmc-rs_amd/csrc/ddmath.h"
-#include "../mvmc-rs_amd/host/model.h"
+#include "expert.h"
 #include "pfapack.h"
 #include "pfapack_c.h"
 #include "sfmt.h"
 
-using mvmc::Model;
 using std::vector;
 
 namespace oracle {
@@ -1132,9 +1132,26 @@ static int stochastic_opt_cg(Run& R) {
 // Optimisation driver: crates/mvmc-core/src/run.rs:1294-1475 with W walkers
 // (multichain.rs:647-689, rank-order left-fold reductions)
 // ---------------------------------------------------------------------------
+// observables.rs:1551-1605 build_direct_projection_tables: per-site
+// Gutzwiller values and the symmetric Jastrow value table (0 on the diagonal).
+static void projection_tables(const Model& m, vector<double>& g, vector<double>& j) {
+  const int n = m.nsite();
+  g.assign(n, 0.0);
+  j.assign((size_t)n * n, 0.0);
+  for (int s = 0; s < n && m.n_gutz > 0; ++s) {
+    const int k = m.gutz_idx[s];
+    if (k >= 0 && k < m.n_gutz) g[s] = m.proj[k];
+  }
+  for (int a = 0; a < n && m.n_jast > 0; ++a)
+    for (int b = 0; b < n; ++b) {
+      const int k = a == b ? -1 : m.jast(a < b ? a : b, a < b ? b : a);
+      if (k >= 0 && k < m.n_jast) j[(size_t)a * n + b] = m.proj[m.n_gutz + k];
+    }
+}
+
 static void refresh_tables(Run& R) {
-  mvmc::build_slater_table(R.m, R.slater);
-  mvmc::direct_projection_tables(R.m, R.dproj_g, R.dproj_j);
+  slater_table(R.m, R.slater);
+  projection_tables(R.m, R.dproj_g, R.dproj_j);
 }
 
 template <class F>
@@ -1207,7 +1224,7 @@ static int opt_step(Run& R, bool do_sr) {
   else throw std::runtime_error("the oracle implements the SR-CG solver only (NSRCG 1)");
   R.t_sr += now_s() - t3;
   if (rc != 0) return rc;
-  mvmc::sync_modified_parameter(m, true);
+  sync_parameters(m, true);
   return 0;
 }
 
@@ -1418,7 +1435,7 @@ void* oracle_run_create(const char* namelist, const char* initial_def, int walke
                         int use_seed_override, char* err, int errlen) {
   try {
     Run* R = new Run();
-    R->m = mvmc::load_namelist(namelist);
+    R->m = read_expert(namelist);
     Model& m = R->m;
     const long long seed = use_seed_override ? seed_override : m.mp.rnd_seed;
     R->w.resize(walkers);
@@ -1435,9 +1452,15 @@ void* oracle_run_create(const char* namelist, const char* initial_def, int walke
       }
     }
     for (double& p : m.proj) p = 0.0;
-    if (initial_def && *initial_def) mvmc::load_initial_def(m, initial_def);
-    mvmc::sync_modified_parameter(m, true);
-    mvmc::translation_maps(m, R->tsite, R->tsgn);
+    if (initial_def && *initial_def) read_initial(m, initial_def);
+    sync_parameters(m, true);
+    {
+      const int ns = m.nsite(), nt = m.n_mp_trans();
+      R->tsite.assign((size_t)nt * ns, -1);
+      R->tsgn.assign((size_t)nt * ns, 1);
+      for (int t = 0; t < nt; ++t)
+        for (int r = 0; r < ns; ++r) translate(m, r, t, R->tsite[(size_t)t * ns + r], R->tsgn[(size_t)t * ns + r]);
+    }
     const int n = m.nsize(), n2 = m.nsite2();
     for (auto& W : R->w) {
       W.ele_idx.assign(n, -1);
@@ -1602,6 +1625,52 @@ double oracle_measure_idx(void* h, const int* ele_idx, double* o_out, int* valid
   if (o_out) std::copy(o.begin(), o.end(), o_out);
   *valid = ok ? 1 : 0;
   return ok ? e : 0.0;
+}
+
+// Gauge sync of bare parameter arrays (expert.cpp sync_parameters) and its
+// Gutzwiller / Jastrow gate, for the pins on the C SetFlagShift records.
+int oracle_gj_shift_gate(int n_gutz, int n_jast, const int* flags, int nflags) {
+  return gj_shift_gate(n_gutz, n_jast, std::vector<int>(flags, flags + nflags)) ? 1 : 0;
+}
+
+void oracle_sync_parameters(int n_gutz, int n_jast, const int* flags, int nflags, double* proj, int n_slater,
+                            double* slater) {
+  Model m;
+  m.n_gutz = n_gutz;
+  m.n_jast = n_jast;
+  m.n_slater = n_slater;
+  m.opt_flags.assign(flags, flags + nflags);
+  m.proj.assign(proj, proj + n_gutz + n_jast);
+  m.slater.assign(slater, slater + n_slater);
+  sync_parameters(m, true);
+  std::copy(m.proj.begin(), m.proj.end(), proj);
+  std::copy(m.slater.begin(), m.slater.end(), slater);
+}
+
+// Gauss-Legendre nodes and weights of the oracle's QP weights on [x1, x2].
+void oracle_gauss_leg(int n, double x1, double x2, double* x, double* w) {
+  std::vector<double> xv, wv;
+  gauss_leg(x1, x2, n, xv, wv);
+  std::copy(xv.begin(), xv.end(), x);
+  std::copy(wv.begin(), wv.end(), w);
+}
+
+// Spin-projection Gauss-Legendre points of the oracle's QP weights:
+// out[0..nl) cos^2(beta/2), [nl..2nl) sin^2, [2nl..3nl) cos sin, then the
+// QP full weights of NMPTrans = 1 with translation weight 1.
+void oracle_spin_gl(int nl, int nstot, double* out) {
+  Model m;
+  m.mp.nsp_gauss_leg = nl;
+  m.mp.nsp_stot = nstot;
+  m.mp.nmp_trans = 1;
+  m.tr_weight.assign(1, 1.0);
+  qp_weights(m);
+  for (int i = 0; i < nl; ++i) {
+    out[i] = m.spgl_cos_cos[i];
+    out[nl + i] = m.spgl_sin_sin[i];
+    out[2 * nl + i] = m.spgl_cos_sin[i];
+    out[3 * nl + i] = m.qp_full_weight[i];
+  }
 }
 
 }  // extern "C"

@@ -17,6 +17,9 @@ Everything written here is data held by the reference's tests:
   * two-hop bilinear bits            tests/fixtures/pfaffian_cg/two_hop_bilinear{,_c}.txt
   * hopping candidate retry tapes    tests/fixtures/candidate/hopping_retry_cap_559.txt
   * zvo_out row layout               tests/fixtures/c_integration_reference/hubbard_chain_real/zvo_out_first10.dat
+  * C SetFlagShift gates             tests/fixtures/optimization_flags/c_shift_flags.txt
+
+Usage: make_fixtures.py [--ref REFERENCE_ROOT] [fixture ...] (default: all).
 """
 import json
 import os
@@ -27,7 +30,10 @@ import sys
 
 import numpy as np
 
-REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+ARGS = sys.argv[1:]
+REF = "/root/reference"
+if len(ARGS) >= 2 and ARGS[0] == "--ref":
+    REF, ARGS = ARGS[1], ARGS[2:]
 OUT = os.path.dirname(os.path.abspath(__file__))
 FIX = os.path.join(REF, "tests", "fixtures")
 
@@ -184,6 +190,12 @@ def zvo_rows():
     os.chmod(os.path.join(OUT, "zvo_out_c_rows.dat"), 0o644)
 
 
+def shift_flags():
+    """C SetFlagShift eligibility records (data file, copied as is)."""
+    shutil.copy(os.path.join(FIX, "optimization_flags", "c_shift_flags.txt"), os.path.join(OUT, "c_shift_flags.txt"))
+    os.chmod(os.path.join(OUT, "c_shift_flags.txt"), 0o644)
+
+
 def bilinear():
     lines = [l for l in open(os.path.join(FIX, "pfaffian_cg/two_hop_bilinear.txt")) if l.strip() and not l.startswith("#")]
     expected = [l.split() for l in open(os.path.join(FIX, "pfaffian_cg/two_hop_bilinear_c.txt")) if not l.startswith("#")]
@@ -200,14 +212,10 @@ def bilinear():
                         values=np.concatenate(values), bits=np.array(bits, dtype=np.uint64))
 
 
+ALL = [sfmt, hubbard, heisenberg, sr_cg, pfapack, pfapack_complex, candidate, zvo_rows, bilinear, shift_flags]
+
 if __name__ == "__main__":
-    sfmt()
-    hubbard()
-    heisenberg()
-    sr_cg()
-    pfapack()
-    pfapack_complex()
-    candidate()
-    zvo_rows()
-    bilinear()
-    print("fixtures written to", OUT)
+    chosen = [f for f in ALL if not ARGS or f.__name__ in ARGS]
+    for f in chosen:
+        f()
+    print("fixtures written to", OUT, ":", ", ".join(f.__name__ for f in chosen))
