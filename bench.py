@@ -6,57 +6,98 @@ optimisation path on one MI355X, 16x16 t-t'-U Hubbard model (Nsize 224,
 A step is one SR step in the steady state: every walker restores its burn
 configuration, runs (NVMCSample + 1) x NVMCInterval x Nsite proposals,
 saves NVMCSample samples, the samples are measured and reduced, and the
-SR-CG solve updates the parameters.  --warmup untimed steps come first (the
+SR-CG solve updates the parameters.  Untimed warm-up steps come first (the
 first one thermalises with NVMCWarmUp x NVMCInterval x Nsite extra
 proposals; --vmc-warmup sets NVMCWarmUp, default 2, which only changes the
 length of that untimed step).
+
+The whole run keeps to a wall budget (--budget seconds from process start,
+default 420): at least one warm-up and two timed steps always run, further
+requested warm-up and timed steps only while the measured step time says
+they fit beside the CPU leg.  `steps` / `warmup` in the JSON line are the
+steps actually run; `steps_requested` / `warmup_requested` what was asked.
+The headline line (value, roofline, cpu_baseline) is printed and flushed as
+soon as the FP64 and CPU legs are done; the secondary legs (direct SR, the
+complex batched Pfaffian, the precision form) run afterwards within what is
+left of the budget, each marked skipped when it does not fit, and a final
+line repeats the headline with their results merged in.
 
 value  = saved samples / second of the whole timed SR steps (all walkers).
 roofline: dominant kernel = the sampling kernel k_block; algorithmic bytes
 per launch = proposals * NQP * 2n*8 (inverse row + Slater row) + accepted
 moves * NQP * 2n^2*8 (inverse read + write), divided by the launch's
-average device time (HIP events on the kernel's stream).
+average device time (HIP events on the kernel's stream).  traffic = HBM
+bytes per launch from the profiled counter passes recorded under profiles/
+(bytes per proposal and per accepted move), applied to this run's counts.
 cpu_baseline: the FP64 oracle (same algorithm and operation order) on a
-bounded walker subset of the same workload, one thread per walker, timed
-over at least 10 s of steady SR steps.
-precision_forms.mixed_fp32_inverse: the same run (inputs, seeds, steps) with
-the sampler's resident inverse in FP32 (mvmc_ens_set_precision 1); its
-samples/s, speed-up over the FP64 line and |dE/site| against the FP64 rows
-(Monte Carlo size once a chain's decisions differ; the FP64 standard error
-is reported beside it).
+bounded walker subset of the same workload, one thread per walker on up to
+16 host cores, timed over at least 10 s of steady SR steps.
 """
-import argparse
-import json
-
-import numpy as np
-import os
-import sys
-import tempfile
 import time
+
+T_START = time.time()
+
+import argparse  # noqa: E402
+import json  # noqa: E402
+import os  # noqa: E402
+import sys  # noqa: E402
+import tempfile  # noqa: E402
+
+import numpy as np  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for sub in ("mvmc-rs_amd", "oracle"):
     sys.path.insert(0, os.path.join(ROOT, sub))
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+METRIC = "Metropolis samples/sec + SR-step s, 16x16 Hubbard Nsize 224; |dE/site| vs FP64"
 
 
 def log(msg):
-    print(msg, file=sys.stderr, flush=True)
+    print(f"[{time.time() - T_START:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic(workload, walkers, steps, warmup):
-    """HBM bytes per k_block launch of the last timed step from the two PMC passes
-    (FETCH_SIZE, WRITE_SIZE, each scaled by a 1 GiB copy) recorded under profiles/
-    for this workload and schedule; None when no record matches this run."""
+def elapsed():
+    return time.time() - T_START
+
+
+def host_info():
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return {"nproc": os.cpu_count(), "cores_available": avail, "cpu_model": model}
+
+
+def traffic_model(workload):
+    """HBM traffic model of k_block recorded under profiles/ from the two PMC
+    passes (FETCH_SIZE, WRITE_SIZE scaled by a device copy of known size):
+    bytes per proposal and per accepted move fitted over the profiled
+    launches, or the profiled traffic / algorithmic ratio."""
     path = os.path.join(ROOT, "profiles", f"k_block_{workload}_traffic.json")
     try:
-        rec = json.load(open(path))
+        return json.load(open(path))
     except (OSError, ValueError):
         return None
-    if (rec.get("walkers"), rec.get("steps"), rec.get("warmup")) != (walkers, steps, warmup):
+
+
+def traffic_per_launch(model, props, accs, launches, alg_bytes):
+    if model is None or launches <= 0:
         return None
-    return rec.get("traffic_bytes_per_launch")
+    if "bytes_per_proposal" in model and "bytes_per_accept" in model:
+        return (model["bytes_per_proposal"] * props + model["bytes_per_accept"] * accs) / launches
+    if "traffic_over_algorithmic" in model:
+        return model["traffic_over_algorithmic"] * alg_bytes / launches
+    return None
 
 
 def cpu_baseline(workload_dir, walkers, threads, min_seconds=10.0, max_steps=60):
@@ -74,39 +115,48 @@ def cpu_baseline(workload_dir, walkers, threads, min_seconds=10.0, max_steps=60)
         steps += 1
     t2 = time.time()
     samples = walkers * run.nsample * steps
+    hi = host_info()
     return {"value": samples / (t2 - t1), "unit": "samples/s", "cores": threads, "kind": "port",
             "sample": f"{walkers} walkers of the same workload, {steps} steady-state SR steps after 1 warm-up step "
-                      f"(warm-up {t1 - t0:.1f} s, timed {t2 - t1:.1f} s), one thread per walker"}
+                      f"(warm-up {t1 - t0:.1f} s, timed {t2 - t1:.1f} s), one thread per walker on {threads} cores "
+                      f"of {hi['cores_available']} available ({hi['nproc']} in the machine), {hi['cpu_model']}",
+            "host": hi}
 
 
-def mixed_form(info, walkers, args, rows64, total64):
+def mixed_form(info, walkers, warmup, steps, rows64, total64, deadline):
     """Warm-up and timed steps of the FP64 run repeated with the sampler's
     inverse in FP32 (same inputs and seeds); |dE/site| per timed step against
-    the FP64 run's zvo rows."""
+    the FP64 run's zvo rows.  Stops at `deadline` (after at least one timed
+    step)."""
     import mvmc_hip
     ens = mvmc_hip.Ensemble(info["namelist"], info["initial"], walkers=walkers)
     ens.set_precision("mixed")
-    for s in range(args.warmup):
+    for s in range(warmup):
         t = time.time()
         ens.opt_step()
         log(f"mixed warmup step {s}: {time.time() - t:.1f} s")
     tm0 = ens.timings()
     times, rows = [], []
-    for s in range(args.steps):
+    for s in range(steps):
+        if s >= 1 and time.time() + max(times) > deadline:
+            break
         t = time.time()
         rows.append(ens.opt_step())
         times.append(time.time() - t)
         log(f"mixed step {s}: E/site = {rows[-1][0] / info['nsite']:.8f}  {times[-1]:.2f} s")
     tm1 = ens.timings()
     total = sum(times)
+    nsample = ens.nsample
     ens.close()
+    k = len(times)
     de = [abs(r[0] - r64[0]) / info["nsite"] for r, r64 in zip(rows, rows64)]
     # Monte Carlo standard error of E/site of the FP64 step (zvo row: Etot, ., Etot2, ...):
     # once a decision flips, the chains differ and dE is of this size
-    se = [float(np.sqrt(max(r64[2] - r64[0] ** 2, 0.0) / (walkers * ens.nsample))) / info["nsite"] for r64 in rows64]
-    return {"value": walkers * ens.nsample * args.steps / total, "unit": "samples/s",
-            "speedup_vs_fp64": total64 / total, "sr_step_s": total / args.steps,
-            "sampling_s_per_step": (tm1["sample_s"] - tm0["sample_s"]) / args.steps,
+    se = [float(np.sqrt(max(r64[2] - r64[0] ** 2, 0.0) / (walkers * nsample))) / info["nsite"] for r64 in rows64[:k]]
+    t64 = total64[:k]
+    return {"value": walkers * nsample * k / total, "unit": "samples/s", "steps": k,
+            "speedup_vs_fp64": sum(t64) / total, "sr_step_s": total / k,
+            "sampling_s_per_step": (tm1["sample_s"] - tm0["sample_s"]) / k,
             "energy_per_site": [r[0] / info["nsite"] for r in rows], "abs_dE_per_site_vs_fp64": de,
             "fp64_stat_err_per_site": se,
             "arithmetic": "sampler inverse stored in FP32, its rank-2 updates in FP32; SR-CG samples stored in "
@@ -142,7 +192,7 @@ def complex_pfinv(n=144, count=8192, cpu_planes=16):
         assert int((st != 0).sum()) == 0
         res[name] = {"ms": ms, "planes_per_s": count / (ms / 1e3),
                      "achieved_TFLOPs": 8.0 * n ** 3 * count / (ms / 1e3) / 1e12,
-                     "frac_fp64": 8.0 * n ** 3 * count / (ms / 1e3) / 1e12 / 78.6}
+                     "frac_fp64": 8.0 * n ** 3 * count / (ms / 1e3) / 1e12 / FP64_PEAK_TFLOPS}
     host = a[:cpu_planes].cpu().numpy()
     del a, inv
     t = time.perf_counter()
@@ -155,15 +205,18 @@ def complex_pfinv(n=144, count=8192, cpu_planes=16):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2, help="timed SR steps requested (at least 2 always run)")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed SR steps requested (at least 1 always runs)")
+    ap.add_argument("--budget", type=float, default=420.0,
+                    help="wall seconds from process start for the whole run; optional steps and legs that do "
+                         "not fit are left out")
     ap.add_argument("--workload", default="square16")
     ap.add_argument("--vmc-warmup", type=int, default=2,
                     help="NVMCWarmUp of the generated workload: thermalisation sweeps of the first (untimed) "
                          "SR step only; steady-state steps do not depend on it")
     ap.add_argument("--walkers", type=int, default=0, help="override (not a valid headline number)")
-    ap.add_argument("--cpu-walkers", type=int, default=16)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="oracle threads (one walker each); default: the cores available, at most 16")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-direct", action="store_true", help="skip the direct-SR measurement")
     ap.add_argument("--no-c128", action="store_true", help="skip the complex batched Pfaffian measurement")
@@ -172,6 +225,9 @@ def main():
                          "not the benchmark line)")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the precision-form run (sampler inverse in FP32) reported beside the FP64 line")
+    ap.add_argument("--launch-log", action="store_true",
+                    help="print every k_block launch of the timed steps (device ms, proposals, accepted, walkers) "
+                         "to stderr, for the per-launch traffic fit of a profiled run")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before the run, copy a buffer of this many MiB once (device-to-device), the known-size "
                          "kernel that scales FETCH_SIZE / WRITE_SIZE counters in a profiler run")
@@ -186,15 +242,13 @@ def main():
         torch.cuda.init()
         print(json.dumps({"k_pfinv_c128": complex_pfinv()}), flush=True)
         return
-    if args.copy_probe_mib > 0 or not args.no_c128:
-        # torch ships its own HIP runtime under the same sonames: it has to
-        # bring the device up before this library's first HIP call
-        import torch
-        torch.cuda.init()
+    # torch ships its own HIP runtime under the same sonames: it has to bring
+    # the device up before this library's first HIP call
+    import torch
+    torch.cuda.init()
     import lattice
     import mvmc_hip
     if args.copy_probe_mib > 0:
-        import torch
         nel = args.copy_probe_mib * (1 << 20) // 8
         src = torch.ones(nel, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
@@ -202,21 +256,41 @@ def main():
         torch.cuda.synchronize()
         log(f"copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written")
         del src, dst
+    hi = host_info()
+    cpu_threads = args.cpu_threads or max(1, min(16, hi["cores_available"]))
+    cpu_reserve = 0.0 if args.no_cpu else 45.0     # oracle warm-up step + >= 10 s timed + set-up
+    margin = 15.0
+    deadline = T_START + args.budget
+
     wl_dir = os.path.join(tempfile.gettempdir(), f"mvmc_bench_{args.workload}")
-    info = lattice.write_workload(args.workload, wl_dir, sr_steps=args.steps + args.warmup, warmup=args.vmc_warmup)
+    info = lattice.write_workload(args.workload, wl_dir, sr_steps=max(args.steps, 2) + max(args.warmup, 1),
+                                  warmup=args.vmc_warmup)
     walkers = args.walkers or info["walkers"]
     log(f"workload {args.workload}: Nsite={info['nsite']} Nsize={info['nsize']} NQP={info['nqp']} "
         f"NPara={info['npara']} walkers={walkers}")
     t = time.time()
     ens = mvmc_hip.Ensemble(info["namelist"], info["initial"], walkers=walkers)
     log(f"ensemble created in {time.time() - t:.1f} s (grid {ens.grid}, LDS {ens.lds_sampler} B)")
-    for s in range(args.warmup):
+
+    # ---- FP64 leg: warm-up steps, then timed steps, within the budget
+    warm_times = []
+    for s in range(max(args.warmup, 1)):
+        est = warm_times[-1] if warm_times else 0.0
+        if s >= 1 and time.time() + est * 3 + cpu_reserve + margin > deadline:
+            log(f"warm-up stops after {s} steps (budget)")
+            break
         t = time.time()
         row = ens.opt_step()
-        log(f"warmup step {s}: E/site = {row[0] / info['nsite']:.8f}  ({time.time() - t:.1f} s)")
+        warm_times.append(time.time() - t)
+        log(f"warmup step {s}: E/site = {row[0] / info['nsite']:.8f}  ({warm_times[-1]:.1f} s)")
+    first_step_s = warm_times[0]
     tm0 = ens.timings()
-    step_times, rows, kernel = [], [], []
-    for s in range(args.steps):
+    step_times, rows, kernel, launch_logs = [], [], [], []
+    for s in range(max(args.steps, 2)):
+        est = step_times[-1] if step_times else (warm_times[-1] if len(warm_times) > 1 else first_step_s)
+        if s >= 2 and time.time() + est + cpu_reserve + margin > deadline:
+            log(f"timed steps stop after {s} (budget)")
+            break
         before = ens.timings()
         t = time.time()
         row = ens.opt_step()
@@ -226,25 +300,39 @@ def main():
         rows.append(row)
         kernel.append((after["block_ms"], after["block_launches"], after["proposals"] - before["proposals"],
                        after["accepted"] - before["accepted"]))
+        ll = ens.launch_log()
+        launch_logs.append(ll)
         log(f"step {s}: E/site = {row[0] / info['nsite']:.8f}  {dt:.2f} s  (sampling {after['sample_s'] - before['sample_s']:.2f} s, "
-            f"measure {after['measure_s'] - before['measure_s']:.2f} s, SR {after['sr_s'] - before['sr_s']:.2f} s)")
+            f"measure {after['measure_s'] - before['measure_s']:.2f} s, SR {after['sr_s'] - before['sr_s']:.2f} s; "
+            f"k_block launches {len(ll)}, walkers per launch {[int(x) for x in ll[:, 3]]})")
+        if args.launch_log:
+            for i, r in enumerate(ll):
+                log(f"launch_log step {s} launch {i}: ms {r[0]:.3f} proposals {int(r[1])} accepted {int(r[2])} "
+                    f"walkers {int(r[3])}")
     tm1 = ens.timings()
+    nsteps = len(step_times)
     total = sum(step_times)
-    samples = walkers * ens.nsample * args.steps
+    samples = walkers * ens.nsample * nsteps
     n, nqp = ens.n, ens.nqp
-    # roofline of the sampling kernel (last timed step's launches)
-    block_ms, launches, props, accs = kernel[-1]
+    # roofline of the sampling kernel over the timed steps' launches
+    launches = sum(int(k[1]) for k in kernel)
+    block_ms = sum(k[0] for k in kernel)
+    props = sum(k[2] for k in kernel)
+    accs = sum(k[3] for k in kernel)
     bytes_total = props * nqp * 2 * n * 8 + accs * nqp * 2 * n * n * 8
     avg_launch_s = block_ms / 1e3 / max(launches, 1)
     achieved = bytes_total / max(launches, 1) / avg_launch_s / 1e9
+    tmodel = traffic_model(args.workload)
     out = {
-        "metric": "Metropolis samples/sec + SR-step s, 16x16 Hubbard Nsize 224; |dE/site| vs FP64",
+        "metric": METRIC,
         "value": samples / total,
         "unit": "samples/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * total / args.steps,
+        "steps": nsteps,
+        "warmup": len(warm_times),
+        "steps_requested": args.steps,
+        "warmup_requested": args.warmup,
+        "ms_per_step": 1e3 * total / nsteps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -254,20 +342,28 @@ def main():
                                f"NQP {nqp}, NPara {info['npara']}, {walkers} walkers, 1 sample/walker, "
                                f"16 sweeps/sample, SR-CG",
                    "walkers": walkers, "nsize": n, "nqp": nqp, "npara": info["npara"], "parallelism": "walkers"},
-        "sr_step_s": total / args.steps,
-        "sampling_s_per_step": (tm1["sample_s"] - tm0["sample_s"]) / args.steps,
-        "measure_s_per_step": (tm1["measure_s"] - tm0["measure_s"]) / args.steps,
-        "sr_cg_s_per_step": (tm1["sr_s"] - tm0["sr_s"]) / args.steps,
+        "sr_step_s": total / nsteps,
+        "sr_step_s_each": step_times,
+        "first_warmup_step_s": first_step_s,
+        "sampling_s_per_step": (tm1["sample_s"] - tm0["sample_s"]) / nsteps,
+        "measure_s_per_step": (tm1["measure_s"] - tm0["measure_s"]) / nsteps,
+        "sr_cg_s_per_step": (tm1["sr_s"] - tm0["sr_s"]) / nsteps,
         "proposals_per_s": (tm1["proposals"] - tm0["proposals"]) / total,
         "acceptance": accs / max(props, 1),
         "energy_per_site": [r[0] / info["nsite"] for r in rows],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload, walkers, args.steps, args.warmup),
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_per_launch(tmodel, props, accs, launches, bytes_total),
                      "kernel": "k_block", "avg_launch_ms": avg_launch_s * 1e3, "launches": launches,
-                     "algorithmic_bytes_per_launch": bytes_total / max(launches, 1)},
+                     "algorithmic_bytes_per_launch": bytes_total / max(launches, 1),
+                     "walkers_per_launch": [[int(x) for x in ll[:, 3]] for ll in launch_logs]},
     }
-    # the other rows' kernels in the timed step, against their algorithmic bytes
-    # (HBM 8 TB/s) and flops (FP64 vector 78.6 TFLOP/s)
+    if tmodel is not None:
+        out["roofline"]["traffic_model"] = {k: tmodel[k] for k in ("bytes_per_proposal", "bytes_per_accept",
+                                                                   "traffic_over_algorithmic", "source")
+                                            if k in tmodel}
+    # the other rows' kernels in the last timed step, against their algorithmic
+    # bytes (HBM 8 TB/s) and flops (FP64 vector 78.6 TFLOP/s)
     tl = ens.timings()
     plane_b = 2 * n * n * 8 + 12            # read A, write invM (+ status, Pf)
     plane_f = 2 * n ** 3                   # LTL (2/3 n^3) + trtri (1/3) + trmm (1)
@@ -278,7 +374,7 @@ def main():
             sec[key] = {"planes": cnt * nqp, "ms": ms, "achieved_GBs": cnt * nqp * plane_b / (ms / 1e3) / 1e9,
                         "frac_hbm": cnt * nqp * plane_b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
                         "achieved_TFLOPs": cnt * nqp * plane_f / (ms / 1e3) / 1e12,
-                        "frac_fp64": cnt * nqp * plane_f / (ms / 1e3) / 1e12 / 78.6}
+                        "frac_fp64": cnt * nqp * plane_f / (ms / 1e3) / 1e12 / FP64_PEAK_TFLOPS}
     if tl["measure_ms"] > 0:
         mb = tl["measured"] * nqp * (n * n + info["ntransfer"] * n) * 8
         sec["k_measure (c): Green ratios + O"] = {"samples": tl["measured"], "ms": tl["measure_ms"],
@@ -289,36 +385,59 @@ def main():
         sec["k_cg_* (d): SR-CG"] = {"iterations": tl["cg_iter"], "ms": tl["cg_ms"],
                                     "achieved_GBs": cb / (tl["cg_ms"] / 1e3) / 1e9,
                                     "frac_hbm": cb / (tl["cg_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS}
-    # direct SR (NSRCG 0 path) on the same measured operands, after the timed
-    # steps: Gram by DGEMM on the matrix cores + Cholesky
-    if not args.no_direct:
-        ens.sr_direct()   # first call: rocBLAS / rocSOLVER initialisation
-        ens.sr_direct()
-        td = ens.timings()
-        na_, rows_ = td["cg_na"], td["cg_rows"]
-        fl = 2.0 * na_ * na_ * rows_ + na_ ** 3 / 3.0 + 2.0 * na_ * na_   # DGEMM (full Gram) + Cholesky
-        sec["direct SR (d, NSRCG 0): DGEMM + DPOTRF/DPOTRS"] = {
-            "active_parameters": na_, "rows": rows_, "ms": td["cg_ms"],
-            "achieved_TFLOPs": fl / (td["cg_ms"] / 1e3) / 1e12,
-            "frac_fp64_mfma": fl / (td["cg_ms"] / 1e3) / 1e12 / 78.6}
-    if not args.no_c128:
-        sec["k_pfinv_c128 (a, complex planes of the 12x12 flux lattice)"] = complex_pfinv()
-    out["secondary_rooflines"] = sec
     out["autotuned"] = {"rebuild_variant": ["3 workgroups/CU, 8-column close batches",
                                             "2 workgroups/CU, 16-column close batches",
                                             "not decided"][tl["rebuild_variant"] if tl["rebuild_variant"] >= 0 else 2],
                         "rebuild_ms_per_walker_measured": list(tl["rebuild_ms_per_walker"])}
-    if not args.no_mixed:
-        # precision form (1): the same workload, seeds and step count with the
-        # sampler's resident inverse in FP32; energies against the FP64 rows
-        ens.close()
-        del ens
-        out["precision_forms"] = {"mixed_fp32_inverse": mixed_form(info, walkers, args, rows, total)}
+    out["secondary_rooflines"] = sec
+
+    # ---- CPU leg (before the headline line, which carries it)
     if not args.no_cpu:
         try:
-            out["cpu_baseline"] = cpu_baseline(wl_dir, args.cpu_walkers, args.cpu_threads)
+            out["cpu_baseline"] = cpu_baseline(wl_dir, cpu_threads, cpu_threads)
         except Exception as e:  # the baseline is reported, never the target
             out["cpu_baseline"] = {"value": None, "error": str(e)}
+        log(f"cpu baseline: {out['cpu_baseline'].get('value')} samples/s")
+    out["wall_s_at_headline"] = elapsed()
+    print(json.dumps(out), flush=True)
+
+    # ---- secondary legs, each only if it fits in what is left of the budget
+    skipped = {}
+    if not args.no_direct:
+        if time.time() + 30 + margin < deadline:
+            # direct SR (NSRCG 0 path) on the same measured operands: Gram by
+            # DGEMM on the matrix cores + Cholesky
+            ens.sr_direct()   # first call: rocBLAS / rocSOLVER initialisation
+            ens.sr_direct()
+            td = ens.timings()
+            na_, rows_ = td["cg_na"], td["cg_rows"]
+            fl = 2.0 * na_ * na_ * rows_ + na_ ** 3 / 3.0 + 2.0 * na_ * na_   # DGEMM (full Gram) + Cholesky
+            sec["direct SR (d, NSRCG 0): DGEMM + DPOTRF/DPOTRS"] = {
+                "active_parameters": na_, "rows": rows_, "ms": td["cg_ms"],
+                "achieved_TFLOPs": fl / (td["cg_ms"] / 1e3) / 1e12,
+                "frac_fp64_mfma": fl / (td["cg_ms"] / 1e3) / 1e12 / FP64_PEAK_TFLOPS}
+        else:
+            skipped["direct_sr"] = "budget"
+    ens.close()
+    del ens
+    if not args.no_c128:
+        if time.time() + 30 + margin < deadline:
+            sec["k_pfinv_c128 (a, complex planes of the 12x12 flux lattice)"] = complex_pfinv()
+            torch.cuda.empty_cache()
+        else:
+            skipped["k_pfinv_c128"] = "budget"
+    if not args.no_mixed:
+        # precision form (1): the same workload, seeds and warm-up with the
+        # sampler's resident inverse in FP32; energies against the FP64 rows
+        need = first_step_s + sum(warm_times[1:]) + 2 * (total / nsteps) + margin
+        if time.time() + need < deadline:
+            out["precision_forms"] = {"mixed_fp32_inverse": mixed_form(
+                info, walkers, len(warm_times), nsteps, rows, step_times, deadline - margin)}
+        else:
+            skipped["precision_form"] = "budget"
+    if skipped:
+        out["skipped_legs"] = skipped
+    out["wall_s"] = elapsed()
     print(json.dumps(out), flush=True)
 
 

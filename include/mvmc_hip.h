@@ -199,6 +199,13 @@ int mvmc_ens_timings(mvmc_ens* e, double* timings20);
  * MVMC_E_UNAVAILABLE.  No reference counterpart (profiling aid, like the
  * reference's timer sections in crates/mvmc-core/src/timer.rs). */
 int mvmc_ens_phase_clocks(mvmc_ens* e, long long* clocks32);
+/* Per-launch record of the sampling kernel over the last mvmc_ens_sample
+ * call: 4 doubles per k_block launch (device ms by HIP events on the
+ * library's stream, proposals, accepted moves, walkers that advanced in the
+ * launch).  Writes min(rows, max_rows) rows to out and returns the number of
+ * launches.  No reference counterpart (profiling aid: the launch tail and
+ * the per-launch traffic fit of the benchmark). */
+int mvmc_ens_launch_log(mvmc_ens* e, double* out, int max_rows);
 /* Sticky device status word (0 ok; 1 init retries exhausted, 2 bad input,
  * 3 rebuild failure, 4 non-finite, 5 CG failure). */
 int mvmc_ens_status(mvmc_ens* e);

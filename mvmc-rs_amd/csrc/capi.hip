@@ -263,9 +263,12 @@ T* dalloc(std::vector<void*>& list, size_t count) {
 }
 
 template <class T>
-T* dupload(std::vector<void*>& list, const std::vector<T>& h) {
+T* dupload(std::vector<void*>& list, const std::vector<T>& h, hipStream_t st) {
   T* p = dalloc<T>(list, h.size());
-  if (!h.empty()) HIPCHK(hipMemcpy(p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+  if (!h.empty()) {
+    HIPCHK(hipMemcpyAsync(p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
   return p;
 }
 
@@ -318,6 +321,9 @@ struct mvmc_ens {
   double t_sample = 0, t_measure = 0, t_sr = 0, proposals = 0, accepted = 0;
   float block_ms = 0;
   int block_launches = 0;
+  // per k_block launch of the last sample call: device ms, proposals,
+  // accepted moves and walkers that advanced in it
+  std::vector<double> launch_log;
   // precision form: 0 FP64 throughout; 1 the sampler's inverse stored in
   // FP32 (inv32), everything else FP64
   int precision = 0;
@@ -350,9 +356,16 @@ struct mvmc_ens {
     HIPCHK(hipMemcpyAsync(d_dj, j.data(), sizeof(double) * j.size(), hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
   }
+  // Device-to-host copy ordered on the handle's own (non-blocking) stream:
+  // never on the legacy stream, which would wait on (and break) another
+  // handle's stream capture in the same process.
+  void d2h(void* dst, const void* src, size_t bytes) {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
   void check_status() {
     int s = 0;
-    HIPCHK(hipMemcpy(&s, v.status, sizeof(int), hipMemcpyDeviceToHost));
+    d2h(&s, v.status, sizeof(int));
     if (s == ST_BAD_INPUT || s == ST_INIT_RETRY)
       throw Fail(MVMC_E_NUMERIC, "device status " + std::to_string(s));
   }
@@ -431,12 +444,12 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
   // model tables
   E->d_slater = dalloc<double>(A, (size_t)v.nqp * v.n2 * v.n2);
   v.slater = E->d_slater;
-  v.qpw = dupload(A, m.qp_full_weight);
+  v.qpw = dupload(A, m.qp_full_weight, E->st);
   E->d_proj = dalloc<double>(A, std::max(v.nproj, 1));
   v.proj = E->d_proj;
-  v.gutz_idx = dupload(A, m.gutz_idx);
-  v.jast_idx = dupload(A, m.jast_idx);
-  v.loc_spn = dupload(A, m.loc_spn);
+  v.gutz_idx = dupload(A, m.gutz_idx, E->st);
+  v.jast_idx = dupload(A, m.jast_idx, E->st);
+  v.loc_spn = dupload(A, m.loc_spn, E->st);
   {
     std::vector<int> t;
     std::vector<double> tv;
@@ -445,21 +458,21 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
       tv.push_back(x.re);
     }
     v.ntrans = (int)m.transfer.size();
-    v.trans = dupload(A, t);
-    v.trans_v = dupload(A, tv);
+    v.trans = dupload(A, t, E->st);
+    v.trans_v = dupload(A, tv, E->st);
     std::vector<int> is;
     std::vector<double> iv;
     for (auto& x : m.coulomb_intra) { is.push_back(x.first); iv.push_back(x.second); }
     v.nintra = (int)is.size();
-    v.intra_site = dupload(A, is);
-    v.intra_v = dupload(A, iv);
+    v.intra_site = dupload(A, is, E->st);
+    v.intra_v = dupload(A, iv, E->st);
     auto pairs = [&](const std::vector<mvmc::PairTerm>& P, const int*& site, const double*& val, int& cnt) {
       std::vector<int> s;
       std::vector<double> vv;
       for (auto& x : P) { s.push_back(x.site1); s.push_back(x.site2); vv.push_back(x.value); }
       cnt = (int)P.size();
-      site = dupload(A, s);
-      val = dupload(A, vv);
+      site = dupload(A, s, E->st);
+      val = dupload(A, vv, E->st);
     };
     pairs(m.coulomb_inter, v.inter_site, v.inter_v, v.ninter);
     pairs(m.hund, v.hund_site, v.hund_v, v.nhund);
@@ -472,14 +485,14 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
   {
     std::vector<int> ts, tg;
     mvmc::translation_maps(m, ts, tg);
-    v.tsite = dupload(A, ts);
-    v.tsgn = dupload(A, tg);
+    v.tsite = dupload(A, ts, E->st);
+    v.tsgn = dupload(A, tg, E->st);
   }
-  v.orb_idx = dupload(A, m.orb_idx);
-  v.orb_sgn = dupload(A, m.orb_sgn);
-  v.spgl_cs = dupload(A, m.spgl_cos_sin);
-  v.spgl_cc = dupload(A, m.spgl_cos_cos);
-  v.spgl_ss = dupload(A, m.spgl_sin_sin);
+  v.orb_idx = dupload(A, m.orb_idx, E->st);
+  v.orb_sgn = dupload(A, m.orb_sgn, E->st);
+  v.spgl_cs = dupload(A, m.spgl_cos_sin, E->st);
+  v.spgl_cc = dupload(A, m.spgl_cos_cos, E->st);
+  v.spgl_ss = dupload(A, m.spgl_sin_sin, E->st);
   // walker state
   const size_t W = (size_t)walkers;
   v.rng = dalloc<uint32_t>(A, W * 625);
@@ -502,13 +515,13 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
   v.sz = dalloc<double>(A, WS);
   v.o_store = dalloc<double>(A, WS * v.size);
   v.status = dalloc<int>(A, 1);
-  HIPCHK(hipMemset(v.status, 0, sizeof(int)));
-  HIPCHK(hipMemset(v.wstate, 0, sizeof(int) * W * 8));
-  HIPCHK(hipMemset(v.counter, 0, sizeof(long long) * W * 4));
-  HIPCHK(hipMemset(v.ele_idx, 0, sizeof(int) * W * v.n));
-  HIPCHK(hipMemset(v.ele_cfg, 0, sizeof(int) * W * v.n2));
-  HIPCHK(hipMemset(v.ele_num, 0, sizeof(int) * W * v.n2));
-  HIPCHK(hipMemset(v.proj_cnt, 0, sizeof(int) * W * std::max(v.nproj, 1)));
+  HIPCHK(hipMemsetAsync(v.status, 0, sizeof(int), E->st));
+  HIPCHK(hipMemsetAsync(v.wstate, 0, sizeof(int) * W * 8, E->st));
+  HIPCHK(hipMemsetAsync(v.counter, 0, sizeof(long long) * W * 4, E->st));
+  HIPCHK(hipMemsetAsync(v.ele_idx, 0, sizeof(int) * W * v.n, E->st));
+  HIPCHK(hipMemsetAsync(v.ele_cfg, 0, sizeof(int) * W * v.n2, E->st));
+  HIPCHK(hipMemsetAsync(v.ele_num, 0, sizeof(int) * W * v.n2, E->st));
+  HIPCHK(hipMemsetAsync(v.proj_cnt, 0, sizeof(int) * W * std::max(v.nproj, 1), E->st));
   // persistent grids sized to the resident capacity
   E->lds_samp = lds_bytes_sampler(v, 256);
   E->lds_meas = lds_bytes_measure(v, 256);
@@ -556,7 +569,7 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
   v.rlist = dalloc<int>(A, (size_t)walkers);
 #ifdef MVMC_PHASE_CLOCKS
   v.clk = dalloc<long long>(A, (size_t)E->slots * 32);
-  HIPCHK(hipMemset(v.clk, 0, sizeof(long long) * E->slots * 32));
+  HIPCHK(hipMemsetAsync(v.clk, 0, sizeof(long long) * E->slots * 32, E->st));
 #else
   v.clk = nullptr;
 #endif
@@ -617,6 +630,9 @@ void sample_impl(mvmc_ens* E) {
   double block_ms = 0, rebuild_ms = 0;
   int launches = 0, rebuilds = 0;
   long long rebuilt = 0;
+  E->launch_log.clear();
+  std::vector<long long> cnt_prev((size_t)v.W * 4), cnt_now((size_t)v.W * 4);
+  E->d2h(cnt_prev.data(), v.counter, sizeof(long long) * cnt_prev.size());
   for (int out = 0; out < n_out; ++out) {
     hipLaunchKernelGGL(k_reset_resume, dim3(nblk(v.W)), dim3(256), 0, E->st, v);
     for (;;) {
@@ -635,6 +651,19 @@ void sample_impl(mvmc_ens* E) {
       HIPCHK(hipEventElapsedTime(&ms, e0, e1));
       block_ms += ms;
       ++launches;
+      {
+        E->d2h(cnt_now.data(), v.counter, sizeof(long long) * cnt_now.size());
+        double props = 0, accs = 0, active = 0;
+        for (int w = 0; w < v.W; ++w) {
+          const size_t b = (size_t)w * 4;
+          const long long dp = cnt_now[b] + cnt_now[b + 2] - cnt_prev[b] - cnt_prev[b + 2];
+          props += (double)dp;
+          accs += (double)(cnt_now[b + 1] + cnt_now[b + 3] - cnt_prev[b + 1] - cnt_prev[b + 3]);
+          active += dp > 0 ? 1.0 : 0.0;
+        }
+        E->launch_log.insert(E->launch_log.end(), {(double)ms, props, accs, active});
+        cnt_prev.swap(cnt_now);
+      }
       if (nf == 0) break;
       rebuilt += nf;
       // variant: the tuned one, else alternate on launches that fill every
@@ -664,7 +693,7 @@ void sample_impl(mvmc_ens* E) {
   E->rebuilt = (double)rebuilt;
   E->check_status();
   std::vector<long long> c((size_t)v.W * 4);
-  HIPCHK(hipMemcpy(c.data(), v.counter, sizeof(long long) * c.size(), hipMemcpyDeviceToHost));
+  E->d2h(c.data(), v.counter, sizeof(long long) * c.size());
   for (int w = 0; w < v.W; ++w) {
     E->proposals += (double)(c[(size_t)w * 4 + 0] + c[(size_t)w * 4 + 2]);
     E->accepted += (double)(c[(size_t)w * 4 + 1] + c[(size_t)w * 4 + 3]);
@@ -962,7 +991,7 @@ int direct_sr(mvmc_ens* E, const std::vector<int>& map, int na, size_t rows, dou
     return 1;
   }
   std::vector<double> x(na);
-  HIPCHK(hipMemcpy(x.data(), E->d_cx, sizeof(double) * na, hipMemcpyDeviceToHost));
+  E->d2h(x.data(), E->d_cx, sizeof(double) * na);
   E->cg_x = x;
   for (double xv : x)
     if (!std::isfinite(xv)) {
@@ -1073,7 +1102,7 @@ int sr_impl(mvmc_ens* E, long long* info_out, bool force_direct = false) {
     }
   }
   std::vector<double> x(na);
-  HIPCHK(hipMemcpy(x.data(), E->d_cx, sizeof(double) * na, hipMemcpyDeviceToHost));
+  E->d2h(x.data(), E->d_cx, sizeof(double) * na);
   E->cg_x = x;
   E->cg_info = info;
   *info_out = info;
@@ -1341,8 +1370,7 @@ int mvmc_ens_get_o_store(mvmc_ens* e, int w, double* o) {
   return guarded([&] {
     const EnsView& v = e->v;
     if (w < 0 || w >= v.W) throw Fail(MVMC_E_INVALID_SHAPE, "walker out of range");
-    HIPCHK(hipMemcpy(o, v.o_store + (size_t)w * v.nsample * v.size, sizeof(double) * v.nsample * v.size,
-                     hipMemcpyDeviceToHost));
+    e->d2h(o, v.o_store + (size_t)w * v.nsample * v.size, sizeof(double) * v.nsample * v.size);
   });
 }
 
@@ -1351,10 +1379,9 @@ int mvmc_ens_get_walker(mvmc_ens* e, int w, int* saved, long long* counters, uin
     const EnsView& v = e->v;
     if (w < 0 || w >= v.W) throw Fail(MVMC_E_INVALID_SHAPE, "walker out of range");
     if (saved)
-      HIPCHK(hipMemcpy(saved, v.s_idx + (size_t)w * v.nsample * v.n, sizeof(int) * v.nsample * v.n,
-                       hipMemcpyDeviceToHost));
-    if (counters) HIPCHK(hipMemcpy(counters, v.counter + (size_t)w * 4, sizeof(long long) * 4, hipMemcpyDeviceToHost));
-    if (rng) HIPCHK(hipMemcpy(rng, v.rng + (size_t)w * 625, sizeof(uint32_t) * 625, hipMemcpyDeviceToHost));
+      e->d2h(saved, v.s_idx + (size_t)w * v.nsample * v.n, sizeof(int) * v.nsample * v.n);
+    if (counters) e->d2h(counters, v.counter + (size_t)w * 4, sizeof(long long) * 4);
+    if (rng) e->d2h(rng, v.rng + (size_t)w * 625, sizeof(uint32_t) * 625);
   });
 }
 
@@ -1365,12 +1392,12 @@ int mvmc_ens_get_tables(mvmc_ens* e, int w, double* inv, double* pf) {
     const size_t nn = (size_t)v.n * v.n;
     if (inv && e->precision == 1) {  // the sampler's FP32 inverse, widened
       std::vector<float> h((size_t)v.nqp * nn);
-      HIPCHK(hipMemcpy(h.data(), e->inv32 + (size_t)w * v.nqp * nn, sizeof(float) * h.size(), hipMemcpyDeviceToHost));
+      e->d2h(h.data(), e->inv32 + (size_t)w * v.nqp * nn, sizeof(float) * h.size());
       for (size_t k = 0; k < h.size(); ++k) inv[k] = h[k];
     } else if (inv) {
-      HIPCHK(hipMemcpy(inv, v.inv + (size_t)w * v.nqp * nn, sizeof(double) * v.nqp * nn, hipMemcpyDeviceToHost));
+      e->d2h(inv, v.inv + (size_t)w * v.nqp * nn, sizeof(double) * v.nqp * nn);
     }
-    if (pf) HIPCHK(hipMemcpy(pf, v.pf + (size_t)w * v.nqp, sizeof(double) * v.nqp, hipMemcpyDeviceToHost));
+    if (pf) e->d2h(pf, v.pf + (size_t)w * v.nqp, sizeof(double) * v.nqp);
   });
 }
 
@@ -1378,11 +1405,11 @@ int mvmc_ens_get_config(mvmc_ens* e, int w, int* ei, int* ec, int* en, int* pc) 
   return guarded([&] {
     const EnsView& v = e->v;
     if (w < 0 || w >= v.W) throw Fail(MVMC_E_INVALID_SHAPE, "walker out of range");
-    if (ei) HIPCHK(hipMemcpy(ei, v.ele_idx + (size_t)w * v.n, sizeof(int) * v.n, hipMemcpyDeviceToHost));
-    if (ec) HIPCHK(hipMemcpy(ec, v.ele_cfg + (size_t)w * v.n2, sizeof(int) * v.n2, hipMemcpyDeviceToHost));
-    if (en) HIPCHK(hipMemcpy(en, v.ele_num + (size_t)w * v.n2, sizeof(int) * v.n2, hipMemcpyDeviceToHost));
+    if (ei) e->d2h(ei, v.ele_idx + (size_t)w * v.n, sizeof(int) * v.n);
+    if (ec) e->d2h(ec, v.ele_cfg + (size_t)w * v.n2, sizeof(int) * v.n2);
+    if (en) e->d2h(en, v.ele_num + (size_t)w * v.n2, sizeof(int) * v.n2);
     if (pc && v.nproj > 0)
-      HIPCHK(hipMemcpy(pc, v.proj_cnt + (size_t)w * v.nproj, sizeof(int) * v.nproj, hipMemcpyDeviceToHost));
+      e->d2h(pc, v.proj_cnt + (size_t)w * v.nproj, sizeof(int) * v.nproj);
   });
 }
 
@@ -1436,7 +1463,7 @@ int mvmc_ens_phase_clocks(mvmc_ens* e, long long* out) {
   return guarded([&] {
     if (!e->v.clk) throw Fail(MVMC_E_UNAVAILABLE, "library built without MVMC_PHASE_CLOCKS");
     std::vector<long long> h((size_t)e->slots * 32);
-    HIPCHK(hipMemcpy(h.data(), e->v.clk, sizeof(long long) * h.size(), hipMemcpyDeviceToHost));
+    e->d2h(h.data(), e->v.clk, sizeof(long long) * h.size());
     for (int k = 0; k < 32; ++k) out[k] = 0;
     for (int sl = 0; sl < e->slots; ++sl)
       for (int k = 0; k < 32; ++k) out[k] += h[(size_t)sl * 32 + k];
@@ -1455,9 +1482,17 @@ int mvmc_ens_timings(mvmc_ens* e, double* t) {
   return MVMC_OK;
 }
 
+int mvmc_ens_launch_log(mvmc_ens* e, double* out, int max_rows) {
+  const int rows = (int)(e->launch_log.size() / 4);
+  for (int i = 0; i < std::min(rows, max_rows) * 4; ++i) out[i] = e->launch_log[(size_t)i];
+  return rows;
+}
+
 int mvmc_ens_status(mvmc_ens* e) {
   int s = -1;
-  if (hipMemcpy(&s, e->v.status, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  if (hipMemcpyAsync(&s, e->v.status, sizeof(int), hipMemcpyDeviceToHost, e->st) != hipSuccess ||
+      hipStreamSynchronize(e->st) != hipSuccess)
+    return -1;
   return s;
 }
 

@@ -55,6 +55,7 @@ def lib():
              ctypes.POINTER(ctypes.c_longlong), ctypes.c_void_p]
         L.mvmc_sr_cg_apply.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 4 + \
             [ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
+        L.mvmc_ens_launch_log.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         L.mvmc_ens_write_opt_data.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
         L.mvmc_ens_set_output_dir.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
         L.mvmc_sr_gram_real.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
@@ -65,7 +66,7 @@ def lib():
 
 
 EXPORTED_SYMBOLS = [
-    "mvmc_last_error", "mvmc_abi_version", "mvmc_pfinv_f64", "mvmc_pfinv_f64_inplace", "mvmc_ens_create",
+    "mvmc_last_error", "mvmc_abi_version", "mvmc_ens_launch_log", "mvmc_pfinv_f64", "mvmc_pfinv_f64_inplace", "mvmc_ens_create",
     "mvmc_ens_destroy", "mvmc_ens_dims", "mvmc_ens_get_params", "mvmc_ens_set_params", "mvmc_ens_sample",
     "mvmc_ens_measure", "mvmc_ens_sr_cg", "mvmc_ens_opt_step", "mvmc_ens_get_energy", "mvmc_ens_get_operands",
     "mvmc_ens_get_o_store", "mvmc_ens_get_walker", "mvmc_ens_get_tables", "mvmc_ens_get_config", "mvmc_ens_get_cg",
@@ -268,6 +269,14 @@ class Ensemble:
         out = {k: c[i] / 1e8 for i, k in enumerate(self.PHASES)}
         out.update({k: c[16 + i] / 1e8 for i, k in enumerate(self.REBUILD_PHASES)})
         return out
+
+    def launch_log(self):
+        """Per k_block launch of the last sample call: rows of (device ms,
+        proposals, accepted moves, walkers that advanced)."""
+        n = lib().mvmc_ens_launch_log(self.h, None, 0)
+        out = np.zeros((max(n, 1), 4))
+        n = lib().mvmc_ens_launch_log(self.h, _p(out), n)
+        return out[:n]
 
     def status(self):
         return int(lib().mvmc_ens_status(self.h))
