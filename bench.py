@@ -100,17 +100,22 @@ def traffic_per_launch(model, props, accs, launches, alg_bytes):
     return None
 
 
-def cpu_baseline(workload_dir, walkers, threads, min_seconds=10.0, max_steps=60):
+def cpu_baseline(workload_dir, walkers, threads, params, min_seconds=10.0, max_steps=60):
     """Oracle leg: one untimed warm-up SR step (thermalisation), then steady SR
-    steps until at least min_seconds of CPU work are timed."""
+    steps until at least min_seconds of CPU work are timed.  Every step starts
+    from `params` (the GPU run's parameters at the start of its timed steps),
+    so both legs sample at the same acceptance rate; the oracle's own SR
+    update of a step is computed and then replaced by them."""
     import oracle_py
     run = oracle_py.Run(os.path.join(workload_dir, "namelist.def"), os.path.join(workload_dir, "initial.def"),
                         walkers=walkers, threads=threads)
     t0 = time.time()
+    run.set_params(*params)
     run.step(do_sr=True)
     t1 = time.time()
     steps = 0
     while steps < max_steps and (steps == 0 or time.time() - t1 < min_seconds):
+        run.set_params(*params)
         run.step(do_sr=True)
         steps += 1
     t2 = time.time()
@@ -118,7 +123,8 @@ def cpu_baseline(workload_dir, walkers, threads, min_seconds=10.0, max_steps=60)
     hi = host_info()
     return {"value": samples / (t2 - t1), "unit": "samples/s", "cores": threads, "kind": "port",
             "sample": f"{walkers} walkers of the same workload, {steps} steady-state SR steps after 1 warm-up step "
-                      f"(warm-up {t1 - t0:.1f} s, timed {t2 - t1:.1f} s), one thread per walker on {threads} cores "
+                      f"(warm-up {t1 - t0:.1f} s, timed {t2 - t1:.1f} s) from the GPU run's parameters at its first timed "
+                      f"step, one thread per walker on {threads} cores "
                       f"of {hi['cores_available']} available ({hi['nproc']} in the machine), {hi['cpu_model']}",
             "host": hi}
 
@@ -284,6 +290,7 @@ def main():
         warm_times.append(time.time() - t)
         log(f"warmup step {s}: E/site = {row[0] / info['nsite']:.8f}  ({warm_times[-1]:.1f} s)")
     first_step_s = warm_times[0]
+    params_at_timed = ens.params()
     tm0 = ens.timings()
     step_times, rows, kernel, launch_logs = [], [], [], []
     for s in range(max(args.steps, 2)):
@@ -394,7 +401,7 @@ def main():
     # ---- CPU leg (before the headline line, which carries it)
     if not args.no_cpu:
         try:
-            out["cpu_baseline"] = cpu_baseline(wl_dir, cpu_threads, cpu_threads)
+            out["cpu_baseline"] = cpu_baseline(wl_dir, cpu_threads, cpu_threads, params_at_timed)
         except Exception as e:  # the baseline is reported, never the target
             out["cpu_baseline"] = {"value": None, "error": str(e)}
         log(f"cpu baseline: {out['cpu_baseline'].get('value')} samples/s")

@@ -47,7 +47,8 @@ def _idx(x, y, dims):
 
 
 def write_workload(name, out_dir, seed=2024, sr_steps=10, nvmc_sample=1, interval=16, warmup=16,
-                   rnd_seed=11272, step_dt=0.01, sta_del=1e-3, red_cut=1e-10, cg_max_iter=0, overrides=None):
+                   rnd_seed=11272, step_dt=0.01, sta_del=1e-3, red_cut=1e-10, cg_max_iter=0, ex_update_path=0,
+                   overrides=None):
     cfg = dict(WORKLOADS[name])
     if overrides:
         cfg.update(overrides)
@@ -184,7 +185,7 @@ def write_workload(name, out_dir, seed=2024, sr_steps=10, nvmc_sample=1, interva
               ("DSROptRedCut", f"{red_cut:.10f}"), ("DSROptStaDel", f"{sta_del:.10f}"),
               ("DSROptStepDt", f"{step_dt:.10f}"), ("NSROptCGMaxIter", cg_max_iter),
               ("NVMCWarmUp", warmup), ("NVMCInterval", interval), ("NVMCSample", nvmc_sample),
-              ("NExUpdatePath", 0), ("RndSeed", rnd_seed), ("NSplitSize", 1), ("NStore", 0), ("NSRCG", 1),
+              ("NExUpdatePath", ex_update_path), ("RndSeed", rnd_seed), ("NSplitSize", 1), ("NStore", 0), ("NSRCG", 1),
               ("NSRCGAbortOnFail", 0)]
         for k, v in kv:
             f.write(f"{k:<14s} {v}\n")

@@ -42,6 +42,8 @@ def lib():
         L.mvmc_last_error.restype = ctypes.c_char_p
         L.mvmc_pfinv_f64.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_void_p, ctypes.c_void_p]
+        L.mvmc_pfinv_f64_inplace.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_void_p]
         L.mvmc_pfinv_c128.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.mvmc_ens_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_longlong,
@@ -66,7 +68,7 @@ def lib():
 
 
 EXPORTED_SYMBOLS = [
-    "mvmc_last_error", "mvmc_abi_version", "mvmc_ens_launch_log", "mvmc_pfinv_f64", "mvmc_pfinv_f64_inplace", "mvmc_ens_create",
+    "mvmc_last_error", "mvmc_abi_version", "mvmc_ens_launch_log" "mvmc_pfinv_f64", "mvmc_pfinv_f64_inplace", "mvmc_ens_create",
     "mvmc_ens_destroy", "mvmc_ens_dims", "mvmc_ens_get_params", "mvmc_ens_set_params", "mvmc_ens_sample",
     "mvmc_ens_measure", "mvmc_ens_sr_cg", "mvmc_ens_opt_step", "mvmc_ens_get_energy", "mvmc_ens_get_operands",
     "mvmc_ens_get_o_store", "mvmc_ens_get_walker", "mvmc_ens_get_tables", "mvmc_ens_get_config", "mvmc_ens_get_cg",
@@ -100,6 +102,16 @@ def pfaffian_inverse_batched(planes_dev, pf_dev, inv_dev, status_dev, n, count, 
     _check(lib().mvmc_pfinv_f64(ctypes.c_void_p(addr(planes_dev)), n, count, ctypes.c_void_p(addr(pf_dev)),
                                 ctypes.c_void_p(addr(inv_dev)), ctypes.c_void_p(addr(status_dev)),
                                 ctypes.c_void_p(stream or 0)))
+
+
+def pfaffian_inverse_batched_inplace(planes_dev, pf_dev, status_dev, n, count, stream=None):
+    """In-place form (pfaffian_inverse_batch_in_place, crates/mvmc-core/src/stage_backend.rs:145-155):
+    the planes are overwritten with the TRUE inverse (zero-filled where a
+    plane fails)."""
+    def addr(x):
+        return x if isinstance(x, int) else x.data_ptr()
+    _check(lib().mvmc_pfinv_f64_inplace(ctypes.c_void_p(addr(planes_dev)), n, count, ctypes.c_void_p(addr(pf_dev)),
+                                        ctypes.c_void_p(addr(status_dev)), ctypes.c_void_p(stream or 0)))
 
 
 def pfaffian_inverse_batched_c128(planes_dev, pf_dev, inv_dev, status_dev, n, count, order=0, stream=None):
