@@ -40,6 +40,19 @@ int mvmc_pfinv_f64(const double* planes, int n, int count, double* pf, double* i
  * overwritten with the true inverse. */
 int mvmc_pfinv_f64_inplace(double* planes, int n, int count, double* pf, int* status, void* stream);
 
+/* No counterpart in the reference: mvmc_pfinv_f64 for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, results and status
+ * codes as mvmc_pfinv_f64 (inv == planes factorises in place), but the
+ * elimination's work buffer is a 128 MiB array in the library's code
+ * object on each device instead of a per-call allocation, so no allocation
+ * call is made.  n must be even, 2 <= n <= 1024 (MVMC_E_INVALID_SHAPE
+ * otherwise).  Calls that overlap on one device share that array and race:
+ * serialise them (one stream, or host-side ordering).  Asynchronous on
+ * `stream`. */
+int mvmc_pfinv_f64_noalloc(const double* planes, int n, int count, double* pf, double* inv, int* status,
+                           void* stream);
+
 /* (a) Batched Pfaffian + inverse of complex skew-symmetric planes.
  * Replaces mvmc_gpu::BatchedEngine::run_c64 (crates/mvmc-gpu/src/lib.rs:109-115;
  * CPU reference crates/mvmc-gpu/src/scalar.rs:146-170): planes [n,n,count]

@@ -253,6 +253,34 @@ void raise_lds_limit(const void* fn, size_t bytes) {
   c = bytes;
 }
 
+// Work array of mvmc_pfinv_f64_noalloc, in the code object (no allocation
+// call): every device the library runs on holds its own copy, 128 MiB, until
+// the process ends.  Host code never uses the name `scratch` as an address:
+// noalloc_scratch() looks the copy of the current device up once with
+// hipGetSymbolAddress and keeps it per device.  Calls that overlap on one
+// device share the array and race; contents persist between calls (the
+// kernel writes every scratch element before it reads it, so nothing needs
+// clearing).
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = (128ull << 20) / sizeof(double);
+constexpr int NOALLOC_MAX_N = 1024;       // one work slot (4 n^2 doubles = 32 MiB) fits
+constexpr int NOALLOC_MAX_SLOTS = 2048;   // workgroups of one call
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+double* noalloc_scratch() {
+  static std::mutex mu;
+  static std::map<int, double*> per_device;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = per_device.find(dev);
+  if (it != per_device.end()) return it->second;
+  void* p = nullptr;
+  HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) throw Fail(MVMC_E_BACKEND, "hipGetSymbolAddress(scratch) returned a null address");
+  per_device[dev] = (double*)p;
+  return (double*)p;
+}
+
 template <class T>
 T* dalloc(std::vector<void*>& list, size_t count) {
   void* p = nullptr;
@@ -1139,7 +1167,8 @@ int mvmc_pfinv_f64(const double* planes, int n, int count, double* pf, double* i
     HIPCHK(hipMallocAsync((void**)&work, sizeof(double) * (size_t)blocks * pfinv_work_doubles(n, np), st));
     size_t lds = pf_bytes(n, np) + 64;
     raise_lds_limit((const void*)k_pfinv_batched, lds);
-    hipLaunchKernelGGL(k_pfinv_batched, dim3(blocks), dim3(256), lds, st, inv, work, n, count, np, pf, status);
+    hipLaunchKernelGGL(k_pfinv_batched, dim3(blocks), dim3(256), lds, st, inv, work, n, count, np, blocks, pf,
+                       status);
     HIPCHK(hipGetLastError());
     HIPCHK(hipFreeAsync(work, st));
   });
@@ -1147,6 +1176,35 @@ int mvmc_pfinv_f64(const double* planes, int n, int count, double* pf, double* i
 
 int mvmc_pfinv_f64_inplace(double* planes, int n, int count, double* pf, int* status, void* stream) {
   return mvmc_pfinv_f64(planes, n, count, pf, planes, status, stream);
+}
+
+int mvmc_pfinv_f64_noalloc(const double* planes, int n, int count, double* pf, double* inv, int* status,
+                           void* stream) {
+  return guarded([&] {
+    require_gpu();
+    if (n < 2 || (n % 2) != 0 || count < 0) throw Fail(MVMC_E_INVALID_SHAPE, "n must be even and >= 2");
+    if (n > NOALLOC_MAX_N)
+      throw Fail(MVMC_E_INVALID_SHAPE, "n = " + std::to_string(n) + " exceeds the no-allocation maximum " +
+                                           std::to_string(NOALLOC_MAX_N));
+    if (count == 0) return;
+    if (!planes || !pf || !inv || !status) throw Fail(MVMC_E_INVALID_SHAPE, "null buffer");
+    double* work = noalloc_scratch();
+    hipStream_t st = (hipStream_t)stream;
+    if ((const double*)inv != planes)
+      HIPCHK(hipMemcpyAsync(inv, planes, sizeof(double) * n * n * count, hipMemcpyDeviceToDevice, st));
+    const int np = n <= 32 ? 8 : 4;
+    const int groups = (count + np - 1) / np;
+    // workgroups = work slots that fit in the scratch array (at least one:
+    // NOALLOC_MAX_N keeps a slot within it), at most one per plane group
+    const size_t per_slot = pfinv_work_doubles(n, np);
+    const int slots = (int)std::min<size_t>({(size_t)groups, NOALLOC_SCRATCH_DOUBLES / per_slot,
+                                             (size_t)NOALLOC_MAX_SLOTS});
+    size_t lds = pf_bytes(n, np) + 64;
+    raise_lds_limit((const void*)k_pfinv_batched, lds);
+    hipLaunchKernelGGL(k_pfinv_batched, dim3(slots), dim3(256), lds, st, inv, work, n, count, np, groups, pf,
+                       status);
+    HIPCHK(hipGetLastError());
+  });
 }
 
 int mvmc_pfinv_c128(const double* planes, int n, int count, int order, double* pf, double* inv, int* status,

@@ -1983,40 +1983,47 @@ __global__ void k_cg_dir(int na, const double* sc, const int* ctl, const double*
 // Batched Pfaffian + inverse (BatchedEngine): planes [count][n][n]
 // column-major; one workgroup per plane batch of np planes.
 // ---------------------------------------------------------------------------
+// Batched Pfaffian + inverse: plane group g = np consecutive planes; the
+// workgroup at blockIdx.x takes groups blockIdx.x, + gridDim.x, ... of the
+// ngroups, using work slot blockIdx.x (pfinv_work_doubles(n, np) doubles)
+// for each in turn.
 __global__ void __launch_bounds__(256) k_pfinv_batched(double* planes, double* work, int n, int count, int np,
-                                                       double* pf, int* status) {
+                                                       int ngroups, double* pf, int* status) {
   extern __shared__ __align__(16) char lds[];
-  const int first = blockIdx.x * np;
-  const int cnt = min(np, count - first);
-  if (cnt <= 0) return;
-  Carver c{lds};
-  PfinvShared sh;
-  carve_pfinv(c, sh, n, np);
-  double* pl = planes + (size_t)first * n * n;
   double* wk = work + (size_t)blockIdx.x * pfinv_work_doubles(n, np);
-  wg_pfinv(pl, wk, n, cnt, sh, pf + first, 1.0);
-  for (int p = threadIdx.x; p < cnt; p += blockDim.x) {
-    const int info = sh.info[p];
-    status[first + p] = info;
-    if (info > 0) {
-      pf[first + p] = 0.0;
-    } else if (info < 0) {
-      // non-finite Pfaffian: reported with its value (crates/mvmc-gpu/src/lib.rs:14-15),
-      // recomputed as wg_pfinv forms it from the LTL form the plane still holds
-      const double* a = pl + (size_t)p * n * n;
-      double v = 1.0;
-      for (int i = 0; i + 1 < n; i += 2) v *= a[i + (size_t)(i + 1) * n];
-      bool neg = false;
-      for (int k = 0; k < n; ++k)
-        if (sh.P(p, k) != k + 1) neg = !neg;
-      const double s = neg ? (0.0 - 1.0) : 1.0;
-      pf[first + p] = s * v;
+  for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const int first = grp * np;
+    const int cnt = min(np, count - first);
+    if (cnt <= 0) break;
+    Carver c{lds};
+    PfinvShared sh;
+    carve_pfinv(c, sh, n, np);
+    double* pl = planes + (size_t)first * n * n;
+    wg_pfinv(pl, wk, n, cnt, sh, pf + first, 1.0);
+    for (int p = threadIdx.x; p < cnt; p += blockDim.x) {
+      const int info = sh.info[p];
+      status[first + p] = info;
+      if (info > 0) {
+        pf[first + p] = 0.0;
+      } else if (info < 0) {
+        // non-finite Pfaffian: reported with its value (crates/mvmc-gpu/src/lib.rs:14-15),
+        // recomputed as wg_pfinv forms it from the LTL form the plane still holds
+        const double* a = pl + (size_t)p * n * n;
+        double v = 1.0;
+        for (int i = 0; i + 1 < n; i += 2) v *= a[i + (size_t)(i + 1) * n];
+        bool neg = false;
+        for (int k = 0; k < n; ++k)
+          if (sh.P(p, k) != k + 1) neg = !neg;
+        const double s = neg ? (0.0 - 1.0) : 1.0;
+        pf[first + p] = s * v;
+      }
     }
+    __syncthreads();
+    for (int p = 0; p < cnt; ++p)
+      if (sh.info[p] != 0)
+        for (int e = threadIdx.x; e < n * n; e += blockDim.x) pl[(size_t)p * n * n + e] = 0.0;
+    __syncthreads();  // the next group re-carves the same LDS and reuses the work slot
   }
-  __syncthreads();
-  for (int p = 0; p < cnt; ++p)
-    if (sh.info[p] != 0)
-      for (int e = threadIdx.x; e < n * n; e += blockDim.x) pl[(size_t)p * n * n + e] = 0.0;
 }
 
 }  // namespace mvmc_hip

@@ -42,6 +42,8 @@ def lib():
         L.mvmc_last_error.restype = ctypes.c_char_p
         L.mvmc_pfinv_f64.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_void_p, ctypes.c_void_p]
+        L.mvmc_pfinv_f64_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.mvmc_pfinv_f64_inplace.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                              ctypes.c_void_p, ctypes.c_void_p]
         L.mvmc_pfinv_c128.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
@@ -68,7 +70,7 @@ def lib():
 
 
 EXPORTED_SYMBOLS = [
-    "mvmc_last_error", "mvmc_abi_version", "mvmc_ens_launch_log" "mvmc_pfinv_f64", "mvmc_pfinv_f64_inplace", "mvmc_ens_create",
+    "mvmc_last_error", "mvmc_abi_version", "mvmc_ens_launch_log", "mvmc_pfinv_f64_noalloc", "mvmc_pfinv_f64", "mvmc_pfinv_f64_inplace", "mvmc_ens_create",
     "mvmc_ens_destroy", "mvmc_ens_dims", "mvmc_ens_get_params", "mvmc_ens_set_params", "mvmc_ens_sample",
     "mvmc_ens_measure", "mvmc_ens_sr_cg", "mvmc_ens_opt_step", "mvmc_ens_get_energy", "mvmc_ens_get_operands",
     "mvmc_ens_get_o_store", "mvmc_ens_get_walker", "mvmc_ens_get_tables", "mvmc_ens_get_config", "mvmc_ens_get_cg",
@@ -102,6 +104,17 @@ def pfaffian_inverse_batched(planes_dev, pf_dev, inv_dev, status_dev, n, count, 
     _check(lib().mvmc_pfinv_f64(ctypes.c_void_p(addr(planes_dev)), n, count, ctypes.c_void_p(addr(pf_dev)),
                                 ctypes.c_void_p(addr(inv_dev)), ctypes.c_void_p(addr(status_dev)),
                                 ctypes.c_void_p(stream or 0)))
+
+
+def pfaffian_inverse_batched_noalloc(planes_dev, pf_dev, inv_dev, status_dev, n, count, stream=None):
+    """mvmc_pfinv_f64_noalloc: pfaffian_inverse_batched without any device
+    allocation (the work buffer is the library's per-device static array;
+    n <= 1024; overlapping calls on one device race)."""
+    def addr(x):
+        return x if isinstance(x, int) else x.data_ptr()
+    _check(lib().mvmc_pfinv_f64_noalloc(ctypes.c_void_p(addr(planes_dev)), n, count, ctypes.c_void_p(addr(pf_dev)),
+                                        ctypes.c_void_p(addr(inv_dev)), ctypes.c_void_p(addr(status_dev)),
+                                        ctypes.c_void_p(stream or 0)))
 
 
 def pfaffian_inverse_batched_inplace(planes_dev, pf_dev, status_dev, n, count, stream=None):
