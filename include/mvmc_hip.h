@@ -17,9 +17,9 @@ extern "C" {
 
 /* Error codes. */
 #define MVMC_OK 0
-#define MVMC_E_INVALID_SHAPE (-1)   /* BatchedError::InvalidShape (crates/mvmc-gpu/src/lib.rs:83-101) */
-#define MVMC_E_UNAVAILABLE (-2)     /* BatchedError::BackendUnavailable: no GPU / HIP error at init */
-#define MVMC_E_BACKEND (-3)         /* BatchedError::Backend / StageError::Failed: HIP runtime error */
+#define MVMC_E_INVALID_SHAPE (-1)   /* mvmc_gpu::Error::InvalidShape (crates/mvmc-gpu/src/lib.rs:83-97) */
+#define MVMC_E_UNAVAILABLE (-2)     /* mvmc_gpu::Error::BackendUnavailable: no GPU / HIP error at init */
+#define MVMC_E_BACKEND (-3)         /* mvmc_gpu::Error::Backend / StageError::Failed: HIP runtime error */
 #define MVMC_E_INPUT (-4)           /* input files / parameters rejected */
 #define MVMC_E_NUMERIC (-5)         /* sticky device status set (see mvmc_ens_status) */
 
@@ -31,8 +31,11 @@ int mvmc_abi_version(void);
  * semantics lib.rs:1-80): planes [n,n,count] column-major (plane p = q + NQP*b),
  * n even >= 2; outputs pf[count], inv [n,n,count] = the TRUE inverse A^{-1}
  * (mVMC's invM = -inv is applied by the caller), status[count]: 0 ok,
- * k > 0 zero pivot at 1-based row k, -1 non-finite Pfaffian; failed planes are
- * zero-filled with pf = 0.  All pointers are device pointers; `stream` is a
+ * k > 0 zero pivot at 1-based row k, -1 non-finite Pfaffian (PlaneStatus
+ * wire codes, lib.rs:64-80); the inverse of a failed plane is zero-filled, a
+ * zero-pivot plane reports pf = 0 and a non-finite plane its computed pf
+ * (lib.rs:12-15).  The results are BatchedEngine::run_f64's BatchOutput<f64>
+ * = (pf, inv, status).  All pointers are device pointers; `stream` is a
  * hipStream_t (NULL = default stream).  Asynchronous on `stream`. */
 int mvmc_pfinv_f64(const double* planes, int n, int count, double* pf, double* inv, int* status, void* stream);
 
@@ -223,6 +226,36 @@ int mvmc_ens_launch_log(mvmc_ens* e, double* out, int max_rows);
  * 3 rebuild failure, 4 non-finite, 5 CG failure). */
 int mvmc_ens_status(mvmc_ens* e);
 
+/* Ensemble group: several independent runs (Expert-mode models of uneven
+ * Nsize, each with its own walkers, seeds RndSeed + w, parameters and SR)
+ * stepped together, one persistent launch per kernel class (placement,
+ * Metropolis sweep, periodic rebuild, measurement rebuild, measurement) over
+ * all runs' walkers: the grid is sized for the largest run's LDS and cut
+ * into per-run slot ranges in proportion to each run's work.  Each run
+ * gives exactly the results of the same ensemble stepped alone (its walkers,
+ * order of operations and walker-order reductions are its own).  No single
+ * reference counterpart: it runs the per-run step of
+ * crates/mvmc-core/src/multichain.rs:647-689 for many models at once.  The
+ * runs share NVMCSample and use the FP64 sampler.  Runs are owned by the
+ * group (do not destroy them); mvmc_group_run gives a run's handle for the
+ * mvmc_ens_* getters. */
+typedef struct mvmc_group mvmc_group;
+int mvmc_group_create(int nruns, const char* const* namelists, const char* const* initial_defs, const int* walkers,
+                      mvmc_group** out);
+int mvmc_group_destroy(mvmc_group* g);
+int mvmc_group_run(mvmc_group* g, int r, mvmc_ens** out);
+/* Sweeps captured as a HIP graph of `rounds` (sweep + rebuild) launches per
+ * replay, replayed until no walker waits (0: launches issued directly). */
+int mvmc_group_set_graph(mvmc_group* g, int rounds);
+/* One SR step of every run; rows[nruns][6] = each run's zvo_out row. */
+int mvmc_group_opt_step(mvmc_group* g, double* rows);
+/* The sampling phase of one SR step of every run (vmc_make_sample). */
+int mvmc_group_sample(mvmc_group* g);
+/* timings10: sampling s, measurement s, SR s of the last step, device ms of
+ * its sweep/rebuild rounds, rounds launched, graph replays, workgroups of the
+ * sweep, rebuild and measurement launches, sampler LDS bytes. */
+int mvmc_group_timings(mvmc_group* g, double* timings10);
+
 /* The standalone SR entry points below (mvmc_sr_cg_apply, mvmc_sr_gram_real,
  * mvmc_sr_cholesky_solve, mvmc_sr_cg_solve) keep one device workspace or
  * rocBLAS handle per process: calls that overlap from several host threads
@@ -250,6 +283,34 @@ int mvmc_sr_gram_real(int n, int samples, const double* store, double* out, void
  * -1 non-finite solution (the reference's Err(())).  Device pointers for s and
  * rhs; synchronises `stream`. */
 int mvmc_sr_cholesky_solve(int n, double* s, double* rhs, int* info, void* stream);
+
+/* SrStages::assemble_s_g (crates/mvmc-core/src/sr_backend.rs:249-256, C order
+ * COrderSr::assemble_s_g :484-516, C stcopt.c:69): S column-major [nm][nm],
+ * S[si + sj nm] = oo[(pi+off) ld + (pj+off)] - oo[pi+off] oo[pj+off] (pi =
+ * map[si]), the diagonal times 1 + sta_del; g[si] = -2 step_dt (ho[pi+off] -
+ * ho[0] oo[pi+off]).  oo / ho are real arrays, or complex arrays (re, im
+ * interleaved) read through their real parts when *_complex != 0 (RealView,
+ * sr_backend.rs:53-70).  Every element as the reference forms it (bit-exact).
+ * Device pointers; asynchronous on `stream`. */
+int mvmc_sr_assemble_s_g(const double* oo, int oo_complex, const double* ho, int ho_complex, const int* map, int nm,
+                         long long ld, int offset, double sta_del, double step_dt, double* s, double* g,
+                         void* stream);
+
+/* SrStages::gram_complex (sr_backend.rs:228-234, C order :386-420):
+ * out[i + j n] = sum_s O[i,s] conj(O[j,s]) over the complex [samples][n] store
+ * (O[i,s] at store[s n + i], re, im interleaved), samples summed in order and
+ * each product formed as the reference's (bit-exact).  Device pointers;
+ * asynchronous on `stream`. */
+int mvmc_sr_gram_complex(int n, int samples, const double* store, double* out, void* stream);
+
+/* SrStages::sr_s_g (sr_backend.rs:206-216, C order :436-460 over centered(),
+ * stage_backend.rs:167-184): o column-major [ns][np] (sample k of component
+ * p at o[k + p ns]), e, w [ns]; S[p + q np] = sum_k w_k d_kp d_kq, g[q] =
+ * sum_k w_k d_kq de_k with d = o - <o>_w, de = e - <e>_w, k in order
+ * (bit-exact).  Device pointers; takes ns*np + ns + 1 doubles of stream-
+ * ordered scratch; asynchronous on `stream`. */
+int mvmc_sr_s_g(const double* o, int ns, int np, const double* e, const double* w, double* s, double* g,
+                void* stream);
 
 /* SrStages::cg_step on caller-provided device operands: samples packed
  * [nrows][na] in walker order (nrows = W*nsample), mean/diag/g [na];

@@ -24,6 +24,7 @@
 #include "../host/model.h"
 #include "ensemble.hip"
 #include "pfinv_c128_dev.h"
+#include "sr_stages.h"
 
 using namespace mvmc_hip;
 
@@ -560,26 +561,26 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
   if (E->lds_samp > (size_t)maxlds || E->lds_meas > (size_t)maxlds)
     throw Fail(MVMC_E_INVALID_SHAPE, "model needs " + std::to_string(std::max(E->lds_samp, E->lds_meas)) +
                                          " B of LDS per workgroup, device allows " + std::to_string(maxlds));
-  raise_lds_limit((const void*)k_block<double>, E->lds_samp);
-  raise_lds_limit((const void*)k_block<float>, E->lds_samp);
-  raise_lds_limit((const void*)k_begin, E->lds_samp);
-  raise_lds_limit((const void*)k_measure, E->lds_meas);
+  raise_lds_limit((const void*)k_block<double, false>, E->lds_samp);
+  raise_lds_limit((const void*)k_block<float, false>, E->lds_samp);
+  raise_lds_limit((const void*)k_begin<false>, E->lds_samp);
+  raise_lds_limit((const void*)k_measure<false>, E->lds_meas);
   E->lds_rebuild = lds_bytes_rebuild(v);
-  raise_lds_limit((const void*)k_rebuild<3, 8>, E->lds_rebuild);
-  raise_lds_limit((const void*)k_rebuild<2, 16>, E->lds_rebuild);
-  raise_lds_limit((const void*)k_meas_rebuild<3, 8>, E->lds_rebuild);
-  raise_lds_limit((const void*)k_meas_rebuild<2, 16>, E->lds_rebuild);
+  raise_lds_limit((const void*)k_rebuild<3, 8, false>, E->lds_rebuild);
+  raise_lds_limit((const void*)k_rebuild<2, 16, false>, E->lds_rebuild);
+  raise_lds_limit((const void*)k_meas_rebuild<3, 8, false>, E->lds_rebuild);
+  raise_lds_limit((const void*)k_meas_rebuild<2, 16, false>, E->lds_rebuild);
   int occ_s = 0, occ_m = 0, occ_r = 0;
   {
     int occ_d = 0, occ_f = 0;
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_d, k_block<double>, 256, E->lds_samp));
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_f, k_block<float>, 256, E->lds_samp));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_d, k_block<double, false>, 256, E->lds_samp));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_f, k_block<float, false>, 256, E->lds_samp));
     occ_s = std::min(occ_d, occ_f);  // one grid serves both forms
   }
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_m, k_measure, 256, E->lds_meas));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_m, k_measure<false>, 256, E->lds_meas));
   int occ_r1 = 0;
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r, k_rebuild<3, 8>, 256, E->lds_rebuild));
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r1, k_rebuild<2, 16>, 256, E->lds_rebuild));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r, k_rebuild<3, 8, false>, 256, E->lds_rebuild));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r1, k_rebuild<2, 16, false>, 256, E->lds_rebuild));
   occ_r1 = std::max(1, occ_r1);
   occ_s = std::max(1, occ_s);
   occ_m = std::max(1, occ_m);
@@ -603,6 +604,8 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
 #endif
   E->gstride = std::max(1, std::max(v.ntrans, v.nexch));
   E->d_gscr = dalloc<double>(A, (size_t)E->slots * E->gstride);
+  v.gscr = E->d_gscr;
+  v.gstride = E->gstride;
   E->d_out5 = dalloc<double>(A, 8);
   E->d_ho = dalloc<double>(A, v.size);
   E->d_mean = dalloc<double>(A, v.size);
@@ -619,17 +622,21 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
 
 void launch_rebuild(mvmc_ens* E, int var) {
   if (var == 1)
-    hipLaunchKernelGGL((k_rebuild<2, 16>), dim3(E->grid_rb[1]), dim3(256), E->lds_rebuild, E->st, E->v);
+    hipLaunchKernelGGL((k_rebuild<2, 16, false>), dim3(E->grid_rb[1]), dim3(256), E->lds_rebuild, E->st, E->v,
+                       GroupArgs{});
   else
-    hipLaunchKernelGGL((k_rebuild<3, 8>), dim3(E->grid_rb[0]), dim3(256), E->lds_rebuild, E->st, E->v);
+    hipLaunchKernelGGL((k_rebuild<3, 8, false>), dim3(E->grid_rb[0]), dim3(256), E->lds_rebuild, E->st, E->v,
+                       GroupArgs{});
   HIPCHK(hipGetLastError());
 }
 
 void launch_meas_rebuild(mvmc_ens* E, int s) {
   if (E->rb_variant == 1)
-    hipLaunchKernelGGL((k_meas_rebuild<2, 16>), dim3(E->grid_rb[1]), dim3(256), E->lds_rebuild, E->st, E->v, s);
+    hipLaunchKernelGGL((k_meas_rebuild<2, 16, false>), dim3(E->grid_rb[1]), dim3(256), E->lds_rebuild, E->st, E->v,
+                       GroupArgs{}, s);
   else
-    hipLaunchKernelGGL((k_meas_rebuild<3, 8>), dim3(E->grid_rb[0]), dim3(256), E->lds_rebuild, E->st, E->v, s);
+    hipLaunchKernelGGL((k_meas_rebuild<3, 8, false>), dim3(E->grid_rb[0]), dim3(256), E->lds_rebuild, E->st, E->v,
+                       GroupArgs{}, s);
   HIPCHK(hipGetLastError());
 }
 
@@ -637,7 +644,7 @@ void sample_impl(mvmc_ens* E) {
   EnsView& v = E->v;
   const double t0 = now_s();
   E->upload_tables();
-  hipLaunchKernelGGL(k_begin, dim3(E->grid_samp), dim3(256), E->lds_samp, E->st, v);
+  hipLaunchKernelGGL(k_begin<false>, dim3(E->grid_samp), dim3(256), E->lds_samp, E->st, v, GroupArgs{});
   HIPCHK(hipGetLastError());
   std::vector<int> ws((size_t)v.W * 8);
   HIPCHK(hipMemcpyAsync(ws.data(), v.wstate, sizeof(int) * ws.size(), hipMemcpyDeviceToHost, E->st));
@@ -667,9 +674,11 @@ void sample_impl(mvmc_ens* E) {
       HIPCHK(hipMemsetAsync(v.nflag, 0, 4 * sizeof(int), E->st));
       HIPCHK(hipEventRecord(e0, E->st));
       if (E->precision == 1)
-        hipLaunchKernelGGL(k_block<float>, dim3(E->grid_samp), dim3(256), E->lds_samp, E->st, v, E->inv32, out, n_in);
+        hipLaunchKernelGGL((k_block<float, false>), dim3(E->grid_samp), dim3(256), E->lds_samp, E->st, v, GroupArgs{},
+                           out, n_in);
       else
-        hipLaunchKernelGGL(k_block<double>, dim3(E->grid_samp), dim3(256), E->lds_samp, E->st, v, v.inv, out, n_in);
+        hipLaunchKernelGGL((k_block<double, false>), dim3(E->grid_samp), dim3(256), E->lds_samp, E->st, v, GroupArgs{},
+                           out, n_in);
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(e1, E->st));
       int nf = 0;
@@ -729,6 +738,8 @@ void sample_impl(mvmc_ens* E) {
   E->t_sample += now_s() - t0;
 }
 
+void measure_finish(mvmc_ens* E);
+
 void measure_impl(mvmc_ens* E) {
   EnsView& v = E->v;
   const double t0 = now_s();
@@ -739,7 +750,7 @@ void measure_impl(mvmc_ens* E) {
     HIPCHK(hipEventRecord(ev[0], E->st));
     launch_meas_rebuild(E, s);
     HIPCHK(hipEventRecord(ev[1], E->st));
-    hipLaunchKernelGGL(k_measure, dim3(E->grid_meas), dim3(256), E->lds_meas, E->st, v, E->d_gscr, E->gstride, 1,
+    hipLaunchKernelGGL(k_measure<false>, dim3(E->grid_meas), dim3(256), E->lds_meas, E->st, v, GroupArgs{}, 1,
                        s);
     HIPCHK(hipEventRecord(ev[2], E->st));
     HIPCHK(hipGetLastError());
@@ -754,6 +765,14 @@ void measure_impl(mvmc_ens* E) {
   E->meas_rebuild_ms = ms_rb;
   E->measure_ms = ms_m;
   E->measured = (double)v.W * v.nsample;
+  measure_finish(E);
+  E->t_measure += now_s() - t0;
+}
+
+// Walker-order reductions of the measured samples and the averages
+// (run.rs:1863-1903, average.rs:18-63), on the handle's stream.
+void measure_finish(mvmc_ens* E) {
+  EnsView& v = E->v;
   hipLaunchKernelGGL(k_reduce_energy, dim3(1), dim3(64), 0, E->st, v, E->d_out5);
   hipLaunchKernelGGL(k_reduce_ops, dim3(nblk(v.size)), dim3(256), 0, E->st, v, E->d_ho, E->d_mean, E->d_diag);
   HIPCHK(hipGetLastError());
@@ -777,7 +796,6 @@ void measure_impl(mvmc_ens* E) {
     for (auto& x : E->oo) x *= inv;
     for (auto& x : E->ho) x *= inv;
   }
-  E->t_measure += now_s() - t0;
 }
 
 // Measurement of caller-supplied configurations: MeasurementStages::
@@ -849,7 +867,7 @@ void measure_configs_impl(mvmc_ens* E, int count, const int* ele_idx, double* e_
     if (np > 0) put(v.s_proj, pc, np);
     v.W = wl;
     launch_meas_rebuild(E, s);
-    hipLaunchKernelGGL(k_measure, dim3(E->grid_meas), dim3(256), E->lds_meas, st, v, E->d_gscr, E->gstride, 1, s);
+    hipLaunchKernelGGL(k_measure<false>, dim3(E->grid_meas), dim3(256), E->lds_meas, st, v, GroupArgs{}, 1, s);
     HIPCHK(hipGetLastError());
     v.W = W0;
     HIPCHK(hipMemcpy2DAsync(e_loc + base, sizeof(double), v.e_loc + s, sizeof(double) * nsamp, sizeof(double), wl,
@@ -1145,6 +1163,265 @@ int sr_impl(mvmc_ens* E, long long* info_out, bool force_direct = false) {
   return 0;
 }
 
+// The rest of one SR step after the measurement (run.rs:1294-1475): zvo row,
+// per-step output, SR solve and update, the optimisation window.
+void finish_opt_step(mvmc_ens* e, double* row) {
+  if (row) {
+    double var = 0.0;
+    if (std::fabs(e->etot) > 1.0e-14) var = (e->etot2 - e->etot * e->etot) / (e->etot * e->etot);
+    const double r6[6] = {e->etot, 0.0, e->etot2, var, e->sztot, e->sztot2};
+    std::memcpy(row, r6, sizeof(r6));
+  }
+  if (e->out_enabled) write_step_output(e, e->opt_steps);
+  long long info = 0;
+  int rc = sr_impl(e, &info);
+  if (rc != 0) throw Fail(MVMC_E_NUMERIC, "SR step skipped (non-finite update)");
+  // store_opt_data for the window [NSROptItrStep - NSROptItrSmp, NSROptItrStep)
+  // (crates/mvmc-core/src/run.rs:1258-1262,1457-1460)
+  const mvmc::Model& m = e->m;
+  const int step = e->opt_steps++;
+  const int len = m.mp.nsr_opt_itr_smp, start = std::max(m.mp.nsr_opt_itr_step, 0) - len;
+  if (len > 0 && step >= start && step - start < len) {
+    const size_t k = (size_t)(step - start);
+    if (e->opt_win.size() < (size_t)len) {
+      e->opt_win.resize(len);
+      e->opt_win_e.resize(len);
+      e->opt_win_e2.resize(len);
+      e->opt_win_set.assign(len, 0);
+    }
+    std::vector<double> p(m.proj);
+    p.insert(p.end(), m.slater.begin(), m.slater.end());
+    e->opt_win[k] = p;
+    e->opt_win_e[k] = e->etot;
+    e->opt_win_e2[k] = e->etot2;
+    e->opt_win_set[k] = 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Ensemble group: several independent runs (models of uneven Nsize) stepped
+// together, one launch per kernel class (walker = rank structure of
+// crates/mvmc-core/src/multichain.rs:647-689 per run).  The grid of each
+// launch is sized for the largest run's LDS and cut into per-run slot ranges
+// (GroupArgs); a run's slot range serves its walkers exactly as its own
+// launch would, so every run's results equal those of the run stepped alone.
+// ---------------------------------------------------------------------------
+struct GroupPart {
+  std::vector<int> start;  // [nruns + 1]
+  int* d = nullptr;        // device copy
+};
+
+}  // namespace
+
+struct mvmc_group {
+  std::vector<mvmc_ens*> runs;
+  hipStream_t st = nullptr;
+  std::vector<void*> allocs;
+  EnsView* d_views = nullptr;
+  int* d_flags = nullptr;  // [nruns][4]: the runs' walker counters (their v.nflag points here)
+  size_t lds_samp = 0, lds_meas = 0, lds_rebuild = 0;
+  GroupPart samp, meas, rb;  // slot ranges of the sampler, measurement and rebuild launches
+  int graph_rounds = 0;      // k_block + k_rebuild rounds per captured graph (0: direct launches)
+  // timings of the last group step
+  double t_sample = 0, t_measure = 0, t_sr = 0, block_ms = 0, rebuild_ms = 0;
+  int launches = 0, graph_replays = 0;
+  ~mvmc_group() {
+    for (void* p : allocs) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+    for (mvmc_ens* e : runs) delete e;
+  }
+};
+
+namespace {
+
+// Slot ranges of one launch class: run r gets a share of `grid` slots
+// proportional to weight[r] (at least 1, at most its own scratch slots and
+// walkers).
+void group_partition(mvmc_group* G, GroupPart& P, int grid, const std::vector<double>& weight) {
+  const int R = (int)G->runs.size();
+  double tot = 0;
+  for (double x : weight) tot += x;
+  std::vector<int> cnt(R);
+  int used = 0;
+  for (int r = 0; r < R; ++r) {
+    const mvmc_ens* E = G->runs[r];
+    int c = (int)std::floor(grid * weight[r] / tot);
+    c = std::max(1, std::min({c, E->slots, E->v.W}));
+    cnt[r] = c;
+    used += c;
+  }
+  // hand out what rounding left, largest weight first
+  std::vector<int> order(R);
+  for (int r = 0; r < R; ++r) order[r] = r;
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return weight[a] > weight[b]; });
+  for (bool more = true; more && used < grid;) {
+    more = false;
+    for (int r : order) {
+      const mvmc_ens* E = G->runs[r];
+      if (used < grid && cnt[r] < std::min(E->slots, E->v.W)) { ++cnt[r]; ++used; more = true; }
+    }
+  }
+  P.start.assign(R + 1, 0);
+  for (int r = 0; r < R; ++r) P.start[r + 1] = P.start[r] + cnt[r];
+  if (!P.d) P.d = dalloc<int>(G->allocs, R + 1);
+  HIPCHK(hipMemcpyAsync(P.d, P.start.data(), sizeof(int) * (R + 1), hipMemcpyHostToDevice, G->st));
+  HIPCHK(hipStreamSynchronize(G->st));
+}
+
+__global__ void k_fill_int(int* p, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+void group_create_impl(mvmc_group* G, int nruns, const char* const* namelists, const char* const* initials,
+                       const int* walkers) {
+  require_gpu();
+  if (nruns < 1) throw Fail(MVMC_E_INVALID_SHAPE, "a group needs at least one run");
+  for (int r = 0; r < nruns; ++r) {
+    mvmc_ens* E = new mvmc_ens();
+    G->runs.push_back(E);
+    create_impl(E, namelists[r], initials ? initials[r] : nullptr, walkers[r], 0, 0);
+    if (E->v.nsample != G->runs[0]->v.nsample)
+      throw Fail(MVMC_E_INPUT, "runs of a group must share NVMCSample");
+  }
+  HIPCHK(hipStreamCreateWithFlags(&G->st, hipStreamNonBlocking));
+  G->d_flags = dalloc<int>(G->allocs, 4 * (size_t)nruns);
+  std::vector<EnsView> views;
+  for (int r = 0; r < nruns; ++r) {
+    mvmc_ens* E = G->runs[r];
+    E->v.nflag = G->d_flags + 4 * r;
+    views.push_back(E->v);
+    G->lds_samp = std::max(G->lds_samp, E->lds_samp);
+    G->lds_meas = std::max(G->lds_meas, E->lds_meas);
+    G->lds_rebuild = std::max(G->lds_rebuild, E->lds_rebuild);
+  }
+  G->d_views = dalloc<EnsView>(G->allocs, nruns);
+  HIPCHK(hipMemcpyAsync(G->d_views, views.data(), sizeof(EnsView) * nruns, hipMemcpyHostToDevice, G->st));
+  raise_lds_limit((const void*)k_begin<true>, G->lds_samp);
+  raise_lds_limit((const void*)k_block<double, true>, G->lds_samp);
+  raise_lds_limit((const void*)k_rebuild<3, 8, true>, G->lds_rebuild);
+  raise_lds_limit((const void*)k_meas_rebuild<3, 8, true>, G->lds_rebuild);
+  raise_lds_limit((const void*)k_measure<true>, G->lds_meas);
+  int dev = 0, ncu = 0, occ_s = 0, occ_m = 0, occ_r = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, k_block<double, true>, 256, G->lds_samp));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_m, k_measure<true>, 256, G->lds_meas));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r, k_rebuild<3, 8, true>, 256, G->lds_rebuild));
+  // work weights: sampling ~ W Nsite n^2 NQP (accepted moves stream the
+  // inverse), rebuild and measurement ~ W n^3 NQP
+  std::vector<double> ws, wr;
+  for (mvmc_ens* E : G->runs) {
+    const double W = E->v.W, n = E->v.n, q = E->v.nqp;
+    ws.push_back(W * E->v.ns * n * n * q);
+    wr.push_back(W * n * n * n * q);
+  }
+  group_partition(G, G->samp, std::max(1, occ_s) * ncu, ws);
+  group_partition(G, G->meas, std::max(1, occ_m) * ncu, wr);
+  group_partition(G, G->rb, std::max(1, occ_r) * ncu, wr);
+}
+
+GroupArgs group_args(mvmc_group* G, const GroupPart& P) {
+  return GroupArgs{G->d_views, P.d, (int)G->runs.size()};
+}
+
+// One round: clear the walker counters, run every walker until its step ends
+// or its rebuild is due, rebuild the walkers handed over.
+void group_round(mvmc_group* G, int out, int n_in_max) {
+  const int R = (int)G->runs.size();
+  hipLaunchKernelGGL(k_fill_int, dim3(nblk(4 * R)), dim3(256), 0, G->st, G->d_flags, 4 * R, 0);
+  hipLaunchKernelGGL((k_block<double, true>), dim3(G->samp.start[R]), dim3(256), G->lds_samp, G->st, EnsView{},
+                     group_args(G, G->samp), out, n_in_max);
+  hipLaunchKernelGGL((k_rebuild<3, 8, true>), dim3(G->rb.start[R]), dim3(256), G->lds_rebuild, G->st, EnsView{},
+                     group_args(G, G->rb));
+}
+
+void group_sample(mvmc_group* G) {
+  const int R = (int)G->runs.size();
+  const double t0 = now_s();
+  for (mvmc_ens* E : G->runs) {
+    if (E->precision != 0) throw Fail(MVMC_E_INVALID_SHAPE, "group runs use the FP64 sampler");
+    E->upload_tables();
+  }
+  hipLaunchKernelGGL(k_begin<true>, dim3(G->samp.start[R]), dim3(256), G->lds_samp, G->st, EnsView{},
+                     group_args(G, G->samp));
+  HIPCHK(hipGetLastError());
+  int n_out = 0;
+  for (mvmc_ens* E : G->runs) {
+    std::vector<int> ws((size_t)E->v.W * 8);
+    HIPCHK(hipMemcpyAsync(ws.data(), E->v.wstate, sizeof(int) * ws.size(), hipMemcpyDeviceToHost, G->st));
+    HIPCHK(hipStreamSynchronize(G->st));
+    for (int w = 0; w < E->v.W; ++w) n_out = std::max(n_out, ws[(size_t)w * 8 + 2]);
+  }
+  for (mvmc_ens* E : G->runs) E->check_status();
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  G->launches = 0;
+  G->graph_replays = 0;
+  double ms_total = 0;
+  std::vector<int> flags(4 * (size_t)R);
+  for (int out = 0; out < n_out; ++out) {
+    for (mvmc_ens* E : G->runs)
+      hipLaunchKernelGGL(k_reset_resume, dim3(nblk(E->v.W)), dim3(256), 0, G->st, E->v);
+    // n_in differs per run (NVMCInterval x Nsite): each run's slots read it
+    // from their view; the argument is ignored in group launches
+    HIPCHK(hipEventRecord(e0, G->st));
+    hipGraphExec_t exec = nullptr;
+    if (G->graph_rounds > 0) {
+      hipGraph_t graph = nullptr;
+      HIPCHK(hipStreamBeginCapture(G->st, hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < G->graph_rounds; ++k) group_round(G, out, 0);
+      HIPCHK(hipStreamEndCapture(G->st, &graph));
+      HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(graph);
+    }
+    for (;;) {
+      if (exec) {
+        HIPCHK(hipGraphLaunch(exec, G->st));
+        ++G->graph_replays;
+        G->launches += G->graph_rounds;
+      } else {
+        group_round(G, out, 0);
+        ++G->launches;
+      }
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(flags.data(), G->d_flags, sizeof(int) * flags.size(), hipMemcpyDeviceToHost, G->st));
+      HIPCHK(hipStreamSynchronize(G->st));
+      bool pending = false;
+      for (int r = 0; r < R; ++r) pending = pending || flags[4 * r] != 0;
+      if (!pending) break;
+    }
+    HIPCHK(hipEventRecord(e1, G->st));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    ms_total += ms;
+    if (exec) (void)hipGraphExecDestroy(exec);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  G->block_ms = ms_total;
+  for (mvmc_ens* E : G->runs) E->check_status();
+  G->t_sample = now_s() - t0;
+}
+
+void group_measure(mvmc_group* G) {
+  const int R = (int)G->runs.size();
+  const double t0 = now_s();
+  const int nsample = G->runs[0]->v.nsample;
+  for (int s = 0; s < nsample; ++s) {
+    hipLaunchKernelGGL((k_meas_rebuild<3, 8, true>), dim3(G->rb.start[R]), dim3(256), G->lds_rebuild, G->st,
+                       EnsView{}, group_args(G, G->rb), s);
+    hipLaunchKernelGGL(k_measure<true>, dim3(G->meas.start[R]), dim3(256), G->lds_meas, G->st, EnsView{},
+                       group_args(G, G->meas), 1, s);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipStreamSynchronize(G->st));
+  for (mvmc_ens* E : G->runs) measure_finish(E);
+  G->t_measure = now_s() - t0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1296,36 +1573,7 @@ int mvmc_ens_opt_step(mvmc_ens* e, double* row) {
   return guarded([&] {
     sample_impl(e);
     measure_impl(e);
-    if (row) {
-      double var = 0.0;
-      if (std::fabs(e->etot) > 1.0e-14) var = (e->etot2 - e->etot * e->etot) / (e->etot * e->etot);
-      const double r6[6] = {e->etot, 0.0, e->etot2, var, e->sztot, e->sztot2};
-      std::memcpy(row, r6, sizeof(r6));
-    }
-    if (e->out_enabled) write_step_output(e, e->opt_steps);
-    long long info = 0;
-    int rc = sr_impl(e, &info);
-    if (rc != 0) throw Fail(MVMC_E_NUMERIC, "SR step skipped (non-finite update)");
-    // store_opt_data for the window [NSROptItrStep - NSROptItrSmp, NSROptItrStep)
-    // (crates/mvmc-core/src/run.rs:1258-1262,1457-1460)
-    const mvmc::Model& m = e->m;
-    const int step = e->opt_steps++;
-    const int len = m.mp.nsr_opt_itr_smp, start = std::max(m.mp.nsr_opt_itr_step, 0) - len;
-    if (len > 0 && step >= start && step - start < len) {
-      const size_t k = (size_t)(step - start);
-      if (e->opt_win.size() < (size_t)len) {
-        e->opt_win.resize(len);
-        e->opt_win_e.resize(len);
-        e->opt_win_e2.resize(len);
-        e->opt_win_set.assign(len, 0);
-      }
-      std::vector<double> p(m.proj);
-      p.insert(p.end(), m.slater.begin(), m.slater.end());
-      e->opt_win[k] = p;
-      e->opt_win_e[k] = e->etot;
-      e->opt_win_e2[k] = e->etot2;
-      e->opt_win_set[k] = 1;
-    }
+    finish_opt_step(e, row);
   });
 }
 
@@ -1611,6 +1859,55 @@ int mvmc_sr_cholesky_solve(int n, double* s, double* rhs, int* info, void* strea
   });
 }
 
+int mvmc_sr_assemble_s_g(const double* oo, int oo_complex, const double* ho, int ho_complex, const int* map, int nm,
+                         long long ld, int offset, double sta_del, double step_dt, double* s, double* g,
+                         void* stream) {
+  return guarded([&] {
+    require_gpu();
+    if (nm < 0 || ld < 0 || offset < 0) throw Fail(MVMC_E_INVALID_SHAPE, "bad S/g dimensions");
+    if (nm == 0) return;
+    if (!oo || !ho || !map || !s || !g) throw Fail(MVMC_E_INVALID_SHAPE, "null buffer");
+    const long long total = (long long)nm * nm + nm;
+    hipLaunchKernelGGL(k_sr_assemble_stage, dim3(nblk(total)), dim3(256), 0, (hipStream_t)stream, oo,
+                       oo_complex ? 1 : 0, ho, ho_complex ? 1 : 0, map, nm, ld, offset, 1.0 + sta_del,
+                       -2.0 * step_dt, s, g);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int mvmc_sr_gram_complex(int n, int samples, const double* store, double* out, void* stream) {
+  return guarded([&] {
+    require_gpu();
+    if (n < 0 || samples < 0) throw Fail(MVMC_E_INVALID_SHAPE, "bad Gram dimensions");
+    if (n == 0) return;
+    if (!out || (!store && samples > 0)) throw Fail(MVMC_E_INVALID_SHAPE, "null buffer");
+    const int tiles = (n + GR_T - 1) / GR_T;
+    hipLaunchKernelGGL(k_gram_complex, dim3(tiles, tiles), dim3(256), 0, (hipStream_t)stream, store, n, samples,
+                       out);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int mvmc_sr_s_g(const double* o, int ns, int np, const double* e, const double* w, double* s, double* g,
+                void* stream) {
+  return guarded([&] {
+    require_gpu();
+    if (ns < 0 || np < 0) throw Fail(MVMC_E_INVALID_SHAPE, "bad S/g dimensions");
+    if (np == 0) return;
+    if (!o || !e || !w || !s || !g) throw Fail(MVMC_E_INVALID_SHAPE, "null buffer");
+    hipStream_t st = (hipStream_t)stream;
+    double* d = nullptr;
+    HIPCHK(hipMallocAsync((void**)&d, sizeof(double) * ((size_t)ns * np + ns + 1), st));
+    double* de = d + (size_t)ns * np;
+    hipLaunchKernelGGL(k_ssg_center, dim3(nblk((long long)np + 1)), dim3(256), 0, st, o, ns, np, e, w, d, de);
+    const int tiles = (np + GR_T - 1) / GR_T;
+    hipLaunchKernelGGL(k_ssg_s, dim3(tiles, tiles), dim3(256), 0, st, d, w, ns, np, s);
+    hipLaunchKernelGGL(k_ssg_g, dim3(nblk(np)), dim3(256), 0, st, d, w, de, ns, np, g);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipFreeAsync(d, st));
+  });
+}
+
 int mvmc_sr_cg_apply(int na, int walkers, int nsample, const double* pack, const double* mean, const double* diag,
                      const double* x, double inv_w, double shift, double* z, void* stream) {
   static CgWork wk;
@@ -1636,6 +1933,57 @@ int mvmc_sr_cg_solve(int na, int walkers, int nsample, const double* pack, const
     *status = cg_solve_dev(na, walkers, nsample, pack, mean, diag, g, inv_w, shift, tol, max_it, variant != 0, x, wk,
                            (hipStream_t)stream);
   });
+}
+
+int mvmc_group_create(int nruns, const char* const* namelists, const char* const* initial_defs, const int* walkers,
+                      mvmc_group** out) {
+  if (!out || !namelists || !walkers) return MVMC_E_INVALID_SHAPE;
+  *out = nullptr;
+  mvmc_group* G = new mvmc_group();
+  int rc = guarded([&] { group_create_impl(G, nruns, namelists, initial_defs, walkers); });
+  if (rc != MVMC_OK) {
+    delete G;
+    return rc;
+  }
+  *out = G;
+  return MVMC_OK;
+}
+
+int mvmc_group_destroy(mvmc_group* g) {
+  delete g;
+  return MVMC_OK;
+}
+
+int mvmc_group_run(mvmc_group* g, int r, mvmc_ens** out) {
+  if (!g || !out || r < 0 || r >= (int)g->runs.size()) return MVMC_E_INVALID_SHAPE;
+  *out = g->runs[r];
+  return MVMC_OK;
+}
+
+int mvmc_group_set_graph(mvmc_group* g, int rounds) {
+  if (!g || rounds < 0) return MVMC_E_INVALID_SHAPE;
+  g->graph_rounds = rounds;
+  return MVMC_OK;
+}
+
+int mvmc_group_opt_step(mvmc_group* g, double* rows) {
+  return guarded([&] {
+    group_sample(g);
+    group_measure(g);
+    const double t0 = now_s();
+    for (size_t r = 0; r < g->runs.size(); ++r) finish_opt_step(g->runs[r], rows ? rows + 6 * r : nullptr);
+    g->t_sr = now_s() - t0;
+  });
+}
+
+int mvmc_group_sample(mvmc_group* g) { return guarded([&] { group_sample(g); }); }
+
+int mvmc_group_timings(mvmc_group* g, double* t) {
+  const int R = (int)g->runs.size();
+  t[0] = g->t_sample; t[1] = g->t_measure; t[2] = g->t_sr; t[3] = g->block_ms; t[4] = g->launches;
+  t[5] = g->graph_replays; t[6] = g->samp.start[R]; t[7] = g->rb.start[R]; t[8] = g->meas.start[R];
+  t[9] = (double)g->lds_samp;
+  return MVMC_OK;
 }
 
 }  // extern "C"

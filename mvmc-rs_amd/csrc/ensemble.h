@@ -64,6 +64,18 @@ struct EnsView {
   double* pend;       // [slots][2][nqp][n]: vec1 / vec2 of the not yet applied hop update
   int* status;        // [1] sticky
   long long* clk;     // [slots][32] phase clocks (MVMC_PHASE_CLOCKS builds), else null
+  double* gscr;       // [slots][gstride] per-term Green ratios of the measurement
+  int gstride;
+};
+
+// One launch over several ensembles (runs of different models): run r owns
+// workgroups [slot_start[r], slot_start[r+1]) of the grid, which serve its
+// walkers exactly as a launch of its own would (slot = blockIdx.x -
+// slot_start[r], its own scratch and walker counters).
+struct GroupArgs {
+  const EnsView* views;   // [nruns] device copies of the runs' views
+  const int* slot_start;  // [nruns + 1]
+  int nruns;              // 0: single-ensemble launch (the view argument)
 };
 
 }  // namespace mvmc_hip

@@ -41,6 +41,22 @@ struct Carver {
 
 __host__ __device__ inline size_t rnd16(size_t b) { return (b + 15) & ~(size_t)15; }
 
+// Run of this workgroup in a group launch (slot_start is small and uniform).
+__device__ inline int group_run(const GroupArgs& g) {
+  int r = 0;
+  while (r + 1 < g.nruns && (int)blockIdx.x >= g.slot_start[r + 1]) ++r;
+  return r;
+}
+
+// Kernel prologue of the walker kernels: `v` is the ensemble this workgroup
+// serves, `slot` its scratch slot and `nslots` the workgroups serving that
+// ensemble (the launch's own grid outside a group).
+#define ENS_SLOT(VARG, GARG)                                                                               \
+  const int grp_run_ = GROUP ? group_run(GARG) : 0;                                                      \
+  const EnsView v = GROUP ? (GARG).views[grp_run_] : (VARG);                                             \
+  const int slot = GROUP ? (int)blockIdx.x - (GARG).slot_start[grp_run_] : (int)blockIdx.x;             \
+  const int nslots = GROUP ? (GARG).slot_start[grp_run_ + 1] - (GARG).slot_start[grp_run_] : (int)gridDim.x;
+
 __host__ __device__ inline size_t union_doubles_sampler(const EnsView& v) {
   const size_t nq = (size_t)v.nqp * v.n;
   size_t hop = 3 * nq;
@@ -398,13 +414,15 @@ __device__ inline void copy_inv32(const EnsView& v, int w) {
   }
 }
 
-__global__ void __launch_bounds__(256) k_begin(EnsView v) {
+template <bool GROUP>
+__global__ void __launch_bounds__(256) k_begin(EnsView v_arg, GroupArgs ga) {
+  ENS_SLOT(v_arg, ga)
   extern __shared__ __align__(16) char lds[];
   SampLds L = carve_sampler(v, lds);
   const int tid = threadIdx.x;
-  double* work = v.work + (size_t)blockIdx.x * pfinv_work_doubles(v.n, v.nqp);
+  double* work = v.work + (size_t)slot * pfinv_work_doubles(v.n, v.nqp);
   char* pfscr = (char*)L.uni;
-  for (int w = blockIdx.x; w < v.W; w += gridDim.x) {
+  for (int w = slot; w < v.W; w += nslots) {
     int* ws = v.wstate + (size_t)w * 8;
     double* planes = v.inv + (size_t)w * v.nqp * v.n * v.n;
     int* cnt = v.proj_cnt + (size_t)w * v.nproj;
@@ -736,7 +754,7 @@ __device__ void stream_planes(T* planes, int n, int nqp, const double* pv1, cons
 #ifdef MVMC_PHASE_CLOCKS
 #define PHASE_INIT                   \
   long long clk_t0 = wall_clock64(); \
-  long long* clk = v.clk + (size_t)blockIdx.x * 32;
+  long long* clk = v.clk + (size_t)slot * 32;
 #define PHASE(k)                           \
   do {                                     \
     if (threadIdx.x == 0) {                \
@@ -756,15 +774,19 @@ enum { M_TYPE = 0, M_REJ, M_MI, M_S, M_RI, M_RJ, M_MJ, M_T, M_ACC, M_NACC, M_NOU
 
 // Elem: storage type of the sampler's inverse (double; float in the
 // mixed-precision form, whose arithmetic stays FP64).
-template <class Elem>
-__global__ void __launch_bounds__(256, 2) k_block(EnsView v, Elem* inv, int out_step, int n_in) {
+template <class Elem, bool GROUP>
+__global__ void __launch_bounds__(256, 2) k_block(EnsView v_arg, GroupArgs ga, int out_step, int n_in_arg) {
+  ENS_SLOT(v_arg, ga)
+  // proposals per outer step (NVMCInterval x Nsite): each run's own in a group
+  const int n_in = GROUP ? v.interval * (v.ns > 0 ? v.ns : 1) : n_in_arg;
+  Elem* inv = sizeof(Elem) == sizeof(float) ? (Elem*)v.inv32 : (Elem*)v.inv;
   extern __shared__ __align__(16) char lds[];
   SampLds L = carve_sampler(v, lds);
   load_tables(v, L);
   const int tid = threadIdx.x, nt = blockDim.x;
   const int n = v.n, ne = v.ne, ns = v.ns, n2 = v.n2, nqp = v.nqp, nn = n * n;
-  double* work = v.work + (size_t)blockIdx.x * pfinv_work_doubles(n, nqp);
-  double* pv1 = v.pend + (size_t)blockIdx.x * 2 * nqp * n;  // pending hop update: vec1
+  double* work = v.work + (size_t)slot * pfinv_work_doubles(n, nqp);
+  double* pv1 = v.pend + (size_t)slot * 2 * nqp * n;  // pending hop update: vec1
   double* pv2 = pv1 + (size_t)nqp * n;                      //                     vec2
   PHASE_INIT
   __shared__ int next_w;
@@ -1214,8 +1236,10 @@ __host__ __device__ inline size_t lds_bytes_rebuild(const EnsView& v) {
 // Two compiled variants, bit-identical in results (only the schedule of the
 // elimination differs), picked at run time by measurement: MINB workgroups
 // per CU, JB columns per batch of the panel close.
-template <int MINB, int JB>
-__global__ void __launch_bounds__(256, MINB) k_rebuild(EnsView v) {
+template <int MINB, int JB, bool GROUP>
+__global__ void __launch_bounds__(256, MINB) k_rebuild(EnsView v_arg, GroupArgs ga) {
+  ENS_SLOT(v_arg, ga)
+  (void)nslots;
   extern __shared__ __align__(16) char lds[];
   Carver c{lds};
   int* ei = c.take<int>(v.n);
@@ -1223,7 +1247,7 @@ __global__ void __launch_bounds__(256, MINB) k_rebuild(EnsView v) {
   double* pfn = c.take<double>(v.nqp);
   char* pfscr = (char*)c.take<char>(pf_bytes(v.n, v.nqp));
   const int tid = threadIdx.x;
-  double* work = v.work + (size_t)blockIdx.x * pfinv_work_doubles(v.n, v.nqp);
+  double* work = v.work + (size_t)slot * pfinv_work_doubles(v.n, v.nqp);
   const int nhand = *v.nflag;  // walkers handed over by the k_block launch before
   __shared__ int next_i;
   for (;;) {
@@ -1240,7 +1264,7 @@ __global__ void __launch_bounds__(256, MINB) k_rebuild(EnsView v) {
     __syncthreads();
     double* planes = v.inv + (size_t)w * v.nqp * v.n * v.n;
     const int bad = wg_calc_m_all<JB>(v, ei, rs, planes, work, pfn, pfscr, false,
-                                  v.clk ? v.clk + (size_t)blockIdx.x * 32 + 16 : nullptr);
+                                  v.clk ? v.clk + (size_t)slot * 32 + 16 : nullptr);
     copy_inv32(v, w);
     if (tid == 0) {
       if (!bad) {
@@ -1293,8 +1317,9 @@ __device__ double direct_proj_ratio_moved(const EnsView& v, int source, int dest
 // Rebuild for measurement: the inverse and Pfaffians of saved sample s of
 // every walker into the walker's planes and pf; s_valid[w][s] = the sample
 // is not the all-zero / all-negative placeholder and calc_m_all succeeded.
-template <int MINB, int JB>
-__global__ void __launch_bounds__(256, MINB) k_meas_rebuild(EnsView v, int s) {
+template <int MINB, int JB, bool GROUP>
+__global__ void __launch_bounds__(256, MINB) k_meas_rebuild(EnsView v_arg, GroupArgs ga, int s) {
+  ENS_SLOT(v_arg, ga)
   extern __shared__ __align__(16) char lds[];
   Carver c{lds};
   int* ei = c.take<int>(v.n);
@@ -1302,9 +1327,9 @@ __global__ void __launch_bounds__(256, MINB) k_meas_rebuild(EnsView v, int s) {
   double* pfn = c.take<double>(v.nqp);
   char* pfscr = (char*)c.take<char>(pf_bytes(v.n, v.nqp));
   const int tid = threadIdx.x;
-  double* work = v.work + (size_t)blockIdx.x * pfinv_work_doubles(v.n, v.nqp);
+  double* work = v.work + (size_t)slot * pfinv_work_doubles(v.n, v.nqp);
   __shared__ int flag;
-  for (int w = blockIdx.x; w < v.W; w += gridDim.x) {
+  for (int w = slot; w < v.W; w += nslots) {
     const size_t ws = (size_t)w * v.nsample + s;
     for (int i = tid; i < v.n; i += blockDim.x) ei[i] = v.s_idx[ws * v.n + i];
     __syncthreads();
@@ -1328,8 +1353,9 @@ __global__ void __launch_bounds__(256, MINB) k_meas_rebuild(EnsView v, int s) {
   }
 }
 
-__global__ void __launch_bounds__(256, 2) k_measure(EnsView v, double* gscratch, int gscratch_stride, int mode_exp,
-                                                   int s_only) {
+template <bool GROUP>
+__global__ void __launch_bounds__(256, 2) k_measure(EnsView v_arg, GroupArgs ga, int mode_exp, int s_only) {
+  ENS_SLOT(v_arg, ga)
   extern __shared__ __align__(16) char lds[];
   Carver c{lds};
   int* ei = c.take<int>(v.n);
@@ -1345,10 +1371,10 @@ __global__ void __launch_bounds__(256, 2) k_measure(EnsView v, double* gscratch,
   int* flags = c.take<int>(64);
   const int tid = threadIdx.x, nt = blockDim.x;
   const int n = v.n, ne = v.ne, ns = v.ns, n2 = v.n2, nqp = v.nqp, nn = n * n;
-  double* work = v.work + (size_t)blockIdx.x * pfinv_work_doubles(n, nqp);
-  double* qorb = v.qorb + (size_t)blockIdx.x * nqp * v.nslater;
-  double* g1 = gscratch + (size_t)blockIdx.x * gscratch_stride;  // per term
-  for (int w = blockIdx.x; w < v.W; w += gridDim.x) {
+  double* work = v.work + (size_t)slot * pfinv_work_doubles(n, nqp);
+  double* qorb = v.qorb + (size_t)slot * nqp * v.nslater;
+  double* g1 = v.gscr + (size_t)slot * v.gstride;  // per term
+  for (int w = slot; w < v.W; w += nslots) {
     double* planes = v.inv + (size_t)w * nqp * nn;
     for (int s = s_only; s == s_only; ++s) {
       const size_t ws = (size_t)w * v.nsample + s;

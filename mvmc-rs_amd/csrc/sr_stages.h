@@ -1,0 +1,176 @@
+// SrStages of the C order besides the CG step, on device operands
+// (crates/mvmc-core/src/sr_backend.rs:193-292, COrderSr :429-516):
+//   k_sr_assemble_stage   assemble_s_g   (S/g of the direct solve, stcopt.c:69)
+//   k_gram_complex        gram_complex   (G = O O^H, samples summed in order)
+//   k_ssg_*               sr_s_g         (weighted covariance S and force g of
+//                                          the validation harness, over centered())
+// Every output element keeps the reference's operation sequence (products
+// rounded separately, sums over samples in ascending order; built with
+// -ffp-contract=off); only independent elements run in parallel.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace mvmc_hip {
+
+// Real view of a real array or of the real parts of a complex one
+// (sr_backend.rs:53-70).
+__device__ __forceinline__ double rview(const double* p, int cplx, long long i) { return cplx ? p[2 * i] : p[i]; }
+
+// assemble_s_g (sr_backend.rs:484-516): element e < nm*nm is S column sj =
+// e / nm, row si = e % nm; e in [nm*nm, nm*nm + nm) is g[e - nm*nm].
+// m2dt = -2 * step_dt (formed on the host, as the reference's left-to-right
+// product does).
+__global__ void k_sr_assemble_stage(const double* oo, int oo_c, const double* ho, int ho_c, const int* map, int nm,
+                                    long long ld, int off, double ratio_diag, double m2dt, double* s, double* g) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nn = (long long)nm * nm;
+  if (e < nn) {
+    const int sj = (int)(e / nm), si = (int)(e % nm);
+    const long long pi = map[si], pj = map[sj];
+    const double tmp = rview(oo, oo_c, pi + off);
+    double v = rview(oo, oo_c, (pi + off) * ld + (pj + off)) - tmp * rview(oo, oo_c, pj + off);
+    if (si == sj) v *= ratio_diag;
+    s[e] = v;
+  } else if (e < nn + nm) {
+    const int si = (int)(e - nn);
+    const long long pi = map[si];
+    const double v = rview(ho, ho_c, pi + off) - rview(ho, ho_c, 0) * rview(oo, oo_c, pi + off);
+    g[si] = m2dt * v;
+  }
+}
+
+// gram_complex (sr_backend.rs:386-420): G[i + j n] = sum_s O[i,s] conj(O[j,s])
+// over a [samples][n] complex store (re, im interleaved).  A workgroup of
+// 16 x 16 threads owns a 64 x 64 tile of G, each thread a 4 x 4 block
+// (rows i0 + ty + 16a, columns j0 + tx + 16b); samples stream through LDS in
+// chunks of GR_KC, and every entry adds its products in sample order:
+// pr = o_i.re c.re - o_i.im c.im, pi = o_i.re c.im + o_i.im c.re with
+// c = conj(o_j) (the num_complex product).
+constexpr int GR_T = 64, GR_KC = 16;
+__global__ void __launch_bounds__(256) k_gram_complex(const double* store, int n, int samples, double* out) {
+  __shared__ double2 ti[GR_KC][GR_T];
+  __shared__ double2 tj[GR_KC][GR_T];
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  const int i0 = blockIdx.x * GR_T, j0 = blockIdx.y * GR_T;
+  const double2* o = reinterpret_cast<const double2*>(store);
+  double ar[4][4], ai[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) ar[a][b] = ai[a][b] = 0.0;
+  for (int s0 = 0; s0 < samples; s0 += GR_KC) {
+    for (int e = threadIdx.x; e < GR_KC * GR_T; e += 256) {
+      const int ks = e / GR_T, c = e % GR_T;
+      const int s = s0 + ks;
+      const bool in_s = s < samples;
+      ti[ks][c] = (in_s && i0 + c < n) ? o[(size_t)s * n + i0 + c] : make_double2(0.0, 0.0);
+      tj[ks][c] = (in_s && j0 + c < n) ? o[(size_t)s * n + j0 + c] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    const int kmax = min(GR_KC, samples - s0);
+    for (int ks = 0; ks < kmax; ++ks) {
+      double2 vi[4], vj[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) vi[a] = ti[ks][ty + 16 * a];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) vj[b] = tj[ks][tx + 16 * b];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const double cr = vj[b].x, ci = -vj[b].y;
+          const double pr = vi[a].x * cr - vi[a].y * ci;
+          const double pim = vi[a].x * ci + vi[a].y * cr;
+          ar[a][b] += pr;
+          ai[a][b] += pim;
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int i = i0 + ty + 16 * a, j = j0 + tx + 16 * b;
+      if (i < n && j < n) {
+        out[2 * ((size_t)j * n + i)] = ar[a][b];
+        out[2 * ((size_t)j * n + i) + 1] = ai[a][b];
+      }
+    }
+}
+
+// sr_s_g, centered() (stage_backend.rs:167-184): thread p < np forms the
+// weighted mean of column p (k ascending) and writes d[., p] = o[., p] - mean;
+// thread np forms <E>_w and writes de.  o, d: column-major [ns][np].
+__global__ void k_ssg_center(const double* o, int ns, int np, const double* e, const double* w, double* d,
+                             double* de) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < np) {
+    const double* col = o + (size_t)p * ns;
+    double mean = 0.0;
+    for (int k = 0; k < ns; ++k) mean += w[k] * col[k];
+    for (int k = 0; k < ns; ++k) d[(size_t)p * ns + k] = col[k] - mean;
+  } else if (p == np) {
+    double em = 0.0;
+    for (int k = 0; k < ns; ++k) em += w[k] * e[k];
+    for (int k = 0; k < ns; ++k) de[k] = e[k] - em;
+  }
+}
+
+// S[p + q np] = sum_k (w_k d_kp) d_kq (sr_backend.rs:447-452): a 64 x 64 tile
+// per workgroup, 4 x 4 entries per thread, k streamed through LDS in order.
+__global__ void __launch_bounds__(256) k_ssg_s(const double* d, const double* w, int ns, int np, double* s) {
+  __shared__ double tp[GR_KC][GR_T];
+  __shared__ double tq[GR_KC][GR_T];
+  __shared__ double tw[GR_KC];
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  const int p0 = blockIdx.x * GR_T, q0 = blockIdx.y * GR_T;
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+  for (int k0 = 0; k0 < ns; k0 += GR_KC) {
+    for (int e = threadIdx.x; e < GR_KC * GR_T; e += 256) {
+      const int kk = e % GR_KC, c = e / GR_KC;   // k fastest: coalesced along a column
+      const int k = k0 + kk;
+      tp[kk][c] = (k < ns && p0 + c < np) ? d[(size_t)(p0 + c) * ns + k] : 0.0;
+      tq[kk][c] = (k < ns && q0 + c < np) ? d[(size_t)(q0 + c) * ns + k] : 0.0;
+    }
+    if (threadIdx.x < GR_KC) tw[threadIdx.x] = k0 + (int)threadIdx.x < ns ? w[k0 + threadIdx.x] : 0.0;
+    __syncthreads();
+    const int kmax = min(GR_KC, ns - k0);
+    for (int kk = 0; kk < kmax; ++kk) {
+      const double wk = tw[kk];
+      double vp[4], vq[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) vp[a] = wk * tp[kk][ty + 16 * a];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) vq[b] = tq[kk][tx + 16 * b];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] += vp[a] * vq[b];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int p = p0 + ty + 16 * a, q = q0 + tx + 16 * b;
+      if (p < np && q < np) s[p + (size_t)q * np] = acc[a][b];
+    }
+}
+
+// g[q] = sum_k (w_k d_kq) de_k (sr_backend.rs:453-457), one thread per q.
+__global__ void k_ssg_g(const double* d, const double* w, const double* de, int ns, int np, double* g) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  const double* col = d + (size_t)q * ns;
+  double acc = 0.0;
+  for (int k = 0; k < ns; ++k) acc += w[k] * col[k] * de[k];
+  g[q] = acc;
+}
+
+}  // namespace mvmc_hip

@@ -141,6 +141,14 @@ void parse_modpara(const std::string& path, ModPara& p) {
   // readdef.c:927-957: NSRCG 2 selects the diagonally scaled CG.
   if (p.nsrcg == 2) { p.nsrcg = 1; p.use_diag_scale = 1; }
   if (p.nsrcg != 1) p.use_diag_scale = 0;
+  // readdef.c:959-975 (crates/mvmc-expert-parsers/src/parsers/modpara.rs:250-266):
+  // a negative DSROptStepDt selects the diagonalisation mode (SRFlag) and is
+  // negated before use; SR-CG has no such mode.  The direct solve is the same
+  // Cholesky solve in both modes (only the SRinfo header differs in C).
+  p.sr_flag = p.dsr_opt_step_dt < 0.0 ? 1 : 0;
+  if (p.sr_flag) p.dsr_opt_step_dt *= -1.0;
+  if (p.sr_flag && p.nsrcg != 0)
+    throw std::runtime_error(path + ": SR-CG cannot be combined with a negative DSROptStepDt");
 }
 
 int complex_type(const std::vector<std::string>& lines, const std::string& path) {

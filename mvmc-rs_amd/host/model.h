@@ -65,6 +65,7 @@ struct ModPara {
   int nsrcg_fallback = 0;
   int nsrcg_abort_on_fail = 1;
   int rescale_smat = 0;
+  int sr_flag = 0;  // diagonalisation mode (negative DSROptStepDt)
 };
 
 struct Model {

@@ -16,6 +16,11 @@ Workloads (sizes of the configurations they are named after):
   square16   16x16 square, t=1, t'=-0.3, U=8, 112+112 (1/8 hole doping),
              4x4-sublattice orbitals (4096), per-sublattice Gutzwiller (16),
              (sublattice, displacement) Jastrow; 8 QP planes
+  uneven     (write_uneven_batch) 24 independent L x L square-lattice runs,
+             L in {6, 8, 10, 12} (6 each), U ~ uniform[2, 10] and filling ~
+             uniform[0.75, 1.0] from a seed (Nsize 28..144, equal spin
+             counts), 2x2-sublattice orbitals, Gutzwiller, distance Jastrow,
+             4 QP planes (4 translations), 2048 walkers each
 Initial parameters are uniform[-0.1, 0.1] from a generator seed, written to
 initial.def (six leading numbers, then (re, im, 0) per parameter).
 """
@@ -33,6 +38,8 @@ WORKLOADS = {
 }
 
 HEADER = "=============================================\n"
+
+UNEVEN_SIDES = (6, 8, 10, 12)
 
 
 def _sites(kind, L):
@@ -207,3 +214,31 @@ def write_workload(name, out_dir, seed=2024, sr_steps=10, nvmc_sample=1, interva
     return {"namelist": os.path.join(out_dir, "namelist.def"), "initial": os.path.join(out_dir, "initial.def"),
             "nsite": ns, "nsize": nelec_total, "n_gutz": ng, "n_jast": nj, "n_slater": no, "npara": npara,
             "nqp": cfg["spgl"] * len(shifts), "walkers": cfg["walkers"], "ntransfer": len(trans)}
+
+
+def write_uneven_batch(out_dir, seed=24, walkers=2048, runs_per_side=6, **kw):
+    """The uneven batch: runs_per_side runs per side L in UNEVEN_SIDES, each a
+    square L x L Hubbard model with t = 1, U ~ uniform[2, 10], filling ~
+    uniform[0.75, 1.0] (electrons rounded to an even count with equal spins,
+    at least 0.75 of the sites), 2x2-sublattice orbitals, Gutzwiller,
+    distance Jastrow and 4 translation QP planes.  Every run gets its own
+    directory run<k>/ and parameter seed (seed + k); keyword arguments go to
+    write_workload (SR steps, warm-up, ...).  Returns the runs' infos, each
+    with its "U" and "filling"."""
+    rng = np.random.default_rng(seed)
+    infos = []
+    k = 0
+    for L in UNEVEN_SIDES:
+        for _ in range(runs_per_side):
+            ns = L * L
+            U = float(rng.uniform(2.0, 10.0))
+            f = float(rng.uniform(0.75, 1.0))
+            nup = int(round(f * ns / 2))
+            nup = max(-(-3 * ns // 8), min(ns // 2, nup))
+            ov = dict(kind="square", L=L, t=1.0, tp=0.0, U=U, nup=nup, sub=(2, 2), spgl=1, ntrans=4,
+                      walkers=walkers, jastrow="distance", gutz="single")
+            info = write_workload("square8", os.path.join(out_dir, f"run{k}"), seed=seed + k, overrides=ov, **kw)
+            info.update(U=U, filling=2 * nup / ns, L=L)
+            infos.append(info)
+            k += 1
+    return infos

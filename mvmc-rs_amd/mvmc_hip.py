@@ -22,15 +22,15 @@ _lib = None
 
 
 class BackendUnavailable(RuntimeError):
-    """BatchedError::BackendUnavailable (crates/mvmc-gpu/src/lib.rs:83-101)."""
+    """mvmc_gpu::Error::BackendUnavailable (crates/mvmc-gpu/src/lib.rs:83-97)."""
 
 
 class InvalidShape(ValueError):
-    """BatchedError::InvalidShape."""
+    """mvmc_gpu::Error::InvalidShape."""
 
 
 class BackendError(RuntimeError):
-    """BatchedError::Backend / StageError::Failed."""
+    """mvmc_gpu::Error::Backend / StageError::Failed."""
 
 
 def lib():
@@ -59,6 +59,21 @@ def lib():
              ctypes.POINTER(ctypes.c_longlong), ctypes.c_void_p]
         L.mvmc_sr_cg_apply.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 4 + \
             [ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
+        L.mvmc_sr_assemble_s_g.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                           ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_int,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_void_p]
+        L.mvmc_sr_gram_complex.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_void_p]
+        L.mvmc_sr_s_g.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 5
+        L.mvmc_group_create.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.POINTER(ctypes.c_void_p)]
+        for name in ["mvmc_group_destroy", "mvmc_group_sample"]:
+            getattr(L, name).argtypes = [ctypes.c_void_p]
+        L.mvmc_group_run.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        L.mvmc_group_set_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.mvmc_group_opt_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.mvmc_group_timings.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.mvmc_ens_launch_log.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         L.mvmc_ens_write_opt_data.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
         L.mvmc_ens_set_output_dir.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
@@ -70,7 +85,9 @@ def lib():
 
 
 EXPORTED_SYMBOLS = [
-    "mvmc_last_error", "mvmc_abi_version", "mvmc_ens_launch_log", "mvmc_pfinv_f64_noalloc", "mvmc_pfinv_f64", "mvmc_pfinv_f64_inplace", "mvmc_ens_create",
+    "mvmc_last_error", "mvmc_abi_version", "mvmc_ens_launch_log", "mvmc_pfinv_f64_noalloc",
+    "mvmc_sr_assemble_s_g", "mvmc_sr_gram_complex", "mvmc_sr_s_g", "mvmc_group_create", "mvmc_group_destroy",
+    "mvmc_group_run", "mvmc_group_set_graph", "mvmc_group_opt_step", "mvmc_group_sample", "mvmc_group_timings", "mvmc_pfinv_f64", "mvmc_pfinv_f64_inplace", "mvmc_ens_create",
     "mvmc_ens_destroy", "mvmc_ens_dims", "mvmc_ens_get_params", "mvmc_ens_set_params", "mvmc_ens_sample",
     "mvmc_ens_measure", "mvmc_ens_sr_cg", "mvmc_ens_opt_step", "mvmc_ens_get_energy", "mvmc_ens_get_operands",
     "mvmc_ens_get_o_store", "mvmc_ens_get_walker", "mvmc_ens_get_tables", "mvmc_ens_get_config", "mvmc_ens_get_cg",
@@ -141,11 +158,16 @@ def pfaffian_inverse_batched_c128(planes_dev, pf_dev, inv_dev, status_dev, n, co
 class Ensemble:
     """W walkers of one Expert-mode model resident on the GPU."""
 
-    def __init__(self, namelist, initial_def=None, walkers=1, seed=None):
-        h = ctypes.c_void_p()
-        rc = lib().mvmc_ens_create(namelist.encode(), initial_def.encode() if initial_def else None, int(walkers),
-                                   0 if seed is None else int(seed), 0 if seed is None else 1, ctypes.byref(h))
-        _check(rc)
+    def __init__(self, namelist, initial_def=None, walkers=1, seed=None, _handle=None):
+        if _handle is None:
+            h = ctypes.c_void_p()
+            rc = lib().mvmc_ens_create(namelist.encode(), initial_def.encode() if initial_def else None, int(walkers),
+                                       0 if seed is None else int(seed), 0 if seed is None else 1, ctypes.byref(h))
+            _check(rc)
+            self._owned = True
+        else:
+            h = _handle         # a run owned by an EnsembleGroup
+            self._owned = False
         self.h = h
         d = np.zeros(16, dtype=np.int32)
         lib().mvmc_ens_dims(self.h, _p(d))
@@ -155,7 +177,8 @@ class Ensemble:
 
     def close(self):
         if getattr(self, "h", None):
-            lib().mvmc_ens_destroy(self.h)
+            if getattr(self, "_owned", True):
+                lib().mvmc_ens_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -307,6 +330,55 @@ class Ensemble:
         return int(lib().mvmc_ens_status(self.h))
 
 
+class EnsembleGroup:
+    """Independent runs (Expert-mode models of uneven Nsize) stepped together:
+    one launch per kernel class over all runs' walkers (mvmc_group_*)."""
+
+    def __init__(self, namelists, initial_defs=None, walkers=None):
+        R = len(namelists)
+        inits = initial_defs or [None] * R
+        walkers = walkers or [1] * R
+        nl = (ctypes.c_char_p * R)(*[x.encode() for x in namelists])
+        ini = (ctypes.c_char_p * R)(*[x.encode() if x else None for x in inits])
+        wk = np.ascontiguousarray(walkers, dtype=np.int32)
+        h = ctypes.c_void_p()
+        _check(lib().mvmc_group_create(R, nl, ini, _p(wk), ctypes.byref(h)))
+        self.h = h
+        self.runs = []
+        for r in range(R):
+            e = ctypes.c_void_p()
+            _check(lib().mvmc_group_run(self.h, r, ctypes.byref(e)))
+            self.runs.append(Ensemble(None, _handle=e))
+
+    def set_graph(self, rounds):
+        _check(lib().mvmc_group_set_graph(self.h, int(rounds)))
+
+    def opt_step(self):
+        rows = np.zeros((len(self.runs), 6))
+        _check(lib().mvmc_group_opt_step(self.h, _p(rows)))
+        return rows
+
+    def sample(self):
+        _check(lib().mvmc_group_sample(self.h))
+
+    def timings(self):
+        t = np.zeros(10)
+        lib().mvmc_group_timings(self.h, _p(t))
+        return {"sample_s": t[0], "measure_s": t[1], "sr_s": t[2], "round_ms": t[3], "rounds": int(t[4]),
+                "graph_replays": int(t[5]), "grid_sampler": int(t[6]), "grid_rebuild": int(t[7]),
+                "grid_measure": int(t[8]), "lds_sampler": int(t[9])}
+
+    def close(self):
+        if getattr(self, "h", None):
+            for e in self.runs:
+                e.h = None
+            lib().mvmc_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
 def sr_cg_apply(pack_dev, mean_dev, diag_dev, x_dev, z_dev, na, walkers, nsample, inv_w, shift, stream=None):
     """SrStages::cg_local_product: z = S x of the sampled operator on device
     operands (torch CUDA tensors)."""
@@ -314,6 +386,32 @@ def sr_cg_apply(pack_dev, mean_dev, diag_dev, x_dev, z_dev, na, walkers, nsample
                                   ctypes.c_void_p(mean_dev.data_ptr()), ctypes.c_void_p(diag_dev.data_ptr()),
                                   ctypes.c_void_p(x_dev.data_ptr()), inv_w, shift,
                                   ctypes.c_void_p(z_dev.data_ptr()), ctypes.c_void_p(stream or 0)))
+
+
+def sr_assemble_s_g(oo_dev, ho_dev, map_dev, nm, ld, offset, sta_del, step_dt, s_dev, g_dev, oo_complex=False,
+                    ho_complex=False, stream=None):
+    """SrStages::assemble_s_g on device operands (torch CUDA tensors): S
+    column-major [nm][nm] and g of the direct solve."""
+    _check(lib().mvmc_sr_assemble_s_g(ctypes.c_void_p(oo_dev.data_ptr()), int(oo_complex),
+                                      ctypes.c_void_p(ho_dev.data_ptr()), int(ho_complex),
+                                      ctypes.c_void_p(map_dev.data_ptr()), nm, ld, offset, sta_del, step_dt,
+                                      ctypes.c_void_p(s_dev.data_ptr()), ctypes.c_void_p(g_dev.data_ptr()),
+                                      ctypes.c_void_p(stream or 0)))
+
+
+def sr_gram_complex(store_dev, out_dev, n, samples, stream=None):
+    """SrStages::gram_complex: out[i + j n] = sum_s O[i,s] conj(O[j,s]) of the
+    complex [samples][n] store (torch complex128 CUDA tensors)."""
+    _check(lib().mvmc_sr_gram_complex(n, samples, ctypes.c_void_p(store_dev.data_ptr()),
+                                      ctypes.c_void_p(out_dev.data_ptr()), ctypes.c_void_p(stream or 0)))
+
+
+def sr_s_g(o_dev, e_dev, w_dev, s_dev, g_dev, ns, np_, stream=None):
+    """SrStages::sr_s_g: o column-major [ns][np] (torch CUDA tensors); S
+    column-major [np][np] and g."""
+    _check(lib().mvmc_sr_s_g(ctypes.c_void_p(o_dev.data_ptr()), ns, np_, ctypes.c_void_p(e_dev.data_ptr()),
+                             ctypes.c_void_p(w_dev.data_ptr()), ctypes.c_void_p(s_dev.data_ptr()),
+                             ctypes.c_void_p(g_dev.data_ptr()), ctypes.c_void_p(stream or 0)))
 
 
 def sr_gram_real(store_dev, out_dev, n, samples, stream=None):

@@ -129,6 +129,15 @@ void read_modpara(const std::string& path, Params& p) {
     p.nsrcg = 1;
   }
   p.use_diag_scale = (diag_scale != 0 && p.nsrcg == 1) ? 1 : 0;
+  // readdef.c:959-975 (modpara.rs:250-266): a negative DSROptStepDt selects
+  // the diagonalisation mode (SRFlag) and is negated; SR-CG has no such mode
+  if (p.dsr_opt_step_dt < 0.0) {
+    p.sr_flag = 1;
+    p.dsr_opt_step_dt *= -1.0;
+  } else {
+    p.sr_flag = 0;
+  }
+  if (p.sr_flag && p.nsrcg != 0) bad(path, "SR-CG cannot be combined with a negative DSROptStepDt");
 }
 
 // (index, flag) pairs in file order: the flag of parameter k is the k-th

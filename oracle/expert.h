@@ -37,6 +37,7 @@ struct Params {  // ModPara keywords used by the path
   long long rnd_seed = 11272;
   int nstore = 1, nsrcg = 0, use_diag_scale = 0, nsrcg_fallback = 0, nsrcg_abort_on_fail = 1;
   int rescale_smat = 0;
+  int sr_flag = 0;  // diagonalisation mode (negative DSROptStepDt)
 };
 
 struct Model {

@@ -18,8 +18,16 @@
 //   * local energy:          brute force, every Green ratio against full
 //                            Pfaffians of the moved configuration
 //   * complex Pfaffian:      see pfapack_c.h
+//   * direct SR (NSRCG 0):   full <OO>, <HO> and rank states of the native-C
+//                            direct cells (tests/golden/mpi179_real_d*.json);
+//                            S/g assembly + Cholesky against the C dposv
+//                            solutions (tests/golden/negative_stepdt_c_sr.json)
+//   * gram_complex, sr_s_g:  exact sums on integer / dyadic inputs, exact
+//                            Hermitian symmetry
 // Parity unpinned: the inverse against OpenBLAS dtrtri/dtrmm bits (residual
-// checks only), glibc exp/log bits on a different glibc version.
+// checks only), the Cholesky solve against OpenBLAS dpotrf/dpotrs bits
+// (within 1e-13 of the C solutions), glibc exp/log bits on a different glibc
+// version.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -897,6 +905,18 @@ static void measure_walker(const Run& R, Walker& W) {
       for (int i = 0; i < size; ++i) W.ho[i] += we * o[i];
     }
   }
+  if (use_store && m.mp.vmc_calc_mode == 0 && m.mp.nsrcg == 0) {
+    // C calculateOO_Store_real (vmccal.c:714-718, sr_backend.rs:305-333): the
+    // full Gram of the stored rows, each entry summed over the samples in
+    // order (DGEMM's own order is the BLAS build's)
+    for (int j = 0; j < size; ++j)
+      for (int i = 0; i < size; ++i) {
+        double acc = 0.0;
+        for (int s = 0; s < W.n_saved; ++s)
+          acc += W.o_store[(size_t)s * size + i] * W.o_store[(size_t)s * size + j];
+        W.oo[(size_t)j * size + i] = acc;
+      }
+  }
   if (use_store && m.mp.vmc_calc_mode == 0 && m.mp.nsrcg != 0) {
     for (int i = 0; i < size; ++i) {
       double mean = 0.0, diag = 0.0;
@@ -1129,6 +1149,176 @@ static int stochastic_opt_cg(Run& R) {
 }
 
 // ---------------------------------------------------------------------------
+// Direct SR (NSRCG 0) and the other C-order SrStages
+// ---------------------------------------------------------------------------
+// A real view of a real array or of the real parts of a complex one
+// (sr_backend.rs:53-70 RealView).
+struct RView {
+  const double* p;
+  int cplx;
+  double at(size_t i) const { return cplx ? p[2 * i] : p[i]; }
+};
+
+// COrderSr::assemble_s_g (sr_backend.rs:484-516, C stcopt.c:69): column sj of S,
+// S[si][sj] = oo[(pi+off) ld + (pj+off)] - oo[pi+off] oo[pj+off], the diagonal
+// times 1 + sta_del; g[si] = (-2 dt) (ho[pi+off] - ho[0] oo[pi+off]).
+static void sr_assemble_s_g(RView oo, RView ho, const int* map, int nm, size_t ld, int off, double sta_del,
+                            double step_dt, double* s, double* g) {
+  const double ratio_diag = 1.0 + sta_del;
+  for (int sj = 0; sj < nm; ++sj) {
+    const size_t pj = (size_t)map[sj];
+    for (int si = 0; si < nm; ++si) {
+      const size_t pi = (size_t)map[si];
+      const double tmp = oo.at(pi + off);
+      double v = oo.at((pi + off) * ld + (pj + off)) - tmp * oo.at(pj + off);
+      if (si == sj) v *= ratio_diag;
+      s[(size_t)sj * nm + si] = v;
+    }
+  }
+  const double ho0 = ho.at(0);
+  for (int si = 0; si < nm; ++si) {
+    const size_t pi = (size_t)map[si];
+    const double v = ho.at(pi + off) - ho0 * oo.at(pi + off);
+    g[si] = -2.0 * step_dt * v;
+  }
+}
+
+// sr.rs:535-577 cholesky_solve: LAPACK dpotrf('U') + dpotrs('U') restated in
+// the reference-BLAS unblocked order (dpotf2: ddot, dgemv 'T', dscal by the
+// reciprocal; dtrsm 'L','U','T','N' then 'L','U','N','N').  The reference
+// links OpenBLAS, whose blocked kernels sum in another order: bit-level
+// parity with it is unpinned (pinned to the C dposv solutions within
+// 1e-15 + 1e-13 |x|, tests/golden/negative_stepdt_c_sr.txt).  Returns 0, or
+// the factorisation's positive info, or -1 for a non-finite solution.
+static int cholesky_solve(double* a, double* b, int n) {
+  auto A = [&](int i, int j) -> double& { return a[i + (size_t)j * n]; };
+  for (int j = 0; j < n; ++j) {
+    double dot = 0.0;
+    for (int k = 0; k < j; ++k) dot += A(k, j) * A(k, j);
+    double ajj = A(j, j) - dot;
+    if (ajj <= 0.0 || std::isnan(ajj)) {
+      A(j, j) = ajj;
+      return j + 1;
+    }
+    ajj = std::sqrt(ajj);
+    A(j, j) = ajj;
+    if (j + 1 < n) {
+      for (int c = j + 1; c < n; ++c) {
+        double t = 0.0;
+        for (int k = 0; k < j; ++k) t += A(k, c) * A(k, j);
+        A(j, c) = A(j, c) + (-1.0) * t;
+      }
+      const double r = 1.0 / ajj;
+      for (int c = j + 1; c < n; ++c) A(j, c) = r * A(j, c);
+    }
+  }
+  for (int i = 0; i < n; ++i) {  // U^T y = b
+    double t = b[i];
+    for (int k = 0; k < i; ++k) t -= A(k, i) * b[k];
+    b[i] = t / A(i, i);
+  }
+  for (int k = n - 1; k >= 0; --k) {  // U x = y
+    if (b[k] != 0.0) {
+      b[k] = b[k] / A(k, k);
+      for (int i = 0; i < k; ++i) b[i] -= b[k] * A(i, k);
+    }
+  }
+  for (int i = 0; i < n; ++i)
+    if (!std::isfinite(b[i])) return -1;
+  return 0;
+}
+
+// COrderSr::gram_complex (sr_backend.rs:386-420, gram_block :349-384):
+// G[i + j n] = sum_s O[i,s] conj(O[j,s]), samples in order, each product the
+// num_complex one with separately rounded parts.  store: [samples][n] complex.
+static void sr_gram_complex(const double* store, int n, int samples, double* out) {
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      double re = 0.0, im = 0.0;
+      for (int s = 0; s < samples; ++s) {
+        const double* o = store + 2 * (size_t)s * n;
+        const double cr = o[2 * j], ci = -o[2 * j + 1];
+        const double pr = o[2 * i] * cr - o[2 * i + 1] * ci;
+        const double pim = o[2 * i] * ci + o[2 * i + 1] * cr;
+        re += pr;
+        im += pim;
+      }
+      out[2 * ((size_t)j * n + i)] = re;
+      out[2 * ((size_t)j * n + i) + 1] = im;
+    }
+}
+
+// COrderSr::sr_s_g (sr_backend.rs:436-460) over centered() (stage_backend.rs:167-184):
+// d = O - <O>_w per column, de = E - <E>_w; S[p + q np] = sum_k (w_k d_kp) d_kq,
+// g[q] = sum_k (w_k d_kq) de_k, k in order.  o: column-major [ns][np].
+static void sr_s_g(const double* o, int ns, int np, const double* e, const double* w, double* s, double* g) {
+  vector<double> d(o, o + (size_t)ns * np), de(ns);
+  for (int p = 0; p < np; ++p) {
+    double mean = 0.0;
+    for (int k = 0; k < ns; ++k) mean += w[k] * o[k + (size_t)p * ns];
+    for (int k = 0; k < ns; ++k) d[k + (size_t)p * ns] -= mean;
+  }
+  double emean = 0.0;
+  for (int k = 0; k < ns; ++k) emean += w[k] * e[k];
+  for (int k = 0; k < ns; ++k) de[k] = e[k] - emean;
+  for (int q = 0; q < np; ++q) {
+    for (int p = 0; p < np; ++p) {
+      double acc = 0.0;
+      for (int k = 0; k < ns; ++k) acc += w[k] * d[k + (size_t)p * ns] * d[k + (size_t)q * ns];
+      s[p + (size_t)q * np] = acc;
+    }
+    double acc = 0.0;
+    for (int k = 0; k < ns; ++k) acc += w[k] * d[k + (size_t)q * ns] * de[k];
+    g[q] = acc;
+  }
+}
+
+// stochastic_opt_real (sr.rs:60-160): redundancy cut on the diagonal of the
+// full <OO> (collect_active_real / select_active_real, sr.rs:361-406: the
+// maximum folded from 0.0), S/g assembly, Cholesky solve, parameter update.
+static int stochastic_opt_direct(Run& R) {
+  Model& m = R.m;
+  const int np = m.n_para(), size = m.sr_opt_size();
+  if (np == 0) return 0;
+  const vector<double>& oo = R.oo;
+  vector<double> sd(np, 0.0);
+  bool nan = false;
+  for (int pi = 0; pi < np; ++pi) {
+    const size_t idx_diag = (size_t)(pi + 1) * size + (pi + 1), idx0 = (size_t)(pi + 1);
+    if (idx_diag < oo.size() && idx0 < oo.size()) sd[pi] = oo[idx_diag] - oo[idx0] * oo[idx0];
+    nan = nan || std::isnan(sd[pi]);
+  }
+  double vmax = 0.0;
+  if (nan) vmax = NAN;
+  else
+    for (double v : sd) vmax = std::fmax(vmax, v);
+  const double cut = vmax * m.mp.dsr_opt_red_cut;
+  vector<int> map;
+  for (int pi = 0; pi < np; ++pi) {
+    if (m.opt_flags[2 * pi] != 1) continue;
+    if (sd[pi] < cut) continue;
+    map.push_back(pi);
+  }
+  if (map.empty()) return 0;
+  const int nm = (int)map.size();
+  vector<double> S((size_t)nm * nm), g(nm);
+  sr_assemble_s_g(RView{oo.data(), 0}, RView{R.ho.data(), 0}, map.data(), nm, (size_t)size, 1, m.mp.dsr_opt_sta_del,
+                  m.mp.dsr_opt_step_dt, S.data(), g.data());
+  R.cg_status = 0;
+  if (cholesky_solve(S.data(), g.data(), nm) != 0) {
+    R.cg_solution.clear();
+    return 1;
+  }
+  R.cg_solution = g;
+  for (int si = 0; si < nm; ++si) {
+    const int pi = map[si];
+    if (pi < m.n_proj()) m.proj[pi] += g[si];
+    else m.slater[pi - m.n_proj()] += g[si];
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
 // Optimisation driver: crates/mvmc-core/src/run.rs:1294-1475 with W walkers
 // (multichain.rs:647-689, rank-order left-fold reductions)
 // ---------------------------------------------------------------------------
@@ -1221,7 +1411,7 @@ static int opt_step(Run& R, bool do_sr) {
   double t3 = now_s();
   int rc = 0;
   if (m.mp.nsrcg != 0) rc = stochastic_opt_cg(R);
-  else throw std::runtime_error("the oracle implements the SR-CG solver only (NSRCG 1)");
+  else rc = stochastic_opt_direct(R);
   R.t_sr += now_s() - t3;
   if (rc != 0) return rc;
   sync_parameters(m, true);
@@ -1238,6 +1428,19 @@ using namespace oracle;
 extern "C" {
 
 void oracle_set_math(int cr) { g_cr_math = cr; }
+
+// SrStages of the C order on host arrays.
+void oracle_sr_assemble_s_g(const double* oo, int oo_cplx, const double* ho, int ho_cplx, const int* map, int nm,
+                            long long ld, int offset, double sta_del, double step_dt, double* s, double* g) {
+  sr_assemble_s_g(RView{oo, oo_cplx}, RView{ho, ho_cplx}, map, nm, (size_t)ld, offset, sta_del, step_dt, s, g);
+}
+int oracle_cholesky_solve(double* s, double* rhs, int n) { return cholesky_solve(s, rhs, n); }
+void oracle_sr_gram_complex(const double* store, int n, int samples, double* out) {
+  sr_gram_complex(store, n, samples, out);
+}
+void oracle_sr_s_g(const double* o, int ns, int np, const double* e, const double* w, double* s, double* g) {
+  sr_s_g(o, ns, np, e, w, s, g);
+}
 double oracle_cr_exp(double x) { return ddm::cr_exp(x); }
 double oracle_cr_log(double x) { return ddm::cr_log(x); }
 // Bulk evaluation: which = 0 exp (fast path + fallback), 1 exp accurate path

@@ -49,12 +49,64 @@ def lib():
         L.oracle_metropolis_word.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint32,
                                              ctypes.POINTER(ctypes.c_double)]
         L.oracle_julia_complex.restype = None
+        L.oracle_sr_assemble_s_g.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                             ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_int,
+                                             ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
+        L.oracle_cholesky_solve.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        L.oracle_sr_gram_complex.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+        L.oracle_sr_s_g.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_void_p]
         _lib = L
     return _lib
 
 
 def ptr(a):
     return ctypes.c_void_p(a.ctypes.data)
+
+
+def sr_assemble_s_g(oo, ho, amap, ld, offset, sta_del, step_dt):
+    """COrderSr::assemble_s_g: oo / ho real arrays or complex arrays (real parts
+    used); returns (S column-major [nm][nm], g)."""
+    oo_c, ho_c = np.iscomplexobj(oo), np.iscomplexobj(ho)
+    oo = np.ascontiguousarray(oo, dtype=np.complex128 if oo_c else np.float64)
+    ho = np.ascontiguousarray(ho, dtype=np.complex128 if ho_c else np.float64)
+    m = np.ascontiguousarray(amap, dtype=np.int32)
+    s = np.zeros(len(m) * len(m))
+    g = np.zeros(len(m))
+    lib().oracle_sr_assemble_s_g(ptr(oo), int(oo_c), ptr(ho), int(ho_c), ptr(m), len(m), int(ld), int(offset),
+                                 float(sta_del), float(step_dt), ptr(s), ptr(g))
+    return s.reshape(len(m), len(m)).T.copy(), g
+
+
+def cholesky_solve(s, rhs):
+    """sr.rs cholesky_solve (dpotrf/dpotrs 'U', unblocked reference order):
+    (info, x); info 0 ok, k > 0 not positive definite, -1 non-finite."""
+    a = np.asfortranarray(s, dtype=np.float64).copy(order="F")
+    b = np.ascontiguousarray(rhs, dtype=np.float64).copy()
+    info = lib().oracle_cholesky_solve(ptr(a), ptr(b), len(b))
+    return info, b
+
+
+def sr_gram_complex(store):
+    """COrderSr::gram_complex of a [samples][n] complex store: G (n x n, G[i, j])."""
+    st = np.ascontiguousarray(store, dtype=np.complex128)
+    samples, n = st.shape
+    out = np.zeros(n * n, dtype=np.complex128)
+    lib().oracle_sr_gram_complex(ptr(st), n, samples, ptr(out))
+    return out.reshape(n, n).T.copy()
+
+
+def sr_s_g(o, e, w):
+    """COrderSr::sr_s_g: o [ns][np] (row = sample), e, w [ns]; returns (S [np][np], g)."""
+    o = np.asarray(o, dtype=np.float64)
+    ns, npar = o.shape
+    ocm = np.ascontiguousarray(o.T)   # column-major [ns][np]: column p contiguous
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    s = np.zeros(npar * npar)
+    g = np.zeros(npar)
+    lib().oracle_sr_s_g(ptr(ocm), ns, npar, ptr(e), ptr(w), ptr(s), ptr(g))
+    return s.reshape(npar, npar).T.copy(), g
 
 
 def set_math(cr):

@@ -8,9 +8,10 @@ Everything written here is data held by the reference's tests:
   * Hubbard chain step-1 operands    tests/fixtures/c_order_sr_operands/hubbard-cg-store0.txt
     and its Expert inputs             tests/fixtures/c_order_sr_operands/inputs/hubbard
     (modpara overrides as in tests/fixtures/c_order_sr_operands/PROVENANCE.md)
-  * Heisenberg multi-rank state      tests/fixtures/mpi_matrix_179/real-cg-r{2,4}w1-s7
+  * Heisenberg multi-rank state      tests/fixtures/mpi_matrix_179/real-{cg,d0}-r{2,4}w1-s7, real-d1-r2w1-s7
     and its Expert inputs             tests/fixtures/native_c_physcal_181/heisenberg_chain_real/inputs
     (overrides from tests/fixtures/mpi_matrix_179/cells.txt)
+  * direct-SR solutions (dposv)      tests/fixtures/negative_stepdt/c_sr.txt
   * SR-CG iterates                   tests/fixtures/sr_cg/c_refresh/real{,_diag_scale}.txt
   * PfaPack input matrices           tests/fixtures/pfapack/real_n*.orig.txt
                                      tests/fixtures/pfapack/complex_n*.orig.txt
@@ -101,7 +102,7 @@ def heisenberg():
             "NStore": 0, "NSRCG": 1, "NSRCGAbortOnFail": 0}
     copy_inputs(os.path.join(FIX, "native_c_physcal_181/heisenberg_chain_real/inputs"),
                 os.path.join(OUT, "inputs/heisenberg_chain"), over)
-    for cell in ["real-cg-r2w1-s7", "real-cg-r4w1-s7"]:
+    for cell in ["real-cg-r2w1-s7", "real-cg-r4w1-s7", "real-d0-r2w1-s7", "real-d0-r4w1-s7", "real-d1-r2w1-s7"]:
         base = os.path.join(FIX, "mpi_matrix_179", cell)
         ranks = []
         r = 0
@@ -126,6 +127,25 @@ def heisenberg():
             if t:
                 ops[t[0]] = [float(x) for x in t[1:]]
         json.dump({"ranks": ranks, "operands": ops}, open(os.path.join(OUT, f"mpi179_{cell.replace('-', '_')}.json"), "w"))
+
+
+def negative_stepdt():
+    """C 1.4.0 direct-SR solutions (stcOptInit + dposv) on fixed operands for
+    positive, negative and shifted DSROptStepDt (data lines, as JSON)."""
+    cases = []
+    for line in open(os.path.join(FIX, "negative_stepdt", "c_sr.txt")):
+        if line.startswith("#") or not line.strip():
+            continue
+        t = line.split()
+        if t[0] == "case":
+            cases.append({"name": t[1], "raw_dt": t[3], "sta_del": t[5]})
+        elif t[0] == "solution":
+            cases[-1]["solution"] = [float(x) for x in t[1:]]
+        elif t[0] in ("srflag", "info"):
+            cases[-1][t[0]] = int(t[1])
+        elif t[0] == "dt":
+            cases[-1]["dt"] = float(t[1])
+    json.dump(cases, open(os.path.join(OUT, "negative_stepdt_c_sr.json"), "w"), indent=0)
 
 
 def sr_cg():
@@ -212,7 +232,8 @@ def bilinear():
                         values=np.concatenate(values), bits=np.array(bits, dtype=np.uint64))
 
 
-ALL = [sfmt, hubbard, heisenberg, sr_cg, pfapack, pfapack_complex, candidate, zvo_rows, bilinear, shift_flags]
+ALL = [sfmt, hubbard, heisenberg, sr_cg, pfapack, pfapack_complex, candidate, zvo_rows, bilinear, shift_flags,
+       negative_stepdt]
 
 if __name__ == "__main__":
     chosen = [f for f in ALL if not ARGS or f.__name__ in ARGS]
