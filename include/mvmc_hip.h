@@ -247,6 +247,9 @@ int mvmc_group_run(mvmc_group* g, int r, mvmc_ens** out);
 /* Sweeps captured as a HIP graph of `rounds` (sweep + rebuild) launches per
  * replay, replayed until no walker waits (0: launches issued directly). */
 int mvmc_group_set_graph(mvmc_group* g, int rounds);
+/* Slot ranges re-cut after every step in proportion to each run's measured
+ * busy time per launch class (on by default; results do not depend on it). */
+int mvmc_group_set_autotune(mvmc_group* g, int on);
 /* One SR step of every run; rows[nruns][6] = each run's zvo_out row. */
 int mvmc_group_opt_step(mvmc_group* g, double* rows);
 /* The sampling phase of one SR step of every run (vmc_make_sample). */

@@ -566,8 +566,8 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
   raise_lds_limit((const void*)k_begin<false>, E->lds_samp);
   raise_lds_limit((const void*)k_measure<false>, E->lds_meas);
   E->lds_rebuild = lds_bytes_rebuild(v);
-  raise_lds_limit((const void*)k_rebuild<3, 8, false>, E->lds_rebuild);
-  raise_lds_limit((const void*)k_rebuild<2, 16, false>, E->lds_rebuild);
+  raise_lds_limit((const void*)k_rebuild<3, 8>, E->lds_rebuild);
+  raise_lds_limit((const void*)k_rebuild<2, 16>, E->lds_rebuild);
   raise_lds_limit((const void*)k_meas_rebuild<3, 8, false>, E->lds_rebuild);
   raise_lds_limit((const void*)k_meas_rebuild<2, 16, false>, E->lds_rebuild);
   int occ_s = 0, occ_m = 0, occ_r = 0;
@@ -579,8 +579,8 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
   }
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_m, k_measure<false>, 256, E->lds_meas));
   int occ_r1 = 0;
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r, k_rebuild<3, 8, false>, 256, E->lds_rebuild));
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r1, k_rebuild<2, 16, false>, 256, E->lds_rebuild));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r, k_rebuild<3, 8>, 256, E->lds_rebuild));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r1, k_rebuild<2, 16>, 256, E->lds_rebuild));
   occ_r1 = std::max(1, occ_r1);
   occ_s = std::max(1, occ_s);
   occ_m = std::max(1, occ_m);
@@ -622,11 +622,9 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
 
 void launch_rebuild(mvmc_ens* E, int var) {
   if (var == 1)
-    hipLaunchKernelGGL((k_rebuild<2, 16, false>), dim3(E->grid_rb[1]), dim3(256), E->lds_rebuild, E->st, E->v,
-                       GroupArgs{});
+    hipLaunchKernelGGL((k_rebuild<2, 16>), dim3(E->grid_rb[1]), dim3(256), E->lds_rebuild, E->st, E->v);
   else
-    hipLaunchKernelGGL((k_rebuild<3, 8, false>), dim3(E->grid_rb[0]), dim3(256), E->lds_rebuild, E->st, E->v,
-                       GroupArgs{});
+    hipLaunchKernelGGL((k_rebuild<3, 8>), dim3(E->grid_rb[0]), dim3(256), E->lds_rebuild, E->st, E->v);
   HIPCHK(hipGetLastError());
 }
 
@@ -1207,8 +1205,11 @@ void finish_opt_step(mvmc_ens* e, double* row) {
 // launch would, so every run's results equal those of the run stepped alone.
 // ---------------------------------------------------------------------------
 struct GroupPart {
-  std::vector<int> start;  // [nruns + 1]
-  int* d = nullptr;        // device copy
+  std::vector<int> start;     // [nruns + 1]
+  int* d = nullptr;           // device copy
+  long long* time = nullptr;  // [capacity] busy ticks per workgroup of the last step
+  int capacity = 0;           // resident workgroups of the launch class
+  std::vector<double> weight; // per-run work estimate the ranges are cut by
 };
 
 }  // namespace
@@ -1220,8 +1221,17 @@ struct mvmc_group {
   EnsView* d_views = nullptr;
   int* d_flags = nullptr;  // [nruns][4]: the runs' walker counters (their v.nflag points here)
   size_t lds_samp = 0, lds_meas = 0, lds_rebuild = 0;
-  GroupPart samp, meas, rb;  // slot ranges of the sampler, measurement and rebuild launches
+  GroupPart samp, meas, rb;  // slot ranges of the placement, measurement and measurement-rebuild launches
+  // shared walker queue of the sweep and periodic-rebuild launches
+  int* d_gq = nullptr;
+  int* d_glist = nullptr;
+  int* d_qstart = nullptr;
+  int* d_qrun = nullptr;
+  double* d_work = nullptr;
+  double* d_pend = nullptr;
+  long long work_stride = 0, pend_stride = 0;
   int graph_rounds = 0;      // k_block + k_rebuild rounds per captured graph (0: direct launches)
+  int autotune = 1;          // re-cut the slot ranges from measured busy times after every step
   // timings of the last group step
   double t_sample = 0, t_measure = 0, t_sr = 0, block_ms = 0, rebuild_ms = 0;
   int launches = 0, graph_replays = 0;
@@ -1239,6 +1249,9 @@ namespace {
 // walkers).
 void group_partition(mvmc_group* G, GroupPart& P, int grid, const std::vector<double>& weight) {
   const int R = (int)G->runs.size();
+  P.capacity = grid;
+  P.weight = weight;
+  if (!P.time) P.time = dalloc<long long>(G->allocs, (size_t)grid);
   double tot = 0;
   for (double x : weight) tot += x;
   std::vector<int> cnt(R);
@@ -1299,7 +1312,7 @@ void group_create_impl(mvmc_group* G, int nruns, const char* const* namelists, c
   HIPCHK(hipMemcpyAsync(G->d_views, views.data(), sizeof(EnsView) * nruns, hipMemcpyHostToDevice, G->st));
   raise_lds_limit((const void*)k_begin<true>, G->lds_samp);
   raise_lds_limit((const void*)k_block<double, true>, G->lds_samp);
-  raise_lds_limit((const void*)k_rebuild<3, 8, true>, G->lds_rebuild);
+  raise_lds_limit((const void*)k_rebuild_q<3, 8>, G->lds_rebuild);
   raise_lds_limit((const void*)k_meas_rebuild<3, 8, true>, G->lds_rebuild);
   raise_lds_limit((const void*)k_measure<true>, G->lds_meas);
   int dev = 0, ncu = 0, occ_s = 0, occ_m = 0, occ_r = 0;
@@ -1307,33 +1320,94 @@ void group_create_impl(mvmc_group* G, int nruns, const char* const* namelists, c
   HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, k_block<double, true>, 256, G->lds_samp));
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_m, k_measure<true>, 256, G->lds_meas));
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r, k_rebuild<3, 8, true>, 256, G->lds_rebuild));
-  // work weights: sampling ~ W Nsite n^2 NQP (accepted moves stream the
-  // inverse), rebuild and measurement ~ W n^3 NQP
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r, k_meas_rebuild<3, 8, true>, 256, G->lds_rebuild));
+  // first work estimates (re-cut from measured busy times after every step):
+  // a proposal costs a fixed serial chain plus row work ~ n NQP, a walker's
+  // step ~ Nsite proposals; a rebuild ~ n^3 NQP on top of a fixed latency
   std::vector<double> ws, wr;
   for (mvmc_ens* E : G->runs) {
     const double W = E->v.W, n = E->v.n, q = E->v.nqp;
-    ws.push_back(W * E->v.ns * n * n * q);
-    wr.push_back(W * n * n * n * q);
+    ws.push_back(W * E->v.ns * (600.0 + n * q));
+    wr.push_back(W * (2.0e5 + n * n * n * q / 16.0));
   }
   group_partition(G, G->samp, std::max(1, occ_s) * ncu, ws);
   group_partition(G, G->meas, std::max(1, occ_m) * ncu, wr);
   group_partition(G, G->rb, std::max(1, occ_r) * ncu, wr);
+  // shared walker queue: runs in descending order of Nsize (longest walkers
+  // first), scratch for the largest run in every workgroup of the sweep and
+  // rebuild launches
+  std::vector<int> order(nruns), qstart(nruns + 1, 0);
+  for (int r = 0; r < nruns; ++r) order[r] = r;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return G->runs[a]->v.n > G->runs[b]->v.n; });
+  for (int k = 0; k < nruns; ++k) qstart[k + 1] = qstart[k] + G->runs[order[k]]->v.W;
+  G->d_qstart = dalloc<int>(G->allocs, nruns + 1);
+  G->d_qrun = dalloc<int>(G->allocs, nruns);
+  G->d_gq = dalloc<int>(G->allocs, 4);
+  G->d_glist = dalloc<int>(G->allocs, std::max(qstart[nruns], 1));
+  HIPCHK(hipMemcpyAsync(G->d_qstart, qstart.data(), sizeof(int) * (nruns + 1), hipMemcpyHostToDevice, G->st));
+  HIPCHK(hipMemcpyAsync(G->d_qrun, order.data(), sizeof(int) * nruns, hipMemcpyHostToDevice, G->st));
+  for (mvmc_ens* E : G->runs) {
+    G->work_stride = std::max<long long>(G->work_stride, (long long)pfinv_work_doubles(E->v.n, E->v.nqp));
+    G->pend_stride = std::max<long long>(G->pend_stride, 2LL * E->v.nqp * E->v.n);
+  }
+  const size_t slots = (size_t)std::max(G->samp.capacity, G->rb.capacity);
+  G->d_work = dalloc<double>(G->allocs, slots * G->work_stride);
+  G->d_pend = dalloc<double>(G->allocs, slots * G->pend_stride);
+  HIPCHK(hipStreamSynchronize(G->st));
 }
 
-GroupArgs group_args(mvmc_group* G, const GroupPart& P) {
-  return GroupArgs{G->d_views, P.d, (int)G->runs.size()};
+GroupArgs group_args(mvmc_group* G, const GroupPart& P, bool timed = true) {
+  GroupArgs a{};
+  a.views = G->d_views;
+  a.slot_start = P.d;
+  a.nruns = (int)G->runs.size();
+  a.slot_time = timed ? P.time : nullptr;
+  a.gq = G->d_gq;
+  a.glist = G->d_glist;
+  a.qstart = G->d_qstart;
+  a.qrun = G->d_qrun;
+  a.work = G->d_work;
+  a.pend = G->d_pend;
+  a.work_stride = G->work_stride;
+  a.pend_stride = G->pend_stride;
+  return a;
+}
+
+// Re-cut a launch class after a step: each run's work is the busy time its
+// workgroups summed (invariant to how many it had), so ranges proportional
+// to it balance the runs' finishing times.  Runs without recorded work keep
+// their previous weight.
+void group_retune(mvmc_group* G, GroupPart& P) {
+  const int R = (int)G->runs.size();
+  std::vector<long long> t((size_t)P.start[R]);
+  HIPCHK(hipMemcpyAsync(t.data(), P.time, sizeof(long long) * t.size(), hipMemcpyDeviceToHost, G->st));
+  HIPCHK(hipStreamSynchronize(G->st));
+  std::vector<double> w(R, 0.0);
+  double tot = 0.0, told = 0.0;
+  for (int r = 0; r < R; ++r) {
+    for (int b = P.start[r]; b < P.start[r + 1]; ++b) w[r] += (double)t[b];
+    tot += w[r];
+    told += P.weight[r];
+  }
+  if (tot <= 0.0) return;
+  for (int r = 0; r < R; ++r)
+    if (w[r] <= 0.0) w[r] = P.weight[r] / told * tot;
+  group_partition(G, P, P.capacity, w);
+}
+
+void group_clear_times(mvmc_group* G) {
+  for (GroupPart* P : {&G->samp, &G->rb, &G->meas})
+    HIPCHK(hipMemsetAsync(P->time, 0, sizeof(long long) * P->capacity, G->st));
 }
 
 // One round: clear the walker counters, run every walker until its step ends
 // or its rebuild is due, rebuild the walkers handed over.
 void group_round(mvmc_group* G, int out, int n_in_max) {
-  const int R = (int)G->runs.size();
-  hipLaunchKernelGGL(k_fill_int, dim3(nblk(4 * R)), dim3(256), 0, G->st, G->d_flags, 4 * R, 0);
-  hipLaunchKernelGGL((k_block<double, true>), dim3(G->samp.start[R]), dim3(256), G->lds_samp, G->st, EnsView{},
-                     group_args(G, G->samp), out, n_in_max);
-  hipLaunchKernelGGL((k_rebuild<3, 8, true>), dim3(G->rb.start[R]), dim3(256), G->lds_rebuild, G->st, EnsView{},
-                     group_args(G, G->rb));
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, G->st, G->d_gq, 4, 0);
+  hipLaunchKernelGGL((k_block<double, true>), dim3(G->samp.capacity), dim3(256), G->lds_samp, G->st, EnsView{},
+                     group_args(G, G->samp, false), out, n_in_max);
+  hipLaunchKernelGGL((k_rebuild_q<3, 8>), dim3(G->rb.capacity), dim3(256), G->lds_rebuild, G->st,
+                     group_args(G, G->rb, false));
 }
 
 void group_sample(mvmc_group* G) {
@@ -1343,6 +1417,7 @@ void group_sample(mvmc_group* G) {
     if (E->precision != 0) throw Fail(MVMC_E_INVALID_SHAPE, "group runs use the FP64 sampler");
     E->upload_tables();
   }
+  group_clear_times(G);
   hipLaunchKernelGGL(k_begin<true>, dim3(G->samp.start[R]), dim3(256), G->lds_samp, G->st, EnsView{},
                      group_args(G, G->samp));
   HIPCHK(hipGetLastError());
@@ -1360,7 +1435,6 @@ void group_sample(mvmc_group* G) {
   G->launches = 0;
   G->graph_replays = 0;
   double ms_total = 0;
-  std::vector<int> flags(4 * (size_t)R);
   for (int out = 0; out < n_out; ++out) {
     for (mvmc_ens* E : G->runs)
       hipLaunchKernelGGL(k_reset_resume, dim3(nblk(E->v.W)), dim3(256), 0, G->st, E->v);
@@ -1386,11 +1460,10 @@ void group_sample(mvmc_group* G) {
         ++G->launches;
       }
       HIPCHK(hipGetLastError());
-      HIPCHK(hipMemcpyAsync(flags.data(), G->d_flags, sizeof(int) * flags.size(), hipMemcpyDeviceToHost, G->st));
+      int handed = 0;
+      HIPCHK(hipMemcpyAsync(&handed, G->d_gq, sizeof(int), hipMemcpyDeviceToHost, G->st));
       HIPCHK(hipStreamSynchronize(G->st));
-      bool pending = false;
-      for (int r = 0; r < R; ++r) pending = pending || flags[4 * r] != 0;
-      if (!pending) break;
+      if (handed == 0) break;
     }
     HIPCHK(hipEventRecord(e1, G->st));
     HIPCHK(hipEventSynchronize(e1));
@@ -1960,6 +2033,12 @@ int mvmc_group_run(mvmc_group* g, int r, mvmc_ens** out) {
   return MVMC_OK;
 }
 
+int mvmc_group_set_autotune(mvmc_group* g, int on) {
+  if (!g) return MVMC_E_INVALID_SHAPE;
+  g->autotune = on ? 1 : 0;
+  return MVMC_OK;
+}
+
 int mvmc_group_set_graph(mvmc_group* g, int rounds) {
   if (!g || rounds < 0) return MVMC_E_INVALID_SHAPE;
   g->graph_rounds = rounds;
@@ -1973,6 +2052,11 @@ int mvmc_group_opt_step(mvmc_group* g, double* rows) {
     const double t0 = now_s();
     for (size_t r = 0; r < g->runs.size(); ++r) finish_opt_step(g->runs[r], rows ? rows + 6 * r : nullptr);
     g->t_sr = now_s() - t0;
+    if (g->autotune) {
+      group_retune(g, g->samp);
+      group_retune(g, g->rb);
+      group_retune(g, g->meas);
+    }
   });
 }
 

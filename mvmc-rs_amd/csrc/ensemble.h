@@ -76,6 +76,18 @@ struct GroupArgs {
   const EnsView* views;   // [nruns] device copies of the runs' views
   const int* slot_start;  // [nruns + 1]
   int nruns;              // 0: single-ensemble launch (the view argument)
+  long long* slot_time;   // [grid] busy clock ticks per workgroup (accumulated), or null
+  // Shared walker queue of the sweep and rebuild launches: queue entry g lies
+  // in [qstart[k], qstart[k+1]) for run qrun[k], walker g - qstart[k]; any
+  // workgroup takes any run's walker, with scratch slot blockIdx.x of the
+  // group's own buffers (strides for the largest run).
+  int* gq;                // [4]: 0 entries handed to the rebuild, 1 next entry, 2 next hand-over taken
+  int* glist;             // [total] queue entries handed to the rebuild
+  const int* qstart;      // [nruns + 1]
+  const int* qrun;        // [nruns]
+  double* work;           // [grid][work_stride]
+  double* pend;           // [grid][pend_stride]
+  long long work_stride, pend_stride;
 };
 
 }  // namespace mvmc_hip

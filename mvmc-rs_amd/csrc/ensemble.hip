@@ -48,10 +48,30 @@ __device__ inline int group_run(const GroupArgs& g) {
   return r;
 }
 
+// Run and walker of queue entry g of a group's shared walker queue.
+__device__ inline int queue_run(const GroupArgs& g, int e, int& w) {
+  int k = 0;
+  while (k + 1 < g.nruns && e >= g.qstart[k + 1]) ++k;
+  w = e - g.qstart[k];
+  return g.qrun[k];
+}
+
 // Kernel prologue of the walker kernels: `v` is the ensemble this workgroup
 // serves, `slot` its scratch slot and `nslots` the workgroups serving that
 // ensemble (the launch's own grid outside a group).
+// Adds the workgroup's lifetime (constant-clock ticks) to *out when it ends.
+struct SlotTimer {
+  long long* out;
+  long long t0;
+  __device__ explicit SlotTimer(long long* o) : out(o), t0(o ? wall_clock64() : 0) {}
+  __device__ ~SlotTimer() {
+    if (out) *out += wall_clock64() - t0;
+  }
+};
+
 #define ENS_SLOT(VARG, GARG)                                                                               \
+  SlotTimer slot_timer_(GROUP && threadIdx.x == 0 && (GARG).slot_time ? (GARG).slot_time + blockIdx.x     \
+                                                                      : nullptr);                        \
   const int grp_run_ = GROUP ? group_run(GARG) : 0;                                                      \
   const EnsView v = GROUP ? (GARG).views[grp_run_] : (VARG);                                             \
   const int slot = GROUP ? (int)blockIdx.x - (GARG).slot_start[grp_run_] : (int)blockIdx.x;             \
@@ -774,19 +794,30 @@ enum { M_TYPE = 0, M_REJ, M_MI, M_S, M_RI, M_RJ, M_MJ, M_T, M_ACC, M_NACC, M_NOU
 
 // Elem: storage type of the sampler's inverse (double; float in the
 // mixed-precision form, whose arithmetic stays FP64).
+// In a group launch (GROUP) the workgroups take entries of the group's
+// shared walker queue, so every run's walkers are spread over the whole grid
+// as they come; the view, LDS carve and tables follow the entry's run.
 template <class Elem, bool GROUP>
 __global__ void __launch_bounds__(256, 2) k_block(EnsView v_arg, GroupArgs ga, int out_step, int n_in_arg) {
-  ENS_SLOT(v_arg, ga)
+  SlotTimer slot_timer_(GROUP && threadIdx.x == 0 && ga.slot_time ? ga.slot_time + blockIdx.x : nullptr);
+  const int slot = blockIdx.x;
+  int cur_run = -1;
+  EnsView v = v_arg;
+  if (GROUP) {
+    int w0 = 0;
+    cur_run = queue_run(ga, 0, w0);
+    v = ga.views[cur_run];
+  }
   // proposals per outer step (NVMCInterval x Nsite): each run's own in a group
-  const int n_in = GROUP ? v.interval * (v.ns > 0 ? v.ns : 1) : n_in_arg;
+  int n_in = GROUP ? v.interval * (v.ns > 0 ? v.ns : 1) : n_in_arg;
   Elem* inv = sizeof(Elem) == sizeof(float) ? (Elem*)v.inv32 : (Elem*)v.inv;
   extern __shared__ __align__(16) char lds[];
   SampLds L = carve_sampler(v, lds);
   load_tables(v, L);
   const int tid = threadIdx.x, nt = blockDim.x;
-  const int n = v.n, ne = v.ne, ns = v.ns, n2 = v.n2, nqp = v.nqp, nn = n * n;
-  double* work = v.work + (size_t)slot * pfinv_work_doubles(n, nqp);
-  double* pv1 = v.pend + (size_t)slot * 2 * nqp * n;  // pending hop update: vec1
+  int n = v.n, ne = v.ne, ns = v.ns, n2 = v.n2, nqp = v.nqp, nn = n * n;
+  double* work = GROUP ? ga.work + (size_t)slot * ga.work_stride : v.work + (size_t)slot * pfinv_work_doubles(n, nqp);
+  double* pv1 = (GROUP ? ga.pend + (size_t)slot * ga.pend_stride : v.pend + (size_t)slot * 2 * nqp * n);  // pending hop update: vec1
   double* pv2 = pv1 + (size_t)nqp * n;                      //                     vec2
   PHASE_INIT
   __shared__ int next_w;
@@ -794,11 +825,28 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v_arg, GroupArgs ga, i
     // walkers are taken dynamically, so the launch ends when the work does
     // (a walker's results do not depend on the slot that runs it: the
     // slot's scratch is rewritten before it is read)
-    if (tid == 0) next_w = atomicAdd(v.nflag + 1, 1);
+    if (tid == 0) next_w = atomicAdd(GROUP ? ga.gq + 1 : v.nflag + 1, 1);
     __syncthreads();
-    const int w = next_w;
+    const int qe = next_w;
     __syncthreads();
-    if (w >= v.W) break;
+    int w = qe;
+    if (GROUP) {
+      if (qe >= ga.qstart[ga.nruns]) break;
+      const int r = queue_run(ga, qe, w);
+      if (r != cur_run) {
+        cur_run = r;
+        v = ga.views[r];
+        n_in = v.interval * (v.ns > 0 ? v.ns : 1);
+        inv = sizeof(Elem) == sizeof(float) ? (Elem*)v.inv32 : (Elem*)v.inv;
+        n = v.n; ne = v.ne; ns = v.ns; n2 = v.n2; nqp = v.nqp; nn = n * n;
+        pv2 = pv1 + (size_t)nqp * n;
+        L = carve_sampler(v, lds);
+        load_tables(v, L);
+        __syncthreads();
+      }
+    } else if (w >= v.W) {
+      break;
+    }
     int* ws = v.wstate + (size_t)w * 8;
     int pm = -1;  // row of the pending update (-1: none); uniform over the workgroup
     if (tid == 0) {
@@ -1185,7 +1233,8 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v_arg, GroupArgs ga, i
         ws[1] = L.misc[M_NACC];
         ws[4] = stop_at;
         ws[5] = 1;
-        v.rlist[atomicAdd(v.nflag, 1)] = w;
+        if (GROUP) ga.glist[atomicAdd(ga.gq, 1)] = qe;
+        else v.rlist[atomicAdd(v.nflag, 1)] = w;
       }
       __syncthreads();
       PHASE(6);
@@ -1236,10 +1285,33 @@ __host__ __device__ inline size_t lds_bytes_rebuild(const EnsView& v) {
 // Two compiled variants, bit-identical in results (only the schedule of the
 // elimination differs), picked at run time by measurement: MINB workgroups
 // per CU, JB columns per batch of the panel close.
-template <int MINB, int JB, bool GROUP>
-__global__ void __launch_bounds__(256, MINB) k_rebuild(EnsView v_arg, GroupArgs ga) {
-  ENS_SLOT(v_arg, ga)
-  (void)nslots;
+// One walker's periodic rebuild (engine.rs:454-473): calc_m_all from its
+// configuration; on success the Pfaffians and log|<x|psi>| are replaced; the
+// accept window restarts either way.
+template <int JB>
+__device__ __forceinline__ void rebuild_walker(const EnsView& v, int w, int* ei, int* rs, double* pfn, char* pfscr,
+                                               double* work, long long* clk) {
+  const int tid = threadIdx.x;
+  int* ws = v.wstate + (size_t)w * 8;
+  for (int i = tid; i < v.n; i += blockDim.x) ei[i] = v.ele_idx[(size_t)w * v.n + i];
+  __syncthreads();
+  double* planes = v.inv + (size_t)w * v.nqp * v.n * v.n;
+  const int bad = wg_calc_m_all<JB>(v, ei, rs, planes, work, pfn, pfscr, false,
+                                clk);
+  copy_inv32(v, w);
+  if (tid == 0) {
+    if (!bad) {
+      for (int q = 0; q < v.nqp; ++q) v.pf[(size_t)w * v.nqp + q] = pfn[q];
+      v.log_ip[w] = ddm::cr_log(fabs(ip_of(v, pfn)));
+    }
+    ws[1] = 0;
+    ws[5] = 0;
+  }
+  __syncthreads();
+}
+
+template <int MINB, int JB>
+__global__ void __launch_bounds__(256, MINB) k_rebuild(EnsView v) {
   extern __shared__ __align__(16) char lds[];
   Carver c{lds};
   int* ei = c.take<int>(v.n);
@@ -1247,7 +1319,7 @@ __global__ void __launch_bounds__(256, MINB) k_rebuild(EnsView v_arg, GroupArgs 
   double* pfn = c.take<double>(v.nqp);
   char* pfscr = (char*)c.take<char>(pf_bytes(v.n, v.nqp));
   const int tid = threadIdx.x;
-  double* work = v.work + (size_t)slot * pfinv_work_doubles(v.n, v.nqp);
+  double* work = v.work + (size_t)blockIdx.x * pfinv_work_doubles(v.n, v.nqp);
   const int nhand = *v.nflag;  // walkers handed over by the k_block launch before
   __shared__ int next_i;
   for (;;) {
@@ -1259,22 +1331,34 @@ __global__ void __launch_bounds__(256, MINB) k_rebuild(EnsView v_arg, GroupArgs 
     __syncthreads();
     if (li >= nhand) break;
     const int w = v.rlist[li];
-    int* ws = v.wstate + (size_t)w * 8;
-    for (int i = tid; i < v.n; i += blockDim.x) ei[i] = v.ele_idx[(size_t)w * v.n + i];
+    rebuild_walker<JB>(v, w, ei, rs, pfn, pfscr, work, v.clk ? v.clk + (size_t)blockIdx.x * 32 + 16 : nullptr);
+  }
+}
+
+// The same over a group's shared hand-over list: entries of any run, the
+// LDS carve following the entry's run, scratch slot blockIdx.x of the
+// group's buffers.
+template <int MINB, int JB>
+__global__ void __launch_bounds__(256, MINB) k_rebuild_q(GroupArgs ga) {
+  extern __shared__ __align__(16) char lds[];
+  const int tid = threadIdx.x;
+  double* work = ga.work + (size_t)blockIdx.x * ga.work_stride;
+  const int nhand = *ga.gq;  // entries handed over by the k_block launch before
+  __shared__ int next_i;
+  for (;;) {
+    if (tid == 0) next_i = atomicAdd(ga.gq + 2, 1);
     __syncthreads();
-    double* planes = v.inv + (size_t)w * v.nqp * v.n * v.n;
-    const int bad = wg_calc_m_all<JB>(v, ei, rs, planes, work, pfn, pfscr, false,
-                                  v.clk ? v.clk + (size_t)slot * 32 + 16 : nullptr);
-    copy_inv32(v, w);
-    if (tid == 0) {
-      if (!bad) {
-        for (int q = 0; q < v.nqp; ++q) v.pf[(size_t)w * v.nqp + q] = pfn[q];
-        v.log_ip[w] = ddm::cr_log(fabs(ip_of(v, pfn)));
-      }
-      ws[1] = 0;
-      ws[5] = 0;
-    }
+    const int li = next_i;
     __syncthreads();
+    if (li >= nhand) break;
+    int w = 0;
+    const EnsView& v = ga.views[queue_run(ga, ga.glist[li], w)];
+    Carver c{lds};
+    int* ei = c.take<int>(v.n);
+    int* rs = c.take<int>(v.n);
+    double* pfn = c.take<double>(v.nqp);
+    char* pfscr = (char*)c.take<char>(pf_bytes(v.n, v.nqp));
+    rebuild_walker<JB>(v, w, ei, rs, pfn, pfscr, work, nullptr);
   }
 }
 

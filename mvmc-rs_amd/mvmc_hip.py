@@ -72,6 +72,7 @@ def lib():
             getattr(L, name).argtypes = [ctypes.c_void_p]
         L.mvmc_group_run.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
         L.mvmc_group_set_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.mvmc_group_set_autotune.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.mvmc_group_opt_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.mvmc_group_timings.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.mvmc_ens_launch_log.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
@@ -87,7 +88,7 @@ def lib():
 EXPORTED_SYMBOLS = [
     "mvmc_last_error", "mvmc_abi_version", "mvmc_ens_launch_log", "mvmc_pfinv_f64_noalloc",
     "mvmc_sr_assemble_s_g", "mvmc_sr_gram_complex", "mvmc_sr_s_g", "mvmc_group_create", "mvmc_group_destroy",
-    "mvmc_group_run", "mvmc_group_set_graph", "mvmc_group_opt_step", "mvmc_group_sample", "mvmc_group_timings", "mvmc_pfinv_f64", "mvmc_pfinv_f64_inplace", "mvmc_ens_create",
+    "mvmc_group_run", "mvmc_group_set_graph", "mvmc_group_set_autotune", "mvmc_group_opt_step", "mvmc_group_sample", "mvmc_group_timings", "mvmc_pfinv_f64", "mvmc_pfinv_f64_inplace", "mvmc_ens_create",
     "mvmc_ens_destroy", "mvmc_ens_dims", "mvmc_ens_get_params", "mvmc_ens_set_params", "mvmc_ens_sample",
     "mvmc_ens_measure", "mvmc_ens_sr_cg", "mvmc_ens_opt_step", "mvmc_ens_get_energy", "mvmc_ens_get_operands",
     "mvmc_ens_get_o_store", "mvmc_ens_get_walker", "mvmc_ens_get_tables", "mvmc_ens_get_config", "mvmc_ens_get_cg",
@@ -352,6 +353,9 @@ class EnsembleGroup:
 
     def set_graph(self, rounds):
         _check(lib().mvmc_group_set_graph(self.h, int(rounds)))
+
+    def set_autotune(self, on):
+        _check(lib().mvmc_group_set_autotune(self.h, 1 if on else 0))
 
     def opt_step(self):
         rows = np.zeros((len(self.runs), 6))
