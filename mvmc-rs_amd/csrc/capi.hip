@@ -282,6 +282,57 @@ double* noalloc_scratch() {
   return (double*)p;
 }
 
+// The register-resident Pfaffian + inverse (64 < n <= RR_NMAX) is on unless
+// MVMC_PFINV_RR=0 selects the panel-delayed kernels (same results).
+bool rr_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("MVMC_PFINV_RR");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+// Register blocks per thread column of the register-resident kernels.
+int rr_r(int n) { return (n + RR_T - 1) / RR_T; }
+
+template <int R>
+void rr_rebuild_limits_r(size_t lds) {
+  raise_lds_limit((const void*)k_rebuild_rr<R>, lds);
+  raise_lds_limit((const void*)k_meas_rebuild_rr<R>, lds);
+}
+
+void rr_rebuild_limits(int r, size_t lds) {
+  switch (r) {
+    case 3: rr_rebuild_limits_r<3>(lds); break;
+    case 4: rr_rebuild_limits_r<4>(lds); break;
+    case 5: rr_rebuild_limits_r<5>(lds); break;
+    case 6: rr_rebuild_limits_r<6>(lds); break;
+    case 7: rr_rebuild_limits_r<7>(lds); break;
+    default: throw Fail(MVMC_E_INVALID_SHAPE, "register-resident path covers 64 < n <= 224");
+  }
+}
+
+// Periodic rebuild (s < 0) or measurement rebuild of saved sample s with the
+// register-resident kernels, one 1024-thread workgroup per slot.
+template <int R>
+void rr_launch_rebuild_r(int grid, size_t lds, hipStream_t st, const EnsView& v, int s) {
+  if (s < 0)
+    hipLaunchKernelGGL(k_rebuild_rr<R>, dim3(grid), dim3(RR_T * RR_T), lds, st, v);
+  else
+    hipLaunchKernelGGL(k_meas_rebuild_rr<R>, dim3(grid), dim3(RR_T * RR_T), lds, st, v, s);
+}
+
+void rr_launch_rebuild(int r, int grid, size_t lds, hipStream_t st, const EnsView& v, int s) {
+  switch (r) {
+    case 3: rr_launch_rebuild_r<3>(grid, lds, st, v, s); break;
+    case 4: rr_launch_rebuild_r<4>(grid, lds, st, v, s); break;
+    case 5: rr_launch_rebuild_r<5>(grid, lds, st, v, s); break;
+    case 6: rr_launch_rebuild_r<6>(grid, lds, st, v, s); break;
+    case 7: rr_launch_rebuild_r<7>(grid, lds, st, v, s); break;
+    default: throw Fail(MVMC_E_INVALID_SHAPE, "register-resident path covers 64 < n <= 224");
+  }
+}
+
 template <class T>
 T* dalloc(std::vector<void*>& list, size_t count) {
   void* p = nullptr;
@@ -320,13 +371,15 @@ struct mvmc_ens {
   hipStream_t st = nullptr;
   int grid_samp = 0, grid_meas = 0, grid_rebuild = 0, slots = 0;
   // rebuild-kernel variants (0: 3 workgroups per CU, 8-column close batches;
-  // 1: 2 per CU, 16 columns), tuned on the first well-filled launches:
-  // rb_variant -1 while measuring, then the faster one
-  int grid_rb[2] = {0, 0};
+  // 1: 2 per CU, 16 columns; 2: register-resident, one 1024-thread workgroup
+  // per CU, for 64 < n <= RR_NMAX), tuned on the first well-filled launches:
+  // rb_variant -1 while measuring, then the fastest one
+  int grid_rb[3] = {0, 0, 0};
+  int rb_nvar = 2;
   int rb_variant = -1;
-  double rb_ms[2] = {0, 0}, rb_walkers[2] = {0, 0};
-  int rb_samples[2] = {0, 0};
-  size_t lds_rebuild = 0;
+  double rb_ms[3] = {0, 0, 0}, rb_walkers[3] = {0, 0, 0};
+  int rb_samples[3] = {0, 0, 0};
+  size_t lds_rebuild = 0, lds_rr = 0;
   double rebuild_ms = 0, rebuild_launches = 0;
   size_t lds_samp = 0, lds_meas = 0;
   double* d_slater = nullptr;
@@ -590,8 +643,25 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
   E->grid_rebuild = (int)std::min<long long>(walkers, (long long)occ_r * ncu);
   E->grid_rb[0] = E->grid_rebuild;
   E->grid_rb[1] = (int)std::min<long long>(walkers, (long long)occ_r1 * ncu);
-  E->slots = std::max(std::max(E->grid_samp, E->grid_meas), std::max(E->grid_rb[0], E->grid_rb[1]));
-  v.work = dalloc<double>(A, (size_t)E->slots * pfinv_work_doubles(v.n, v.nqp));
+  size_t work_per_slot = pfinv_work_doubles(v.n, v.nqp);
+  if (rr_enabled() && v.n > 64 && v.n <= RR_NMAX) {
+    // ei, rs, pfn, then the register-resident carve; 2 n^2 doubles of work
+    E->lds_rr = 2 * rnd16((size_t)v.n * 4) + rnd16((size_t)v.nqp * 8) + rr_lds_bytes(v.n);
+    if (E->lds_rr <= (size_t)maxlds) {
+      rr_rebuild_limits(rr_r(v.n), E->lds_rr);
+      E->rb_nvar = 3;
+      E->grid_rb[2] = (int)std::min<long long>(walkers, ncu);
+      work_per_slot = std::max(work_per_slot, (size_t)2 * v.n * v.n);
+    }
+  }
+  // MVMC_RB_VARIANT=k fixes the rebuild variant instead of tuning it
+  if (const char* f = std::getenv("MVMC_RB_VARIANT")) {
+    const int k = std::atoi(f);
+    if (k >= 0 && k < E->rb_nvar) E->rb_variant = k;
+  }
+  E->slots = std::max(std::max(E->grid_samp, E->grid_meas),
+                      std::max(std::max(E->grid_rb[0], E->grid_rb[1]), E->grid_rb[2]));
+  v.work = dalloc<double>(A, (size_t)E->slots * work_per_slot);
   v.qorb = dalloc<double>(A, (size_t)E->slots * v.nqp * std::max(v.nslater, 1));
   v.pend = dalloc<double>(A, (size_t)E->slots * 2 * v.nqp * v.n);
   v.nflag = dalloc<int>(A, 4);
@@ -621,7 +691,9 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
 }
 
 void launch_rebuild(mvmc_ens* E, int var) {
-  if (var == 1)
+  if (var == 2)
+    rr_launch_rebuild(rr_r(E->v.n), E->grid_rb[2], E->lds_rr, E->st, E->v, -1);
+  else if (var == 1)
     hipLaunchKernelGGL((k_rebuild<2, 16>), dim3(E->grid_rb[1]), dim3(256), E->lds_rebuild, E->st, E->v);
   else
     hipLaunchKernelGGL((k_rebuild<3, 8>), dim3(E->grid_rb[0]), dim3(256), E->lds_rebuild, E->st, E->v);
@@ -629,7 +701,9 @@ void launch_rebuild(mvmc_ens* E, int var) {
 }
 
 void launch_meas_rebuild(mvmc_ens* E, int s) {
-  if (E->rb_variant == 1)
+  if (E->rb_variant == 2)
+    rr_launch_rebuild(rr_r(E->v.n), E->grid_rb[2], E->lds_rr, E->st, E->v, s);
+  else if (E->rb_variant == 1)
     hipLaunchKernelGGL((k_meas_rebuild<2, 16, false>), dim3(E->grid_rb[1]), dim3(256), E->lds_rebuild, E->st, E->v,
                        GroupArgs{}, s);
   else
@@ -703,8 +777,14 @@ void sample_impl(mvmc_ens* E) {
       rebuilt += nf;
       // variant: the tuned one, else alternate on launches that fill every
       // slot twice over (smaller ones run variant 0 and are not counted)
-      const bool tune = E->rb_variant < 0 && nf >= 2 * std::max(E->grid_rb[0], E->grid_rb[1]);
-      const int var = E->rb_variant >= 0 ? E->rb_variant : (tune ? (E->rb_samples[0] <= E->rb_samples[1] ? 0 : 1) : 0);
+      const int nvar = E->rb_nvar;
+      int gmax = 0, least = 0;
+      for (int k = 0; k < nvar; ++k) {
+        gmax = std::max(gmax, E->grid_rb[k]);
+        if (E->rb_samples[k] < E->rb_samples[least]) least = k;
+      }
+      const bool tune = E->rb_variant < 0 && nf >= 2 * gmax;
+      const int var = E->rb_variant >= 0 ? E->rb_variant : (tune ? least : 0);
       HIPCHK(hipEventRecord(r0, E->st));
       launch_rebuild(E, var);
       HIPCHK(hipEventRecord(r1, E->st));
@@ -715,8 +795,15 @@ void sample_impl(mvmc_ens* E) {
       if (tune) {
         E->rb_ms[var] += ms;
         E->rb_walkers[var] += nf;
-        if (++E->rb_samples[var] >= 2 && E->rb_samples[1 - var] >= 2)
-          E->rb_variant = E->rb_ms[0] / E->rb_walkers[0] <= E->rb_ms[1] / E->rb_walkers[1] ? 0 : 1;
+        ++E->rb_samples[var];
+        bool done = true;
+        for (int k = 0; k < nvar; ++k) done &= E->rb_samples[k] >= 2;
+        if (done) {
+          int best = 0;
+          for (int k = 1; k < nvar; ++k)
+            if (E->rb_ms[k] / E->rb_walkers[k] < E->rb_ms[best] / E->rb_walkers[best]) best = k;
+          E->rb_variant = best;
+        }
       }
     }
   }
@@ -1212,6 +1299,33 @@ struct GroupPart {
   std::vector<double> weight; // per-run work estimate the ranges are cut by
 };
 
+template <int R>
+void launch_pfinv_rr_r(const double* in, double* out, int n, int count, double* gs, int grid, double* pf, int* status,
+                       hipStream_t st) {
+  const size_t lds = rr_lds_bytes(n);
+  raise_lds_limit((const void*)k_pfinv_rr<R>, lds);
+  hipLaunchKernelGGL(k_pfinv_rr<R>, dim3(grid), dim3(RR_T * RR_T), lds, st, in, out, n, count, gs, pf, status);
+}
+
+void launch_pfinv_rr(const double* in, double* out, int n, int count, double* pf, int* status, hipStream_t st) {
+  int dev = 0, ncu = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const int grid = std::min(count, std::max(ncu, 1));
+  double* gs = nullptr;
+  HIPCHK(hipMallocAsync((void**)&gs, sizeof(double) * 2 * (size_t)n * n * grid, st));
+  switch ((n + RR_T - 1) / RR_T) {
+    case 3: launch_pfinv_rr_r<3>(in, out, n, count, gs, grid, pf, status, st); break;
+    case 4: launch_pfinv_rr_r<4>(in, out, n, count, gs, grid, pf, status, st); break;
+    case 5: launch_pfinv_rr_r<5>(in, out, n, count, gs, grid, pf, status, st); break;
+    case 6: launch_pfinv_rr_r<6>(in, out, n, count, gs, grid, pf, status, st); break;
+    case 7: launch_pfinv_rr_r<7>(in, out, n, count, gs, grid, pf, status, st); break;
+    default: throw Fail(MVMC_E_INVALID_SHAPE, "register-resident path covers 64 < n <= 224");
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipFreeAsync(gs, st));
+}
+
 }  // namespace
 
 struct mvmc_group {
@@ -1509,6 +1623,10 @@ int mvmc_pfinv_f64(const double* planes, int n, int count, double* pf, double* i
     if (count == 0) return;
     if (!planes || !pf || !inv || !status) throw Fail(MVMC_E_INVALID_SHAPE, "null buffer");
     hipStream_t st = (hipStream_t)stream;
+    if (rr_enabled() && n > 64 && n <= RR_NMAX) {
+      launch_pfinv_rr(planes, inv, n, count, pf, status, st);
+      return;
+    }
     if ((const double*)inv != planes)
       HIPCHK(hipMemcpyAsync(inv, planes, sizeof(double) * n * n * count, hipMemcpyDeviceToDevice, st));
     const int np = n <= 32 ? 8 : 4;  // the inverse runs one wave per plane: 4 planes fill a workgroup
@@ -1857,7 +1975,7 @@ int mvmc_ens_timings(mvmc_ens* e, double* t) {
   t[16] = e->rb_variant;
   t[17] = e->rb_walkers[0] > 0 ? e->rb_ms[0] / e->rb_walkers[0] : 0.0;
   t[18] = e->rb_walkers[1] > 0 ? e->rb_ms[1] / e->rb_walkers[1] : 0.0;
-  t[19] = 0.0;
+  t[19] = e->rb_walkers[2] > 0 ? e->rb_ms[2] / e->rb_walkers[2] : 0.0;
   return MVMC_OK;
 }
 

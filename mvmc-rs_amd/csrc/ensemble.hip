@@ -40,6 +40,9 @@ struct Carver {
 };
 
 __host__ __device__ inline size_t rnd16(size_t b) { return (b + 15) & ~(size_t)15; }
+}  // namespace mvmc_hip
+#include "pfinv_rr.h"
+namespace mvmc_hip {
 
 // Run of this workgroup in a group launch (slot_start is small and uniform).
 __device__ inline int group_run(const GroupArgs& g) {
@@ -2093,6 +2096,148 @@ __global__ void k_cg_dir(int na, const double* sc, const int* ctl, const double*
 // Batched Pfaffian + inverse (BatchedEngine): planes [count][n][n]
 // column-major; one workgroup per plane batch of np planes.
 // ---------------------------------------------------------------------------
+// Batched Pfaffian + inverse of 64 < n <= RR_NMAX planes, one plane per
+// 1024-thread workgroup at a time (planes blockIdx.x, + gridDim.x, ...),
+// register-resident (pfinv_rr.h); gs: 2 n^2 doubles of scratch per
+// workgroup.  Results and statuses as k_pfinv_batched; in == out works in
+// place (the plane is read whole before it is written).
+template <int R>
+__global__ void __launch_bounds__(1024, 1) k_pfinv_rr(const double* in, double* out, int n, int count, double* gs_base,
+                                                   double* pf, int* status) {
+  extern __shared__ __align__(16) char lds[];
+  Carver c{lds};
+  RrLds L = carve_rr(c, n);
+  double* gs = gs_base + (size_t)blockIdx.x * 2 * n * n;
+  for (int p = blockIdx.x; p < count; p += gridDim.x) {
+    const double* A = in + (size_t)p * n * n;
+    double* O = out + (size_t)p * n * n;
+    double pfv = 0.0;
+    int info = 0;
+    bool az = false;
+    rr_pfinv<R>([&](int i, int j) { return A[i + (size_t)j * n]; }, n, O, 1.0, gs, L, pfv, info, false, az);
+    if (threadIdx.x == 0) {
+      status[p] = info;
+      pf[p] = info > 0 ? 0.0 : pfv;
+    }
+    if (info != 0)
+      for (int e = threadIdx.x; e < n * n; e += blockDim.x) O[e] = 0.0;
+    __syncthreads();
+  }
+}
+
+// Periodic rebuild (engine.rs:454-473) of the handed-over walkers with the
+// register-resident Pfaffian + inverse, one walker per 1024-thread
+// workgroup, its NQP planes in turn, gathered straight from the Slater table
+// (A(r, c) = -S[rs_c][rs_r], pfaffian.rs:1107-1188).  A plane that is all
+// zero or fails stops the walker's rebuild (its Pfaffians are then not
+// replaced; the accept window restarts either way).
+template <int R>
+__device__ __forceinline__ int rr_calc_m_all(const EnsView& v, const int* ei, int* rs, double* planes, double* gs,
+                                             RrLds& L, double* pfn) {
+  const int tid = threadIdx.x;
+  __shared__ int bad_site;
+  if (tid == 0) bad_site = 0;
+  __syncthreads();
+  for (int i = tid; i < v.n; i += blockDim.x) {
+    const int r = ei[i];
+    rs[i] = r + (i / v.ne) * v.ns;
+    if (r < 0 || r >= v.ns) bad_site = 1;
+  }
+  __syncthreads();
+  if (bad_site) {
+    if (tid == 0) set_status(v.status, ST_BAD_INPUT);
+    __syncthreads();
+    return 1;
+  }
+  const int n = v.n, n2 = v.n2;
+  for (int q = 0; q < v.nqp; ++q) {
+    const double* S = v.slater + (size_t)q * n2 * n2;
+    double pfv = 0.0;
+    int info = 0;
+    bool az = false;
+    rr_pfinv<R>([&](int i, int j) { return -S[(size_t)rs[j] * n2 + rs[i]]; }, n, planes + (size_t)q * n * n, -1.0,
+                gs, L, pfv, info, true, az);
+    if (az || info != 0) return 1;
+    if (tid == 0) pfn[q] = pfv;
+  }
+  __syncthreads();
+  return 0;
+}
+
+template <int R>
+__global__ void __launch_bounds__(1024, 1) k_rebuild_rr(EnsView v) {
+  extern __shared__ __align__(16) char lds[];
+  Carver c{lds};
+  int* ei = c.take<int>(v.n);
+  int* rs = c.take<int>(v.n);
+  double* pfn = c.take<double>(v.nqp);
+  RrLds L = carve_rr(c, v.n);
+  const int tid = threadIdx.x;
+  double* gs = v.work + (size_t)blockIdx.x * 2 * v.n * v.n;
+  const int nhand = *v.nflag;
+  __shared__ int next_i;
+  for (;;) {
+    if (tid == 0) next_i = atomicAdd(v.nflag + 2, 1);
+    __syncthreads();
+    const int li = next_i;
+    __syncthreads();
+    if (li >= nhand) break;
+    const int w = v.rlist[li];
+    int* ws = v.wstate + (size_t)w * 8;
+    for (int i = tid; i < v.n; i += blockDim.x) ei[i] = v.ele_idx[(size_t)w * v.n + i];
+    __syncthreads();
+    double* planes = v.inv + (size_t)w * v.nqp * v.n * v.n;
+    const int bad = rr_calc_m_all<R>(v, ei, rs, planes, gs, L, pfn);
+    copy_inv32(v, w);
+    if (tid == 0) {
+      if (!bad) {
+        for (int q = 0; q < v.nqp; ++q) v.pf[(size_t)w * v.nqp + q] = pfn[q];
+        v.log_ip[w] = ddm::cr_log(fabs(ip_of(v, pfn)));
+      }
+      ws[1] = 0;
+      ws[5] = 0;
+    }
+    __syncthreads();
+  }
+}
+
+// Measurement rebuild of saved sample s (as k_meas_rebuild) with the
+// register-resident Pfaffian + inverse, walkers blockIdx.x, + gridDim.x, ...
+template <int R>
+__global__ void __launch_bounds__(1024, 1) k_meas_rebuild_rr(EnsView v, int s) {
+  extern __shared__ __align__(16) char lds[];
+  Carver c{lds};
+  int* ei = c.take<int>(v.n);
+  int* rs = c.take<int>(v.n);
+  double* pfn = c.take<double>(v.nqp);
+  RrLds L = carve_rr(c, v.n);
+  const int tid = threadIdx.x;
+  double* gs = v.work + (size_t)blockIdx.x * 2 * v.n * v.n;
+  __shared__ int flag;
+  for (int w = blockIdx.x; w < v.W; w += gridDim.x) {
+    const size_t ws = (size_t)w * v.nsample + s;
+    for (int i = tid; i < v.n; i += blockDim.x) ei[i] = v.s_idx[ws * v.n + i];
+    __syncthreads();
+    if (tid == 0) {
+      int all0 = 1, allneg = 1;
+      for (int k = 0; k < v.n; ++k) { all0 &= (ei[k] == 0); allneg &= (ei[k] < 0); }
+      flag = (all0 || allneg) ? 0 : 1;
+    }
+    __syncthreads();
+    int valid = flag;
+    if (valid) {
+      double* planes = v.inv + (size_t)w * v.nqp * v.n * v.n;
+      valid = !rr_calc_m_all<R>(v, ei, rs, planes, gs, L, pfn);
+    }
+    if (tid == 0) {
+      v.s_valid[ws] = valid;
+      if (valid)
+        for (int q = 0; q < v.nqp; ++q) v.pf[(size_t)w * v.nqp + q] = pfn[q];
+    }
+    __syncthreads();
+  }
+}
+
 // Batched Pfaffian + inverse: plane group g = np consecutive planes; the
 // workgroup at blockIdx.x takes groups blockIdx.x, + gridDim.x, ... of the
 // ngroups, using work slot blockIdx.x (pfinv_work_doubles(n, np) doubles)

@@ -170,6 +170,7 @@ class Ensemble:
             h = _handle         # a run owned by an EnsembleGroup
             self._owned = False
         self.h = h
+        self._destroy = lib().mvmc_ens_destroy   # kept: module globals may be gone at interpreter exit
         d = np.zeros(16, dtype=np.int32)
         lib().mvmc_ens_dims(self.h, _p(d))
         (self.nsite, self.ne, self.n, self.nqp, self.nproj, self.nslater, self.size, self.nsample, self.walkers,
@@ -179,7 +180,7 @@ class Ensemble:
     def close(self):
         if getattr(self, "h", None):
             if getattr(self, "_owned", True):
-                lib().mvmc_ens_destroy(self.h)
+                self._destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -301,7 +302,7 @@ class Ensemble:
                 "block_ms": t[5], "block_launches": t[6], "rebuild_ms": t[7], "rebuilt": t[8],
                 "meas_rebuild_ms": t[9], "measure_ms": t[10], "measured": t[11], "cg_ms": t[12],
                 "cg_iter": t[13], "cg_na": t[14], "cg_rows": t[15], "rebuild_variant": int(t[16]),
-                "rebuild_ms_per_walker": (t[17], t[18])}
+                "rebuild_ms_per_walker": (t[17], t[18], t[19])}
 
     PHASES = ("candidate", "projection", "staging", "ratio", "metropolis", "accept_pass", "rebuild", "exchange",
               "block_end")
@@ -345,6 +346,7 @@ class EnsembleGroup:
         h = ctypes.c_void_p()
         _check(lib().mvmc_group_create(R, nl, ini, _p(wk), ctypes.byref(h)))
         self.h = h
+        self._destroy = lib().mvmc_group_destroy
         self.runs = []
         for r in range(R):
             e = ctypes.c_void_p()
@@ -376,7 +378,7 @@ class EnsembleGroup:
         if getattr(self, "h", None):
             for e in self.runs:
                 e.h = None
-            lib().mvmc_group_destroy(self.h)
+            self._destroy(self.h)
             self.h = None
 
     def __del__(self):

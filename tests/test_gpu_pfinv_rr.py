@@ -1,0 +1,98 @@
+"""Register-resident batched Pfaffian + inverse (64 < n <= 224, one plane per
+1024-thread workgroup) through mvmc_pfinv_f64: Pfaffians, statuses and
+inverses bit-exact against the oracle for every even n of the range's
+register-block sizes, more planes than workgroups, in place, and with zero
+and non-finite planes inside the batch."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def M():
+    import torch
+    torch.zeros(1, device="cuda")
+    torch.cuda.synchronize()
+    import mvmc_hip
+    return mvmc_hip
+
+
+def _skew(rng, count, n):
+    a = rng.standard_normal((count, n, n))
+    return np.ascontiguousarray(a - a.transpose(0, 2, 1))
+
+
+@pytest.mark.parametrize("n", [66, 96, 98, 128, 130, 144, 160, 192, 194, 200, 222, 224])
+def test_rr_bit_exact_vs_oracle(M, n):
+    import torch
+    rng = np.random.default_rng(1000 + n)
+    count = 300 if n == 224 else 40
+    planes = _skew(rng, count, n)
+    planes[count // 3] = 0.0                 # zero pivot at once
+    planes[count // 2] *= 1e300              # non-finite Pfaffian
+    log(f"rr n={n} count={count}")
+    pf_o, inv_o, st_o = O.pfinv(planes)
+    assert st_o[count // 3] > 0 and st_o[count // 2] == -1
+    dp = torch.tensor(planes, device="cuda")
+    pf = torch.zeros(count, dtype=torch.float64, device="cuda")
+    st = torch.zeros(count, dtype=torch.int32, device="cuda")
+    inv = torch.full_like(dp, 5.0)
+    M.pfaffian_inverse_batched(dp, pf, inv, st, n, count)
+    torch.cuda.synchronize()
+    assert np.array_equal(st.cpu().numpy(), st_o)
+    pg = pf.cpu().numpy()
+    assert np.array_equal(pg[np.isfinite(pf_o)], pf_o[np.isfinite(pf_o)])
+    assert np.array_equal(np.isinf(pg), np.isinf(pf_o))
+    assert np.array_equal(inv.cpu().numpy(), inv_o)
+    # in place
+    M.pfaffian_inverse_batched_inplace(dp, pf, st, n, count)
+    torch.cuda.synchronize()
+    assert np.array_equal(dp.cpu().numpy(), inv_o)
+
+
+@pytest.mark.parametrize("case,W", [("uneven10", 2), ("uneven12", 2), ("square16", 1)])
+def test_rr_rebuilds_in_the_sampler_vs_oracle(M, tmp_path, case, W, monkeypatch):
+    """The sampler's periodic rebuild and the measurement rebuild through the
+    register-resident kernels (rebuild variant 2, fixed with
+    MVMC_RB_VARIANT): uneven-batch runs of side 10 and 12 (Nsize 90, 112) and
+    the 16x16 lattice (Nsize 224), one SR-CG step: zvo row, parameters and
+    every walker's state bit-exact against the oracle, with periodic
+    rebuilds done."""
+    import lattice
+    if case == "square16":
+        info = lattice.write_workload("square16", str(tmp_path / case), sr_steps=1, warmup=1)
+    else:
+        side = int(case[len("uneven"):])
+        info = [i for i in lattice.write_uneven_batch(str(tmp_path / "b"), walkers=W, runs_per_side=1, sr_steps=1,
+                                                      warmup=2) if i["L"] == side][0]
+        assert info["nsize"] > 64
+    monkeypatch.setenv("MVMC_RB_VARIANT", "2")
+    log(f"rr rebuild {case} W={W}")
+    ens = M.Ensemble(info["namelist"], info["initial"], walkers=W)
+    row = ens.opt_step()
+    t = ens.timings()
+    log(f"  rebuilt {t['rebuilt']}, variant {t['rebuild_variant']}")
+    assert t["rebuild_variant"] == 2 and t["rebuilt"] > 0, t
+    O.set_math(1)
+    try:
+        run = O.Run(info["namelist"], info["initial"], walkers=W)
+        run.step(do_sr=True)
+        assert np.array_equal(row, run.energies()[-1])
+        assert np.array_equal(np.concatenate(ens.params()), np.concatenate(run.params()))
+        for w in range(W):
+            sg, so = ens.walker(w), run.walker(w)
+            assert np.array_equal(sg["ele_idx"], so["ele_idx"]), w
+            assert np.array_equal(sg["rng"], so["rng"]) and sg["rng_idx"] == so["rng_idx"], w
+    finally:
+        O.set_math(0)
+        ens.close()
