@@ -12,15 +12,23 @@ proposals; --vmc-warmup sets NVMCWarmUp, default 2, which only changes the
 length of that untimed step).
 
 The whole run keeps to a wall budget (--budget seconds from process start,
-default 420): at least one warm-up and two timed steps always run, further
+default 470): at least one warm-up and two timed steps always run, further
 requested warm-up and timed steps only while the measured step time says
-they fit beside the CPU leg.  `steps` / `warmup` in the JSON line are the
+they fit beside the CPU leg and the precision form (one warm-up and two
+timed steps).  `steps` / `warmup` in the JSON line are the
 steps actually run; `steps_requested` / `warmup_requested` what was asked.
 The headline line (value, roofline, cpu_baseline) is printed and flushed as
 soon as the FP64 and CPU legs are done; the secondary legs (direct SR, the
 complex batched Pfaffian, the precision form) run afterwards within what is
 left of the budget, each marked skipped when it does not fit, and a final
 line repeats the headline with their results merged in.
+
+Precision form (sampler inverse stored and updated in FP32): its first step
+starts from the same walker states (seeds, placement) and parameters as the
+FP64 run's first step, so that step is a replay on identical inputs: the
+|dE/site| of its zvo row and the relative difference of its SR update
+against FP64 are reported as `replay_step0`; later steps run from parameters
+that already differ and are reported for speed.
 
 value  = saved samples / second of the whole timed SR steps (all walkers).
 roofline: dominant kernel = the sampling kernel k_block; algorithmic bytes
@@ -129,41 +137,60 @@ def cpu_baseline(workload_dir, walkers, threads, params, min_seconds=10.0, max_s
             "host": hi}
 
 
-def mixed_form(info, walkers, warmup, steps, rows64, total64, deadline):
-    """Warm-up and timed steps of the FP64 run repeated with the sampler's
-    inverse in FP32 (same inputs and seeds); |dE/site| per timed step against
-    the FP64 run's zvo rows.  Stops at `deadline` (after at least one timed
-    step)."""
+def mixed_form(info, walkers, p_init, step0, rows64, total64, deadline):
+    """The FP64 run repeated with the sampler's inverse in FP32 (same inputs
+    and seeds): one thermalising step, then timed steps (as many as the FP64
+    run timed, stopping at `deadline` after at least one).  The first step
+    starts from the same walker states and parameters as the FP64 run's
+    first step (`step0` = its zvo row and parameters after it, `p_init` the
+    parameters before it): |dE/site| of that row and the relative difference
+    of the SR update are a replay on identical inputs.  The timed steps
+    start from parameters that already differ (the SR updates differ in the
+    last bits), so their |dE/site| against the FP64 rows is of the size of
+    the Monte Carlo error once any decision flips."""
     import mvmc_hip
+    nsite = info["nsite"]
     ens = mvmc_hip.Ensemble(info["namelist"], info["initial"], walkers=walkers)
     ens.set_precision("mixed")
-    for s in range(warmup):
-        t = time.time()
-        ens.opt_step()
-        log(f"mixed warmup step {s}: {time.time() - t:.1f} s")
+    t = time.time()
+    row0 = ens.opt_step()
+    p1 = ens.params()
+    log(f"mixed first step: {time.time() - t:.1f} s")
+    nsample = ens.nsample
+    replay = None
+    if step0 is not None:
+        row64, p1_64 = step0
+        d64 = np.concatenate(p1_64) - np.concatenate(p_init)
+        d32 = np.concatenate(p1) - np.concatenate(p_init)
+        replay = {"abs_dE_per_site": abs(row0[0] - row64[0]) / nsite,
+                  "energy_per_site_fp64": row64[0] / nsite, "energy_per_site_mixed": row0[0] / nsite,
+                  "fp64_stat_err_per_site": float(np.sqrt(max(row64[2] - row64[0] ** 2, 0.0) /
+                                                          (walkers * nsample))) / nsite,
+                  "sr_update_rel_diff": float(np.linalg.norm(d32 - d64) / max(np.linalg.norm(d64), 1e-300)),
+                  "what": "first SR step of both runs from identical walker states, seeds and parameters: "
+                          "the zvo energy and the SR parameter update of the precision form against FP64"}
     tm0 = ens.timings()
     times, rows = [], []
-    for s in range(steps):
+    for s in range(len(total64)):
         if s >= 1 and time.time() + max(times) > deadline:
             break
         t = time.time()
         rows.append(ens.opt_step())
         times.append(time.time() - t)
-        log(f"mixed step {s}: E/site = {rows[-1][0] / info['nsite']:.8f}  {times[-1]:.2f} s")
+        log(f"mixed step {s}: E/site = {rows[-1][0] / nsite:.8f}  {times[-1]:.2f} s")
     tm1 = ens.timings()
     total = sum(times)
-    nsample = ens.nsample
     ens.close()
     k = len(times)
-    de = [abs(r[0] - r64[0]) / info["nsite"] for r, r64 in zip(rows, rows64)]
-    # Monte Carlo standard error of E/site of the FP64 step (zvo row: Etot, ., Etot2, ...):
-    # once a decision flips, the chains differ and dE is of this size
-    se = [float(np.sqrt(max(r64[2] - r64[0] ** 2, 0.0) / (walkers * nsample))) / info["nsite"] for r64 in rows64[:k]]
+    de = [abs(r[0] - r64[0]) / nsite for r, r64 in zip(rows, rows64)]
+    se = [float(np.sqrt(max(r64[2] - r64[0] ** 2, 0.0) / (walkers * nsample))) / nsite for r64 in rows64[:k]]
     t64 = total64[:k]
     return {"value": walkers * nsample * k / total, "unit": "samples/s", "steps": k,
             "speedup_vs_fp64": sum(t64) / total, "sr_step_s": total / k,
             "sampling_s_per_step": (tm1["sample_s"] - tm0["sample_s"]) / k,
-            "energy_per_site": [r[0] / info["nsite"] for r in rows], "abs_dE_per_site_vs_fp64": de,
+            "sr_s_per_step": (tm1["sr_s"] - tm0["sr_s"]) / k,
+            "replay_step0": replay,
+            "energy_per_site": [r[0] / nsite for r in rows], "abs_dE_per_site_vs_fp64_steps": de,
             "fp64_stat_err_per_site": se,
             "arithmetic": "sampler inverse stored in FP32, its rank-2 updates in FP32; SR-CG samples stored in "
                           "FP32, accumulated in FP64; ratios, vec1 sums, Pfaffians, periodic rebuilds, "
@@ -213,7 +240,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2, help="timed SR steps requested (at least 2 always run)")
     ap.add_argument("--warmup", type=int, default=1, help="untimed SR steps requested (at least 1 always runs)")
-    ap.add_argument("--budget", type=float, default=420.0,
+    ap.add_argument("--budget", type=float, default=470.0,
                     help="wall seconds from process start for the whole run; optional steps and legs that do "
                          "not fit are left out")
     ap.add_argument("--workload", default="square16")
@@ -266,6 +293,10 @@ def main():
     cpu_threads = args.cpu_threads or max(1, min(16, hi["cores_available"]))
     cpu_reserve = 0.0 if args.no_cpu else 45.0     # oracle warm-up step + >= 10 s timed + set-up
     margin = 15.0
+
+    def mixed_reserve(first, est):
+        # precision form: set-up, its first (thermalising) step and two timed steps
+        return 0.0 if args.no_mixed else 15.0 + 0.95 * first + 2.0 * est
     deadline = T_START + args.budget
 
     wl_dir = os.path.join(tempfile.gettempdir(), f"mvmc_bench_{args.workload}")
@@ -280,14 +311,18 @@ def main():
 
     # ---- FP64 leg: warm-up steps, then timed steps, within the budget
     warm_times = []
+    p_init = ens.params()
+    step0 = None
     for s in range(max(args.warmup, 1)):
         est = warm_times[-1] if warm_times else 0.0
-        if s >= 1 and time.time() + est * 3 + cpu_reserve + margin > deadline:
+        if s >= 1 and time.time() + est * 3 + cpu_reserve + mixed_reserve(warm_times[0], est) + margin > deadline:
             log(f"warm-up stops after {s} steps (budget)")
             break
         t = time.time()
         row = ens.opt_step()
         warm_times.append(time.time() - t)
+        if s == 0:
+            step0 = (row, ens.params())
         log(f"warmup step {s}: E/site = {row[0] / info['nsite']:.8f}  ({warm_times[-1]:.1f} s)")
     first_step_s = warm_times[0]
     params_at_timed = ens.params()
@@ -295,10 +330,11 @@ def main():
     step_times, rows, kernel, launch_logs = [], [], [], []
     for s in range(max(args.steps, 2)):
         est = step_times[-1] if step_times else (warm_times[-1] if len(warm_times) > 1 else first_step_s)
-        if s >= 2 and time.time() + est + cpu_reserve + margin > deadline:
+        if s >= 2 and time.time() + est + cpu_reserve + mixed_reserve(first_step_s, est) + margin > deadline:
             log(f"timed steps stop after {s} (budget)")
             break
         before = ens.timings()
+        p_last = ens.params()
         t = time.time()
         row = ens.opt_step()
         dt = time.time() - t
@@ -410,6 +446,38 @@ def main():
 
     # ---- secondary legs, each only if it fits in what is left of the budget
     skipped = {}
+    if not args.no_mixed:
+        # precision form (1): the same workload and seeds with the sampler's
+        # resident inverse in FP32
+        need = mixed_reserve(first_step_s, total / nsteps) + margin
+        if time.time() + need < deadline:
+            out["precision_forms"] = {"mixed_fp32_inverse": mixed_form(
+                info, walkers, p_init, step0, rows, step_times, deadline - margin)}
+        else:
+            skipped["precision_form"] = "budget"
+    if time.time() + 20 + margin < deadline:
+        # derivative form (3): VJP of the last measurement by the adjoint
+        # kernels (no O rows) and over the stored O rows
+        rng = np.random.default_rng(0)
+        cot = rng.standard_normal(walkers * ens.nsample)
+        ens.set_params(*p_last)   # the parameters the last measurement's O rows were formed with
+        t = time.time()
+        g_rows = ens.vjp(cot)
+        t_rows = time.time() - t
+        g_adj = ens.vjp_adjoint(cot)   # first call: accumulators and orbital map
+        t = time.time()
+        g_adj = ens.vjp_adjoint(cot)
+        t_adj = time.time() - t
+        ta = ens.timings()
+        scale = float(np.max(np.abs(g_rows)))
+        sec["derivative form (3): adjoint VJP"] = {
+            "samples": walkers * ens.nsample, "parameters": ens.size - 1,
+            "adjoint_wall_ms": 1e3 * t_adj, "adjoint_device_ms": ta["adjoint_vjp_ms"],
+            "of_which_plane_rebuild_ms": ta["adjoint_vjp_rebuild_ms"],
+            "stored_rows_vjp_wall_ms": 1e3 * t_rows,
+            "max_abs_diff_vs_rows_over_max": float(np.max(np.abs(g_adj - g_rows))) / max(scale, 1e-300)}
+    else:
+        skipped["derivative_form"] = "budget"
     if not args.no_direct:
         if time.time() + 30 + margin < deadline:
             # direct SR (NSRCG 0 path) on the same measured operands: Gram by
@@ -433,15 +501,6 @@ def main():
             torch.cuda.empty_cache()
         else:
             skipped["k_pfinv_c128"] = "budget"
-    if not args.no_mixed:
-        # precision form (1): the same workload, seeds and warm-up with the
-        # sampler's resident inverse in FP32; energies against the FP64 rows
-        need = first_step_s + sum(warm_times[1:]) + 2 * (total / nsteps) + margin
-        if time.time() + need < deadline:
-            out["precision_forms"] = {"mixed_fp32_inverse": mixed_form(
-                info, walkers, len(warm_times), nsteps, rows, step_times, deadline - margin)}
-        else:
-            skipped["precision_form"] = "budget"
     if skipped:
         out["skipped_legs"] = skipped
     out["wall_s"] = elapsed()

@@ -186,12 +186,27 @@ int mvmc_ens_get_o_store(mvmc_ens* e, int walker, double* o_nsample_size);
  * reference entry points: they expose the derivative the reference computes
  * inside its measurement. */
 int mvmc_ens_vjp(mvmc_ens* e, const double* cot, double* grad);
+/* mvmc_ens_vjp_adjoint: the same VJP by the adjoint kernels, without the O
+ * rows: for every saved sample (its planes rebuilt as for the measurement)
+ * each inverse element, scaled by cot[r] w_q Pf_q / ip (d log Pf(A) =
+ * tr(A^-1 dA) / 2), is added at the Slater-table position it was gathered
+ * from; the orbital map of update_slater_elm (slater_update.rs:17-126:
+ * translations, orbital signs, spin-projection factors) is then applied once
+ * to the sums instead of once per sample and QP point as the per-parameter
+ * assembly does (slater_derivative.rs:289-356,490-548).  Same host arrays
+ * as mvmc_ens_vjp; sums in a fixed order (per-workgroup accumulators over
+ * walkers in order, folded in workgroup order), not that of mvmc_ens_vjp
+ * (agreement to rounding).  Uses the saved configurations of the last
+ * sampling call and the current parameters, and leaves the measurement
+ * planes rebuilt (as mvmc_ens_log_psi).  Device memory: up to 4 GiB of
+ * per-workgroup accumulators, allocated on the first call. */
+int mvmc_ens_vjp_adjoint(mvmc_ens* e, const double* cot, double* grad);
 int mvmc_ens_log_psi(mvmc_ens* e, double* log_psi);
 int mvmc_ens_get_walker(mvmc_ens* e, int walker, int* saved_ele_idx, long long* counters4, uint32_t* rng625);
 int mvmc_ens_get_tables(mvmc_ens* e, int walker, double* inv, double* pf);
 int mvmc_ens_get_config(mvmc_ens* e, int walker, int* ele_idx, int* ele_cfg, int* ele_num, int* proj_cnt);
 int mvmc_ens_get_cg(mvmc_ens* e, double* solution, int* n_active);
-/* timings[20]: sampling s, measurement s, SR s (wall, summed over calls),
+/* timings[24]: sampling s, measurement s, SR s (wall, summed over calls),
  * proposals, accepted, device ms of the last sampling call's k_block
  * launches, their number, device ms of its periodic-rebuild launches,
  * walkers those rebuilt; device ms of the last measurement call's rebuilds
@@ -199,9 +214,11 @@ int mvmc_ens_get_cg(mvmc_ens* e, double* solution, int* n_active);
  * last SR call's CG solve, its iterations, active parameters, sample rows
  * (HIP events on the library's stream); the rebuild-kernel variant in use
  * (-1 while it is being measured; 0: three workgroups per CU, 8-column
- * panel-close batches; 1: two per CU, 16 columns) and the device ms per
- * rebuilt walker measured for variants 0 and 1; reserved. */
-int mvmc_ens_timings(mvmc_ens* e, double* timings20);
+ * panel-close batches; 1: two per CU, 16 columns; 2: register-resident,
+ * one 1024-thread workgroup per CU, 64 < n <= 224) and the device ms per
+ * rebuilt walker measured for variants 0, 1 and 2; device ms of the last
+ * mvmc_ens_vjp_adjoint call and of its plane rebuilds; two reserved. */
+int mvmc_ens_timings(mvmc_ens* e, double* timings24);
 /* Device phase clocks of the sampling kernel, summed over walkers (ticks of
  * the 100 MHz constant clock): 0 candidate, 1 projection deltas + ratio,
  * 2 inverse/Slater row staging, 3 Pfaffian ratio dots, 4 Metropolis,

@@ -417,9 +417,18 @@ struct mvmc_ens {
   size_t S_cap = 0;
   int* d_info = nullptr;
   double* d_grad = nullptr;  // [size]
+  // adjoint VJP: per-slot Slater-table cotangents [adj_slots][nqp][n2^2] and
+  // constant + projection sums [adj_slots][1 + nproj]; the orbital map as
+  // CSR (per orbital: table positions in ascending order and their weights)
+  int adj_slots = 0;
+  double* d_sbar = nullptr;
+  double* d_gproj = nullptr;
+  int* d_adj_start = nullptr;
+  long long* d_adj_ent = nullptr;
+  double* d_adj_wt = nullptr;
   // device times of the last calls (HIP events on the library's stream)
   double rebuilt = 0, meas_rebuild_ms = 0, measure_ms = 0, measured = 0, cg_ms = 0, cg_iter = 0, cg_na = 0,
-         cg_rows = 0;
+         cg_rows = 0, adj_ms = 0, adj_rebuild_ms = 0;
   ~mvmc_ens() {
     cg.release();
     if (d_S) (void)hipFree(d_S);
@@ -1946,6 +1955,118 @@ int mvmc_ens_vjp(mvmc_ens* e, const double* cot, double* grad) {
   });
 }
 
+__global__ void k_fill64(double* p, long long n, double v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Orbital map of the adjoint VJP, built once per handle: for every plane q
+// (translation q / NSPGaussLeg, Gauss-Legendre point q % NSPGaussLeg) and
+// Slater-table position (a, b) = (site + spin Nsite, site' + spin' Nsite)
+// the orbital the position's element derives from and its weight, the same
+// factors the per-sample assembly applies (translation signs, orbital sign,
+// cos/sin products of the spin block), grouped by orbital in ascending
+// (q, a, b) order.
+void adj_prepare(mvmc_ens* e) {
+  EnsView& v = e->v;
+  if (e->d_sbar) return;
+  int dev = 0, ncu = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const size_t pl = (size_t)v.n2 * v.n2, per = (size_t)v.nqp * pl;
+  const size_t cap = ((size_t)4 << 30) / sizeof(double);  // at most 4 GiB of slot accumulators
+  e->adj_slots = (int)std::max<size_t>(1, std::min<size_t>({(size_t)v.W, (size_t)std::max(ncu, 1), cap / per}));
+  e->d_sbar = dalloc<double>(e->allocs, (size_t)e->adj_slots * per);
+  e->d_gproj = dalloc<double>(e->allocs, (size_t)e->adj_slots * (1 + v.nproj));
+  const mvmc::Model& m = e->m;
+  std::vector<int> ts, tg;
+  mvmc::translation_maps(m, ts, tg);
+  const int ns = v.ns, nleg = std::max(v.nleg, 1);
+  std::vector<int> cnt((size_t)v.nslater + 1, 0);
+  auto visit = [&](auto&& f) {
+    for (int q = 0; q < v.nqp; ++q) {
+      const int tr = q / nleg, sp = q % nleg;
+      const double cs = m.spgl_cos_sin[sp], cc = m.spgl_cos_cos[sp], ss = m.spgl_sin_sin[sp];
+      for (int a = 0; a < v.n2; ++a)
+        for (int b = 0; b < v.n2; ++b) {
+          const int sa = a % ns, sb = b % ns;
+          const bool up_a = a < ns, up_b = b < ns;
+          const int ti = ts[(size_t)tr * ns + sa], tj = ts[(size_t)tr * ns + sb];
+          const int orb = m.orb_idx[(size_t)ti * ns + tj];
+          if (orb < 0 || orb >= v.nslater) continue;
+          const int sg = tg[(size_t)tr * ns + sa] * tg[(size_t)tr * ns + sb] * m.orb_sgn[(size_t)ti * ns + tj];
+          const double fac = up_b ? (up_a ? cs : ss) : -(up_a ? cc : cs);
+          f(orb, (long long)q * (long long)pl + (long long)a * v.n2 + b, fac * (double)sg);
+        }
+    }
+  };
+  visit([&](int orb, long long, double) { ++cnt[(size_t)orb + 1]; });
+  for (int o = 0; o < v.nslater; ++o) cnt[(size_t)o + 1] += cnt[o];
+  if (cnt[v.nslater] > INT32_MAX) throw Fail(MVMC_E_INVALID_SHAPE, "orbital map too large for the adjoint VJP");
+  std::vector<long long> ent((size_t)std::max(cnt[v.nslater], 1));
+  std::vector<double> wt(ent.size());
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  visit([&](int orb, long long k, double w) {
+    const int t = fill[orb]++;
+    ent[t] = k;
+    wt[t] = w;
+  });
+  e->d_adj_start = dupload(e->allocs, cnt, e->st);
+  e->d_adj_ent = dupload(e->allocs, ent, e->st);
+  e->d_adj_wt = dupload(e->allocs, wt, e->st);
+}
+
+int mvmc_ens_vjp_adjoint(mvmc_ens* e, const double* cot, double* grad) {
+  return guarded([&] {
+    require_gpu();
+    EnsView& v = e->v;
+    if (!cot || !grad) throw Fail(MVMC_E_INVALID_SHAPE, "null buffer");
+    const size_t rows = (size_t)v.W * v.nsample;
+    if (!e->d_rows) e->d_rows = dalloc<double>(e->allocs, rows);
+    if (!e->d_grad) e->d_grad = dalloc<double>(e->allocs, (size_t)v.size);
+    adj_prepare(e);
+    e->upload_tables();
+    HIPCHK(hipMemcpyAsync(e->d_rows, cot, sizeof(double) * rows, hipMemcpyHostToDevice, e->st));
+    const long long len = (long long)v.nqp * v.n2 * v.n2, glen = 1 + v.nproj;
+    const long long tot = len * e->adj_slots;
+    hipLaunchKernelGGL(k_fill64, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, e->st, e->d_sbar, tot, 0.0);
+    hipLaunchKernelGGL(k_fill64, dim3((unsigned)((glen * e->adj_slots + 255) / 256)), dim3(256), 0, e->st,
+                       e->d_gproj, glen * e->adj_slots, 0.0);
+    hipEvent_t ev[3];
+    for (auto& x : ev) HIPCHK(hipEventCreate(&x));
+    const size_t lds = 2 * rnd16((size_t)v.n * 4) + rnd16((size_t)v.nqp * 8);
+    float rb = 0, tot_ms = 0;
+    HIPCHK(hipEventRecord(ev[0], e->st));
+    for (int s = 0; s < v.nsample; ++s) {
+      HIPCHK(hipEventRecord(ev[1], e->st));
+      launch_meas_rebuild(e, s);
+      HIPCHK(hipEventRecord(ev[2], e->st));
+      hipLaunchKernelGGL(k_adj_scatter, dim3(e->adj_slots), dim3(256), lds, e->st, v, s, e->d_rows, e->d_sbar,
+                         e->d_gproj);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipEventSynchronize(ev[2]));
+      float a = 0;
+      HIPCHK(hipEventElapsedTime(&a, ev[1], ev[2]));
+      rb += a;
+    }
+    hipLaunchKernelGGL(k_adj_fold, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, e->st, e->d_sbar,
+                       e->adj_slots, len);
+    hipLaunchKernelGGL(k_adj_fold, dim3((unsigned)((glen + 255) / 256)), dim3(256), 0, e->st, e->d_gproj,
+                       e->adj_slots, glen);
+    hipLaunchKernelGGL(k_adj_orbital, dim3(nblk(v.size)), dim3(256), 0, e->st, e->d_sbar, e->d_adj_start,
+                       e->d_adj_ent, e->d_adj_wt, e->d_gproj, v.nproj, v.nslater, e->d_grad);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ev[1], e->st));
+    HIPCHK(hipEventSynchronize(ev[1]));
+    HIPCHK(hipEventElapsedTime(&tot_ms, ev[0], ev[1]));
+    for (auto& x : ev) (void)hipEventDestroy(x);
+    e->adj_ms = tot_ms;
+    e->adj_rebuild_ms = rb;
+    HIPCHK(hipMemcpyAsync(grad, e->d_grad, sizeof(double) * v.size, hipMemcpyDeviceToHost, e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
+  });
+}
+
 int mvmc_ens_set_precision(mvmc_ens* e, int mode) {
   return guarded([&] {
     if (mode != 0 && mode != 1) throw Fail(MVMC_E_INVALID_SHAPE, "precision form must be 0 (FP64) or 1 (mixed)");
@@ -1976,6 +2097,7 @@ int mvmc_ens_timings(mvmc_ens* e, double* t) {
   t[17] = e->rb_walkers[0] > 0 ? e->rb_ms[0] / e->rb_walkers[0] : 0.0;
   t[18] = e->rb_walkers[1] > 0 ? e->rb_ms[1] / e->rb_walkers[1] : 0.0;
   t[19] = e->rb_walkers[2] > 0 ? e->rb_ms[2] / e->rb_walkers[2] : 0.0;
+  t[20] = e->adj_ms; t[21] = e->adj_rebuild_ms; t[22] = 0.0; t[23] = 0.0;
   return MVMC_OK;
 }
 

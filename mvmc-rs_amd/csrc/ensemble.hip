@@ -1754,6 +1754,97 @@ __global__ void k_vjp(EnsView v, const double* cot, double* grad) {
   grad[k] = acc;
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint VJP (derivative form): grad = sum_r cot[r] O[r] without the
+// per-sample O rows.  d log Pf(A) = tr(A^-1 dA) / 2, so the cotangent of the
+// gathered plane is the inverse itself; by linearity of the Slater table in
+// the orbital parameters (update_slater_elm, slater_update.rs:17-126) the
+// orbital map is applied once to the sum over samples instead of once per
+// sample (slater_derivative.rs:289-356 assembles per sample and per QP).
+// ---------------------------------------------------------------------------
+// Saved sample s of every walker (after k_meas_rebuild(s)): element (i, j)
+// of inverse plane q goes back to the Slater-table position it was gathered
+// from, sbar[q][rs_i][rs_j] += (cot w_q Pf_q / ip) invM_q[i][j] (rs_i =
+// site + spin * Nsite of electron i: distinct electrons sit on distinct
+// (site, spin), so one sample writes every position at most once per
+// plane).  Workgroup `slot` owns sbar + slot * nqp * n2^2 and gproj +
+// slot * (1 + nproj) and takes walkers slot, slot + nslots, ... in order,
+// so every accumulator sums in a fixed order.  gproj gets cot (the O row's
+// constant component) and cot * cnt_p (the projection components).
+// Invalid samples (zero O rows) add nothing.
+__global__ void __launch_bounds__(256) k_adj_scatter(EnsView v, int s, const double* cot, double* sbar,
+                                                     double* gproj) {
+  extern __shared__ __align__(16) char lds[];
+  Carver c{lds};
+  int* rs = c.take<int>(v.n);
+  double* coef = c.take<double>(v.nqp);
+  __shared__ int ok_s;
+  const int slot = blockIdx.x, nslots = gridDim.x, tid = threadIdx.x, nt = blockDim.x;
+  const int n = v.n, n2 = v.n2, nqp = v.nqp;
+  const size_t nn = (size_t)n * n, pl = (size_t)n2 * n2;
+  double* sb = sbar + (size_t)slot * nqp * pl;
+  double* gp = gproj + (size_t)slot * (1 + v.nproj);
+  for (int w = slot; w < v.W; w += nslots) {
+    const size_t ws = (size_t)w * v.nsample + s;
+    const double ct = cot[ws];
+    if (tid == 0) {
+      int ok = v.s_valid[ws];
+      const double* pf = v.pf + (size_t)w * nqp;
+      if (ok) {
+        const double ip = ip_of(v, pf);
+        if (fabs(ip) < 1.0e-100) ok = 0;
+        const double inv_ip = julia_reciprocal_re(ip);
+        for (int q = 0; q < nqp; ++q) coef[q] = ct * (v.qpw[q] * (pf[q] * inv_ip));
+      }
+      ok_s = ok;
+    }
+    for (int i = tid; i < n; i += nt) rs[i] = v.s_idx[ws * n + i] + (i / v.ne) * v.ns;
+    __syncthreads();
+    if (ok_s) {
+      if (tid == 0) gp[0] += ct;
+      for (int p = tid; p < v.nproj; p += nt) gp[1 + p] += ct * (double)v.s_proj[ws * v.nproj + p];
+      if (v.nslater > 0) {
+        const double* planes = v.inv + (size_t)w * nqp * nn;
+        for (size_t e = tid; e < (size_t)nqp * nn; e += nt) {
+          const int q = (int)(e / nn);
+          const int r = (int)(e % nn);
+          const int i = r / n, j = r % n;
+          double& x = sb[q * pl + (size_t)rs[i] * n2 + rs[j]];
+          x += coef[q] * planes[e];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// buf[e] = sum over slots (ascending) of buf[slot * len + e].
+__global__ void k_adj_fold(double* buf, int nslots, long long len) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= len) return;
+  double a = buf[e];
+  for (int sl = 1; sl < nslots; ++sl) a += buf[(size_t)sl * len + e];
+  buf[e] = a;
+}
+
+// grad[0 .. nproj] = gproj; grad[1 + nproj + orb] = sum over the orbital's
+// table positions k = ent[start[orb] .. start[orb + 1]) (ascending (q, a,
+// b)) of sbar[k] * wt[k], wt = translation signs x orbital sign x the
+// Gauss-Legendre spin factor of the (spin a, spin b) block.
+__global__ void k_adj_orbital(const double* sbar, const int* start, const long long* ent, const double* wt,
+                              const double* gproj, int nproj, int nslater, double* grad) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < 1 + nproj) {
+    grad[k] = gproj[k];
+    return;
+  }
+  const int orb = k - 1 - nproj;
+  if (orb >= nslater) return;
+  double a = 0.0;
+  for (int t = start[orb]; t < start[orb + 1]; ++t) a += sbar[ent[t]] * wt[t];
+  grad[k] = a;
+}
+
 __global__ void k_reduce_energy(EnsView v, double* out5) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   double tot[5] = {0, 0, 0, 0, 0};

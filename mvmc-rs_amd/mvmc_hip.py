@@ -93,7 +93,7 @@ EXPORTED_SYMBOLS = [
     "mvmc_ens_measure", "mvmc_ens_sr_cg", "mvmc_ens_opt_step", "mvmc_ens_get_energy", "mvmc_ens_get_operands",
     "mvmc_ens_get_o_store", "mvmc_ens_get_walker", "mvmc_ens_get_tables", "mvmc_ens_get_config", "mvmc_ens_get_cg",
     "mvmc_ens_timings", "mvmc_ens_phase_clocks", "mvmc_ens_status", "mvmc_sr_cg_solve", "mvmc_ens_set_precision",
-    "mvmc_ens_vjp", "mvmc_ens_log_psi", "mvmc_ens_sr_direct", "mvmc_pfinv_c128",
+    "mvmc_ens_vjp", "mvmc_ens_vjp_adjoint", "mvmc_ens_log_psi", "mvmc_ens_sr_direct", "mvmc_pfinv_c128",
     "mvmc_ens_measure_configs", "mvmc_sr_cg_apply", "mvmc_sr_gram_real", "mvmc_sr_cholesky_solve",
     "mvmc_ens_write_opt_data", "mvmc_ens_set_output_dir",
 ]
@@ -243,6 +243,18 @@ class Ensemble:
         _check(lib().mvmc_ens_vjp(self.h, _p(c), _p(g)))
         return g
 
+    def vjp_adjoint(self, cot):
+        """The same VJP by the adjoint kernels (no O rows: inverse elements
+        back onto the Slater table, the orbital map applied once); uses the
+        saved configurations of the last sample call and rebuilds their
+        planes."""
+        c = np.ascontiguousarray(cot, dtype=np.float64).reshape(-1)
+        if c.size != self.walkers * self.nsample:
+            raise InvalidShape(f"cotangent length {c.size} != walkers * NVMCSample")
+        g = np.zeros(self.size)
+        _check(lib().mvmc_ens_vjp_adjoint(self.h, _p(c), _p(g)))
+        return g
+
     def log_psi(self):
         """log|psi| of every saved sample at the current parameters, [W, NVMCSample]."""
         out = np.zeros(self.walkers * self.nsample)
@@ -296,13 +308,14 @@ class Ensemble:
         return st, x[:na.value].copy()
 
     def timings(self):
-        t = np.zeros(20)
+        t = np.zeros(24)
         lib().mvmc_ens_timings(self.h, _p(t))
         return {"sample_s": t[0], "measure_s": t[1], "sr_s": t[2], "proposals": t[3], "accepted": t[4],
                 "block_ms": t[5], "block_launches": t[6], "rebuild_ms": t[7], "rebuilt": t[8],
                 "meas_rebuild_ms": t[9], "measure_ms": t[10], "measured": t[11], "cg_ms": t[12],
                 "cg_iter": t[13], "cg_na": t[14], "cg_rows": t[15], "rebuild_variant": int(t[16]),
-                "rebuild_ms_per_walker": (t[17], t[18], t[19])}
+                "rebuild_ms_per_walker": (t[17], t[18], t[19]), "adjoint_vjp_ms": t[20],
+                "adjoint_vjp_rebuild_ms": t[21]}
 
     PHASES = ("candidate", "projection", "staging", "ratio", "metropolis", "accept_pass", "rebuild", "exchange",
               "block_end")
