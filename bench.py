@@ -59,6 +59,7 @@ for sub in ("mvmc-rs_amd", "oracle"):
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_MFMA_PEAK_TFLOPS = 157.3
 METRIC = "Metropolis samples/sec + SR-step s, 16x16 Hubbard Nsize 224; |dE/site| vs FP64"
 
 
@@ -137,7 +138,7 @@ def cpu_baseline(workload_dir, walkers, threads, params, min_seconds=10.0, max_s
             "host": hi}
 
 
-def mixed_form(info, walkers, p_init, step0, rows64, total64, deadline):
+def mixed_form(info, walkers, p_init, step0, rows64, times64, deadline):
     """The FP64 run repeated with the sampler's inverse in FP32 (same inputs
     and seeds): one thermalising step, then timed steps (as many as the FP64
     run timed, stopping at `deadline` after at least one).  The first step
@@ -147,7 +148,10 @@ def mixed_form(info, walkers, p_init, step0, rows64, total64, deadline):
     of the SR update are a replay on identical inputs.  The timed steps
     start from parameters that already differ (the SR updates differ in the
     last bits), so their |dE/site| against the FP64 rows is of the size of
-    the Monte Carlo error once any decision flips."""
+    the Monte Carlo error once any decision flips.  times64: the FP64 run's
+    step times from its first step on; the speedup compares steps of the same
+    index (the step time grows with the acceptance as the parameters
+    improve, so only steps at the same point of the optimisation compare)."""
     import mvmc_hip
     nsite = info["nsite"]
     ens = mvmc_hip.Ensemble(info["namelist"], info["initial"], walkers=walkers)
@@ -171,7 +175,7 @@ def mixed_form(info, walkers, p_init, step0, rows64, total64, deadline):
                           "the zvo energy and the SR parameter update of the precision form against FP64"}
     tm0 = ens.timings()
     times, rows = [], []
-    for s in range(len(total64)):
+    for s in range(max(1, min(2, len(times64) - 1))):
         if s >= 1 and time.time() + max(times) > deadline:
             break
         t = time.time()
@@ -182,11 +186,12 @@ def mixed_form(info, walkers, p_init, step0, rows64, total64, deadline):
     total = sum(times)
     ens.close()
     k = len(times)
-    de = [abs(r[0] - r64[0]) / nsite for r, r64 in zip(rows, rows64)]
-    se = [float(np.sqrt(max(r64[2] - r64[0] ** 2, 0.0) / (walkers * nsample))) / nsite for r64 in rows64[:k]]
-    t64 = total64[:k]
+    de = [abs(r[0] - r64[0]) / nsite for r, r64 in zip(rows, rows64[1:])]
+    se = [float(np.sqrt(max(r64[2] - r64[0] ** 2, 0.0) / (walkers * nsample))) / nsite for r64 in rows64[1:k + 1]]
+    t64 = times64[1:k + 1]
     return {"value": walkers * nsample * k / total, "unit": "samples/s", "steps": k,
-            "speedup_vs_fp64": sum(t64) / total, "sr_step_s": total / k,
+            "speedup_vs_fp64": sum(t64) / total if len(t64) == k else None,
+            "fp64_same_steps_s": t64, "sr_step_s": total / k, "sr_step_s_each": times,
             "sampling_s_per_step": (tm1["sample_s"] - tm0["sample_s"]) / k,
             "sr_s_per_step": (tm1["sr_s"] - tm0["sr_s"]) / k,
             "replay_step0": replay,
@@ -310,7 +315,7 @@ def main():
     log(f"ensemble created in {time.time() - t:.1f} s (grid {ens.grid}, LDS {ens.lds_sampler} B)")
 
     # ---- FP64 leg: warm-up steps, then timed steps, within the budget
-    warm_times = []
+    warm_times, warm_rows = [], []
     p_init = ens.params()
     step0 = None
     for s in range(max(args.warmup, 1)):
@@ -321,6 +326,7 @@ def main():
         t = time.time()
         row = ens.opt_step()
         warm_times.append(time.time() - t)
+        warm_rows.append(row)
         if s == 0:
             step0 = (row, ens.params())
         log(f"warmup step {s}: E/site = {row[0] / info['nsite']:.8f}  ({warm_times[-1]:.1f} s)")
@@ -452,7 +458,7 @@ def main():
         need = mixed_reserve(first_step_s, total / nsteps) + margin
         if time.time() + need < deadline:
             out["precision_forms"] = {"mixed_fp32_inverse": mixed_form(
-                info, walkers, p_init, step0, rows, step_times, deadline - margin)}
+                info, walkers, p_init, step0, warm_rows + rows, warm_times + step_times, deadline - margin)}
         else:
             skipped["precision_form"] = "budget"
     if time.time() + 20 + margin < deadline:
@@ -482,15 +488,24 @@ def main():
         if time.time() + 30 + margin < deadline:
             # direct SR (NSRCG 0 path) on the same measured operands: Gram by
             # DGEMM on the matrix cores + Cholesky
-            ens.sr_direct()   # first call: rocBLAS / rocSOLVER initialisation
+            ens.sr_direct()   # first call: rocSOLVER initialisation
             ens.sr_direct()
             td = ens.timings()
             na_, rows_ = td["cg_na"], td["cg_rows"]
-            fl = 2.0 * na_ * na_ * rows_ + na_ ** 3 / 3.0 + 2.0 * na_ * na_   # DGEMM (full Gram) + Cholesky
-            sec["direct SR (d, NSRCG 0): DGEMM + DPOTRF/DPOTRS"] = {
-                "active_parameters": na_, "rows": rows_, "ms": td["cg_ms"],
-                "achieved_TFLOPs": fl / (td["cg_ms"] / 1e3) / 1e12,
-                "frac_fp64_mfma": fl / (td["cg_ms"] / 1e3) / 1e12 / FP64_PEAK_TFLOPS}
+            gram_fl = 1.0 * na_ * na_ * rows_            # upper triangle of the Gram (SYRK count)
+            fl = gram_fl + na_ ** 3 / 3.0 + 2.0 * na_ * na_   # + Cholesky and the two triangular solves
+            leg = {"active_parameters": na_, "rows": rows_, "ms": td["cg_ms"],
+                   "achieved_TFLOPs": fl / (td["cg_ms"] / 1e3) / 1e12,
+                   "gram_mfma_fp64": {"ms": td["gram_ms"], "achieved_TFLOPs": gram_fl / (td["gram_ms"] / 1e3) / 1e12,
+                                      "frac_fp64_mfma_peak": gram_fl / (td["gram_ms"] / 1e3) / 1e12 / FP64_PEAK_TFLOPS}}
+            ens.set_precision("mixed")   # samples packed in FP32, FP32-MFMA Gram
+            ens.sr_direct()
+            tf = ens.timings()
+            leg["gram_mfma_fp32"] = {"ms": tf["gram_ms"], "achieved_TFLOPs": gram_fl / (tf["gram_ms"] / 1e3) / 1e12,
+                                     "frac_fp32_mfma_peak": gram_fl / (tf["gram_ms"] / 1e3) / 1e12 / FP32_MFMA_PEAK_TFLOPS,
+                                     "solve_ms": tf["cg_ms"]}
+            ens.set_precision("fp64")
+            sec["direct SR (d, NSRCG 0): MFMA Gram + DPOTRF/DPOTRS"] = leg
         else:
             skipped["direct_sr"] = "budget"
     ens.close()

@@ -135,9 +135,11 @@ int mvmc_ens_measure_configs(mvmc_ens* e, int count, const int* ele_idx, double*
  * status (iterations or -(iter+1)). */
 int mvmc_ens_sr_cg(mvmc_ens* e, long long* cg_info);
 /* Direct SR on the reduced operands (crates/mvmc-core/src/sr.rs:60-160,
- * stochastic_opt_real; assembly sr_backend.rs:484-516): Gram of the packed
- * active samples by rocBLAS DGEMM (matrix cores), S and g as the reference
- * assembles them, Cholesky by rocSOLVER DPOTRF/DPOTRS, parameter update and
+ * stochastic_opt_real; assembly sr_backend.rs:484-516): upper triangle of
+ * the Gram of the packed active samples by the library's MFMA kernel
+ * (FP64 MFMA; in the precision form, mvmc_ens_set_precision 1, the samples
+ * packed in FP32 and FP32 MFMA with FP32 accumulation), S and g as the
+ * reference assembles them, Cholesky by rocSOLVER DPOTRF/DPOTRS, parameter update and
  * gauge sync.  The path NSRCG 0 selects in mvmc_ens_opt_step; skipped gets
  * 1 when the factorisation fails or the solution is not finite (no update),
  * as the reference's cholesky_solve. */
@@ -217,7 +219,8 @@ int mvmc_ens_get_cg(mvmc_ens* e, double* solution, int* n_active);
  * panel-close batches; 1: two per CU, 16 columns; 2: register-resident,
  * one 1024-thread workgroup per CU, 64 < n <= 224) and the device ms per
  * rebuilt walker measured for variants 0, 1 and 2; device ms of the last
- * mvmc_ens_vjp_adjoint call and of its plane rebuilds; two reserved. */
+ * mvmc_ens_vjp_adjoint call and of its plane rebuilds; device ms of the
+ * last direct SR's Gram kernel; one reserved. */
 int mvmc_ens_timings(mvmc_ens* e, double* timings24);
 /* Device phase clocks of the sampling kernel, summed over walkers (ticks of
  * the 100 MHz constant clock): 0 candidate, 1 projection deltas + ratio,
@@ -292,9 +295,17 @@ int mvmc_sr_cg_apply(int na, int walkers, int nsample, const double* pack, const
 
 /* SrStages::gram_real (sr_backend.rs:219-226, C order c_order_gram_real
  * sr_backend.rs:305-333): out = O O^T, the full n x n column-major Gram of
- * the [n, samples] column-major store, one DGEMM('N','T') (rocBLAS on the
- * matrix cores).  Device pointers, asynchronous on `stream`. */
+ * the [n, samples] column-major store, by the library's FP64 MFMA Gram
+ * kernel (64 x 64 tiles of the upper triangle mirrored to the lower; 4 rows
+ * per MFMA, the MFMA partial sums in row order: not the serial sum of the
+ * reference's BLAS, agreement to rounding).  Device pointers, asynchronous
+ * on `stream`. */
 int mvmc_sr_gram_real(int n, int samples, const double* store, double* out, void* stream);
+/* Precision form of gram_real: the store in FP32, FP32 MFMA with FP32
+ * accumulation, the Gram written as doubles (the direct SR of the precision
+ * form uses the same kernel).  No reference counterpart (the reference's
+ * Gram is FP64 only). */
+int mvmc_sr_gram_real_f32(int n, int samples, const float* store, double* out, void* stream);
 
 /* SrStages::cholesky_solve (sr_backend.rs:243-247, crates/mvmc-core/src/sr.rs:535):
  * solves the symmetric positive-definite S x = rhs in place (rhs becomes x,

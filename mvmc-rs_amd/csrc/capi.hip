@@ -428,7 +428,7 @@ struct mvmc_ens {
   double* d_adj_wt = nullptr;
   // device times of the last calls (HIP events on the library's stream)
   double rebuilt = 0, meas_rebuild_ms = 0, measure_ms = 0, measured = 0, cg_ms = 0, cg_iter = 0, cg_na = 0,
-         cg_rows = 0, adj_ms = 0, adj_rebuild_ms = 0;
+         cg_rows = 0, adj_ms = 0, adj_rebuild_ms = 0, gram_ms = 0;
   ~mvmc_ens() {
     cg.release();
     if (d_S) (void)hipFree(d_S);
@@ -1080,7 +1080,7 @@ void apply_sr_update(mvmc_ens* E, const std::vector<int>& map, const std::vector
 // DPOTRS, upper) on g in place.  A failed factorisation or a non-finite
 // solution skips the update (return 1), as the reference's cholesky_solve.
 int direct_sr(mvmc_ens* E, const std::vector<int>& map, int na, size_t rows, double inv_w, double t0,
-              long long* info_out) {
+              long long* info_out, bool f32) {
   mvmc::Model& m = E->m;
   hipStream_t st = E->st;
   if (!E->blas) {
@@ -1099,11 +1099,20 @@ int direct_sr(mvmc_ens* E, const std::vector<int>& map, int na, size_t rows, dou
   HIPCHK(hipEventCreate(&c0));
   HIPCHK(hipEventCreate(&c1));
   HIPCHK(hipEventRecord(c0, st));
-  const double zero = 0.0;
-  // the full Gram by DGEMM (P P^T): twice the flops of DSYRK, but DGEMM is
-  // the matrix-core kernel the library tunes (measured ~9x faster here)
-  BLASCHK(rocblas_dgemm(E->blas, rocblas_operation_none, rocblas_operation_transpose, na, na, (rocblas_int)rows,
-                        &inv_w, (const double*)E->d_pack, na, (const double*)E->d_pack, na, &zero, E->d_S, na));
+  // upper triangle of the Gram inv_w P P^T on the matrix cores (FP64 MFMA;
+  // FP32 MFMA with FP32 accumulation in the precision form)
+  if (rows > (size_t)INT32_MAX) throw Fail(MVMC_E_INVALID_SHAPE, "too many sample rows for the Gram kernel");
+  const unsigned tiles = (unsigned)gm_tiles(na);
+  hipEvent_t g1;
+  HIPCHK(hipEventCreate(&g1));
+  if (f32)
+    hipLaunchKernelGGL(k_gram_mfma<float>, dim3(tiles), dim3(256), 0, st, (const float*)E->d_pack, (int)rows, na,
+                       inv_w, E->d_S, 0);
+  else
+    hipLaunchKernelGGL(k_gram_mfma<double>, dim3(tiles), dim3(256), 0, st, (const double*)E->d_pack, (int)rows, na,
+                       inv_w, E->d_S, 0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(g1, st));
   hipLaunchKernelGGL(k_sr_assemble, dim3(nblk((long long)nn)), dim3(256), 0, st, na, E->d_cmean,
                      1.0 + m.mp.dsr_opt_sta_del, E->d_S);
   HIPCHK(hipGetLastError());
@@ -1115,11 +1124,14 @@ int direct_sr(mvmc_ens* E, const std::vector<int>& map, int na, size_t rows, dou
   if (info == 0) BLASCHK(rocsolver_dpotrs(E->blas, rocblas_fill_upper, na, 1, E->d_S, na, E->d_cx, na));
   HIPCHK(hipEventRecord(c1, st));
   HIPCHK(hipEventSynchronize(c1));
-  float ms = 0;
+  float ms = 0, gms = 0;
   HIPCHK(hipEventElapsedTime(&ms, c0, c1));
+  HIPCHK(hipEventElapsedTime(&gms, c0, g1));
   (void)hipEventDestroy(c0);
   (void)hipEventDestroy(c1);
+  (void)hipEventDestroy(g1);
   E->cg_ms = ms;
+  E->gram_ms = gms;
   E->cg_iter = 0;
   E->cg_na = na;
   E->cg_rows = (double)rows;
@@ -1186,10 +1198,11 @@ int sr_impl(mvmc_ens* E, long long* info_out, bool force_direct = false) {
   HIPCHK(hipMemcpyAsync(E->d_cdiag, diag.data(), sizeof(double) * na, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(E->d_cg, g.data(), sizeof(double) * na, hipMemcpyHostToDevice, st));
   const size_t rows = (size_t)v.W * v.nsample;
-  // the precision form streams the CG operand in FP32 (accumulated in FP64);
-  // direct SR and the FP64 path pack doubles
+  // the precision form packs the samples in FP32: the CG operand is
+  // accumulated in FP64, the direct Gram in FP32 on the matrix cores; the
+  // FP64 path packs doubles
   const bool direct = m.mp.nsrcg == 0 || force_direct;
-  const bool f32 = E->precision == 1 && !direct;
+  const bool f32 = E->precision == 1;
   const size_t pbytes = rows * na * (f32 ? sizeof(float) : sizeof(double));
   if (E->pack_cap < pbytes) {
     if (E->d_pack) (void)hipFree(E->d_pack);
@@ -1205,7 +1218,7 @@ int sr_impl(mvmc_ens* E, long long* info_out, bool force_direct = false) {
                        (double*)E->d_pack);
   HIPCHK(hipGetLastError());
   const double inv_w = 1.0 / E->wc;
-  if (direct) return direct_sr(E, map, na, rows, inv_w, t0, info_out);
+  if (direct) return direct_sr(E, map, na, rows, inv_w, t0, info_out, f32);
   const int max_it = m.mp.nsr_opt_cg_max_iter > 0 ? m.mp.nsr_opt_cg_max_iter : na;
   hipEvent_t c0, c1;
   HIPCHK(hipEventCreate(&c0));
@@ -2097,7 +2110,7 @@ int mvmc_ens_timings(mvmc_ens* e, double* t) {
   t[17] = e->rb_walkers[0] > 0 ? e->rb_ms[0] / e->rb_walkers[0] : 0.0;
   t[18] = e->rb_walkers[1] > 0 ? e->rb_ms[1] / e->rb_walkers[1] : 0.0;
   t[19] = e->rb_walkers[2] > 0 ? e->rb_ms[2] / e->rb_walkers[2] : 0.0;
-  t[20] = e->adj_ms; t[21] = e->adj_rebuild_ms; t[22] = 0.0; t[23] = 0.0;
+  t[20] = e->adj_ms; t[21] = e->adj_rebuild_ms; t[22] = e->gram_ms; t[23] = 0.0;
   return MVMC_OK;
 }
 
@@ -2130,11 +2143,21 @@ int mvmc_sr_gram_real(int n, int samples, const double* store, double* out, void
     if (n < 0 || samples < 0) throw Fail(MVMC_E_INVALID_SHAPE, "bad Gram dimensions");
     if (n == 0) return;
     if (!out || (!store && samples > 0)) throw Fail(MVMC_E_INVALID_SHAPE, "null buffer");
-    rocblas_handle h = sr_blas();
-    BLASCHK(rocblas_set_stream(h, (hipStream_t)stream));
-    const double one = 1.0, zero = 0.0;
-    BLASCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, n, n, samples, &one, store, n,
-                          store, n, &zero, out, n));
+    hipLaunchKernelGGL(k_gram_mfma<double>, dim3((unsigned)gm_tiles(n)), dim3(256), 0, (hipStream_t)stream,
+                       store, samples, n, 1.0, out, 1);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int mvmc_sr_gram_real_f32(int n, int samples, const float* store, double* out, void* stream) {
+  return guarded([&] {
+    require_gpu();
+    if (n < 0 || samples < 0) throw Fail(MVMC_E_INVALID_SHAPE, "bad Gram dimensions");
+    if (n == 0) return;
+    if (!out || (!store && samples > 0)) throw Fail(MVMC_E_INVALID_SHAPE, "null buffer");
+    hipLaunchKernelGGL(k_gram_mfma<float>, dim3((unsigned)gm_tiles(n)), dim3(256), 0, (hipStream_t)stream,
+                       store, samples, n, 1.0, out, 1);
+    HIPCHK(hipGetLastError());
   });
 }
 

@@ -99,6 +99,113 @@ __global__ void __launch_bounds__(256) k_gram_complex(const double* store, int n
     }
 }
 
+// Gram of the stored samples on the matrix cores (COrderSr::gram_real,
+// sr_backend.rs:312-333, and the Gram of the direct solve, sr.rs:60-160):
+// G[i + j na] = scale * sum_r P[r][i] P[r][j] over a row-major [rows][na]
+// store.  One 256-thread workgroup per 64 x 64 tile of the upper triangle
+// (tiles (ti, tj), ti <= tj, enumerated column by column); each wave owns a
+// 32 x 32 quarter as 2 x 2 blocks of 16 x 16 accumulators; rows stream
+// through LDS GM_KC at a time and every block consumes them 4 per MFMA
+// (v_mfma_f64_16x16x4f64, or v_mfma_f32_16x16x4f32 with FP32 accumulation
+// for a float store).  MFMA operand layout (16 x 16 x 4, one block): lane l
+// supplies A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator element e of
+// lane l is D[row(l, e)][l % 16] with row = l / 16 + 4 e for the FP64 MFMA
+// and 4 (l / 16) + e for the FP32 one (both measured on the device against
+// a host Gram of small integer stores).  Diagonal tiles write both halves;
+// full != 0 mirrors every off-diagonal tile into the lower triangle.
+// Summation order: per MFMA 4 rows, MFMAs in row order (not a serial sum).
+typedef double gm_v4d __attribute__((ext_vector_type(4)));
+typedef float gm_v4f __attribute__((ext_vector_type(4)));
+template <class P>
+struct GramMfma;
+template <>
+struct GramMfma<double> {
+  using acc_t = gm_v4d;
+  static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int e) { return (lane >> 4) + 4 * e; }
+};
+template <>
+struct GramMfma<float> {
+  using acc_t = gm_v4f;
+  static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int e) { return 4 * (lane >> 4) + e; }
+};
+constexpr int GM_T = 64, GM_KC = 16;
+constexpr int GM_LD = 80;  // LDS row stride (elements): the 4 k-rows of one operand fetch fall in two bank halves
+
+__host__ __device__ inline long long gm_tiles(int na) {
+  const long long nt = (na + GM_T - 1) / GM_T;
+  return nt * (nt + 1) / 2;
+}
+
+template <class P>
+__global__ void __launch_bounds__(256) k_gram_mfma(const P* pack, int rows, int na, double scale, double* G,
+                                                   int full) {
+  using Op = GramMfma<P>;
+  using acc_t = typename Op::acc_t;
+  __shared__ P sA[GM_KC][GM_LD];
+  __shared__ P sB[GM_KC][GM_LD];
+  // tile (ti, tj), ti <= tj, from the column-by-column index t
+  const long long t = blockIdx.x;
+  int tj = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((long long)(tj + 1) * (tj + 2) / 2 <= t) ++tj;
+  while ((long long)tj * (tj + 1) / 2 > t) --tj;
+  const int ti = (int)(t - (long long)tj * (tj + 1) / 2);
+  const int i0 = ti * GM_T, j0 = tj * GM_T;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wi = wave >> 1, wj = wave & 1;
+  acc_t acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+  const int lr = tid >> 4, lc = (tid & 15) * 4;  // loader: row lr of the chunk, 4 columns from lc
+  for (int r0 = 0; r0 < rows; r0 += GM_KC) {
+    const int r = r0 + lr;
+    const bool rin = r < rows;
+    const P* row = pack + (size_t)(rin ? r : 0) * na;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int ci = i0 + lc + u, cj = j0 + lc + u;
+      sA[lr][lc + u] = (rin && ci < na) ? row[ci] : (P)0;
+      sB[lr][lc + u] = (rin && cj < na) ? row[cj] : (P)0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k0 = 0; k0 < GM_KC; k0 += 4) {
+      const int k = k0 + (lane >> 4), m = lane & 15;
+      P av[2], bv[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) av[a] = sA[k][32 * wi + 16 * a + m];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) bv[b] = sB[k][32 * wj + 16 * b + m];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = Op::op(av[a], bv[b], acc[a][b]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = i0 + 32 * wi + 16 * a + Op::row(lane, e);
+        const int j = j0 + 32 * wj + 16 * b + (lane & 15);
+        if (i < na && j < na) {
+          const double x = scale * (double)acc[a][b][e];
+          G[i + (size_t)j * na] = x;
+          if (full && ti != tj) G[j + (size_t)i * na] = x;
+        }
+      }
+}
+
 // sr_s_g, centered() (stage_backend.rs:167-184): thread p < np forms the
 // weighted mean of column p (k ascending) and writes d[., p] = o[., p] - mean;
 // thread np forms <E>_w and writes de.  o, d: column-major [ns][np].

@@ -94,7 +94,7 @@ EXPORTED_SYMBOLS = [
     "mvmc_ens_get_o_store", "mvmc_ens_get_walker", "mvmc_ens_get_tables", "mvmc_ens_get_config", "mvmc_ens_get_cg",
     "mvmc_ens_timings", "mvmc_ens_phase_clocks", "mvmc_ens_status", "mvmc_sr_cg_solve", "mvmc_ens_set_precision",
     "mvmc_ens_vjp", "mvmc_ens_vjp_adjoint", "mvmc_ens_log_psi", "mvmc_ens_sr_direct", "mvmc_pfinv_c128",
-    "mvmc_ens_measure_configs", "mvmc_sr_cg_apply", "mvmc_sr_gram_real", "mvmc_sr_cholesky_solve",
+    "mvmc_ens_measure_configs", "mvmc_sr_cg_apply", "mvmc_sr_gram_real", "mvmc_sr_gram_real_f32", "mvmc_sr_cholesky_solve",
     "mvmc_ens_write_opt_data", "mvmc_ens_set_output_dir",
 ]
 
@@ -315,7 +315,7 @@ class Ensemble:
                 "meas_rebuild_ms": t[9], "measure_ms": t[10], "measured": t[11], "cg_ms": t[12],
                 "cg_iter": t[13], "cg_na": t[14], "cg_rows": t[15], "rebuild_variant": int(t[16]),
                 "rebuild_ms_per_walker": (t[17], t[18], t[19]), "adjoint_vjp_ms": t[20],
-                "adjoint_vjp_rebuild_ms": t[21]}
+                "adjoint_vjp_rebuild_ms": t[21], "gram_ms": t[22]}
 
     PHASES = ("candidate", "projection", "staging", "ratio", "metropolis", "accept_pass", "rebuild", "exchange",
               "block_end")
@@ -438,6 +438,13 @@ def sr_gram_real(store_dev, out_dev, n, samples, stream=None):
     (torch CUDA tensors)."""
     _check(lib().mvmc_sr_gram_real(n, samples, ctypes.c_void_p(store_dev.data_ptr()),
                                    ctypes.c_void_p(out_dev.data_ptr()), ctypes.c_void_p(stream or 0)))
+
+
+def sr_gram_real_f32(store_dev, out_dev, n, samples, stream=None):
+    """Precision form of gram_real: float32 store, FP32-MFMA Gram with FP32
+    accumulation, written as float64."""
+    _check(lib().mvmc_sr_gram_real_f32(n, samples, ctypes.c_void_p(store_dev.data_ptr()),
+                                       ctypes.c_void_p(out_dev.data_ptr()), ctypes.c_void_p(stream or 0)))
 
 
 def sr_cholesky_solve(s_dev, rhs_dev, n, stream=None):

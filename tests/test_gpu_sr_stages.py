@@ -147,3 +147,74 @@ def test_direct_sr_step_matches_oracle(M, tmp_path):
         ens.close()
     finally:
         O.set_math(0)
+
+
+@pytest.mark.parametrize("n,samples", [(1, 5), (63, 17), (130, 1000), (200, 33)])
+def test_gram_mfma_shapes(M, n, samples):
+    """SrStages::gram_real on the MFMA Gram kernel over ragged shapes (one
+    partial tile, several tiles, rows not a multiple of the 16-row chunk):
+    every entry within 4 samples eps of sum_r |O_ri O_rj| of the exact Gram
+    (numpy in extended accumulation is within the same bound), both
+    triangles written and exactly symmetric."""
+    import torch
+    rng = np.random.default_rng(n * 1000 + samples)
+    store = rng.standard_normal((samples, n))
+    out = torch.full((n, n), 7.0, dtype=torch.float64, device="cuda")
+    M.sr_gram_real(_dev(store), out, n, samples)
+    torch.cuda.synchronize()
+    G = out.cpu().numpy().T
+    assert np.array_equal(G, G.T)
+    ref = store.T @ store
+    bound = 4 * samples * np.finfo(float).eps * (np.abs(store).T @ np.abs(store))
+    assert np.all(np.abs(G - ref) <= 2 * bound + 1e-300), np.abs(G - ref).max()
+
+
+def test_direct_sr_fp32_mfma_gram(M, tmp_path):
+    """Direct SR of one measurement twice: FP64 MFMA Gram, then (precision
+    form) samples packed in FP32 with the FP32-MFMA Gram; the FP32 update
+    within 1e-3 relative of the FP64 one (8x8 lattice, 512 samples, the
+    shift 1 + DSROptStaDel on the diagonal)."""
+    import lattice
+    info = lattice.write_workload("square8", str(tmp_path / "sq8"), sr_steps=1, warmup=4)
+    ens = M.Ensemble(info["namelist"], info["initial"], walkers=512)
+    ens.sample()
+    ens.measure()
+    p0 = ens.params()
+    assert ens.sr_direct() == 0
+    p64 = np.concatenate(ens.params()) - np.concatenate(p0)
+    ms64 = ens.timings()["cg_ms"]
+    ens.set_params(*p0)
+    ens.set_precision("mixed")
+    assert ens.sr_direct() == 0
+    p32 = np.concatenate(ens.params()) - np.concatenate(p0)
+    rel = np.linalg.norm(p32 - p64) / np.linalg.norm(p64)
+    log(f"direct SR square8 512 samples: FP32-MFMA update vs FP64 relative {rel:.2e} "
+        f"(FP64 {ms64:.2f} ms, FP32 {ens.timings()['cg_ms']:.2f} ms)")
+    assert 0 < rel <= 1e-3
+    ens.close()
+
+
+@pytest.mark.parametrize("n,samples", [(16, 4), (130, 1000), (200, 33)])
+def test_gram_mfma_f32(M, n, samples):
+    """Precision form of gram_real (FP32 store, FP32 MFMA, FP32
+    accumulation): every entry within 4 (samples + 2) eps32 of sum_r |O_ri
+    O_rj| of the exact Gram of the float32 store; small-integer stores
+    (every product and partial sum exact in FP32) give the exact Gram."""
+    import torch
+    rng = np.random.default_rng(n + samples)
+    ints = rng.integers(-3, 4, (samples, n)).astype(np.float32)
+    out = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    M.sr_gram_real_f32(_dev(ints, torch.float32), out, n, samples)
+    torch.cuda.synchronize()
+    G = out.cpu().numpy().T
+    ref = ints.astype(np.float64).T @ ints.astype(np.float64)
+    assert np.array_equal(G, ref)
+    store = rng.standard_normal((samples, n)).astype(np.float32)
+    M.sr_gram_real_f32(_dev(store, torch.float32), out, n, samples)
+    torch.cuda.synchronize()
+    G = out.cpu().numpy().T
+    s64 = store.astype(np.float64)
+    ref = s64.T @ s64
+    bound = 4 * (samples + 2) * np.finfo(np.float32).eps * (np.abs(s64).T @ np.abs(s64))
+    assert np.array_equal(G, G.T)
+    assert np.all(np.abs(G - ref) <= bound), np.abs(G - ref).max()
