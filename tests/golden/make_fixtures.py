@@ -18,6 +18,7 @@ Everything written here is data held by the reference's tests:
   * two-hop bilinear bits            tests/fixtures/pfaffian_cg/two_hop_bilinear{,_c}.txt
   * hopping candidate retry tapes    tests/fixtures/candidate/hopping_retry_cap_559.txt
   * zvo_out row layout               tests/fixtures/c_integration_reference/hubbard_chain_real/zvo_out_first10.dat
+  * ten C optimisation steps         tests/fixtures/c_integration_reference/heisenberg_chain_real/zvo_out_first10.dat
   * C SetFlagShift gates             tests/fixtures/optimization_flags/c_shift_flags.txt
 
 Usage: make_fixtures.py [--ref REFERENCE_ROOT] [fixture ...] (default: all).
@@ -210,6 +211,18 @@ def zvo_rows():
     os.chmod(os.path.join(OUT, "zvo_out_c_rows.dat"), 0o644)
 
 
+def heisenberg_opt_rows():
+    """C 1.4.0 zvo_out.dat rows of ten optimisation steps of the Heisenberg
+    chain (c_integration_reference/heisenberg_chain_real, seed 1, direct SR,
+    NStore 1, 100 samples; data file, as is) and the same model's Expert
+    inputs in optimisation mode (NVMCCalMode 0; every other key as given)."""
+    copy_inputs(os.path.join(FIX, "native_c_physcal_181/heisenberg_chain_real/inputs"),
+                os.path.join(OUT, "inputs/heisenberg_chain_opt"), {"NVMCCalMode": 0})
+    shutil.copy(os.path.join(FIX, "c_integration_reference", "heisenberg_chain_real", "zvo_out_first10.dat"),
+                os.path.join(OUT, "zvo_out_heisenberg_real_first10.dat"))
+    os.chmod(os.path.join(OUT, "zvo_out_heisenberg_real_first10.dat"), 0o644)
+
+
 def shift_flags():
     """C SetFlagShift eligibility records (data file, copied as is)."""
     shutil.copy(os.path.join(FIX, "optimization_flags", "c_shift_flags.txt"), os.path.join(OUT, "c_shift_flags.txt"))
@@ -232,8 +245,8 @@ def bilinear():
                         values=np.concatenate(values), bits=np.array(bits, dtype=np.uint64))
 
 
-ALL = [sfmt, hubbard, heisenberg, sr_cg, pfapack, pfapack_complex, candidate, zvo_rows, bilinear, shift_flags,
-       negative_stepdt]
+ALL = [sfmt, hubbard, heisenberg, sr_cg, pfapack, pfapack_complex, candidate, zvo_rows, heisenberg_opt_rows, bilinear,
+       shift_flags, negative_stepdt]
 
 if __name__ == "__main__":
     chosen = [f for f in ALL if not ARGS or f.__name__ in ARGS]
