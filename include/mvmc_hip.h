@@ -36,7 +36,12 @@ int mvmc_abi_version(void);
  * zero-pivot plane reports pf = 0 and a non-finite plane its computed pf
  * (lib.rs:12-15).  The results are BatchedEngine::run_f64's BatchOutput<f64>
  * = (pf, inv, status).  All pointers are device pointers; `stream` is a
- * hipStream_t (NULL = default stream).  Asynchronous on `stream`. */
+ * hipStream_t (NULL = default stream).  Asynchronous on `stream`.
+ * n <= 64 and 152 <= n <= 224: one plane per workgroup held in registers
+ * (256 threads for n <= 64, 1024 above); other n: the panel-delayed kernel
+ * (several planes per workgroup through a global work buffer), whichever
+ * measured faster at that size; MVMC_PFINV_RR=0 selects the latter for
+ * every n.  Both give the same bits. */
 int mvmc_pfinv_f64(const double* planes, int n, int count, double* pf, double* inv, int* status, void* stream);
 
 /* In-place form (crates/mvmc-core/src/stage_backend.rs:145-155): planes are

@@ -34,7 +34,7 @@ def build_library(defines=(), out=None):
     os.makedirs(os.path.dirname(out), exist_ok=True)
     srcs = [os.path.join(HERE, "csrc", "capi.hip"), os.path.join(HERE, "host", "model.cpp")]
     tmp = out + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++20", "-fPIC", "-shared", "-ffp-contract=off",
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
            "-Wno-unused-result"] + [f"-D{d}" for d in defines] + ["-o", tmp] + srcs + \
           ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]

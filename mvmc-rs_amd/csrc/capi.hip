@@ -282,55 +282,83 @@ double* noalloc_scratch() {
   return (double*)p;
 }
 
-// The register-resident Pfaffian + inverse (64 < n <= RR_NMAX) is on unless
-// MVMC_PFINV_RR=0 selects the panel-delayed kernels (same results).
-bool rr_enabled() {
-  static const bool on = [] {
-    const char* e = std::getenv("MVMC_PFINV_RR");
-    return !(e && e[0] == '0');
-  }();
-  return on;
+// The register-resident Pfaffian + inverse (full form n <= 64, panel form
+// 64 < n <= RR_NMAX) is a rebuild variant of the sampler (chosen by
+// measurement) and serves the batched entry points where it measured faster
+// than the panel-delayed kernel: n <= 64 and RR_BATCHED_MIN <= n <= RR_NMAX
+// (MI355X, random planes: 1.46-1.65x at n = 16..64; 0.81x at n = 96, 0.97x
+// at 144, 1.10x at 160, 1.18x at 176, 1.29x at 192 and 208, 1.36x at 224).  MVMC_PFINV_RR=0 selects the
+// panel-delayed kernels throughout, MVMC_PFINV_RR=2 the register-resident
+// ones for every 2 <= n <= RR_NMAX (same results either way).
+constexpr int RR_BATCHED_MIN = 152;
+int rr_mode() {
+  const char* e = std::getenv("MVMC_PFINV_RR");
+  return e ? std::atoi(e) : 1;
+}
+bool rr_enabled() { return rr_mode() != 0; }
+bool rr_batched(int n) {
+  if (!rr_enabled() || n < 2 || n > RR_NMAX) return false;
+  return rr_mode() == 2 || n <= RR_NFULL || n >= RR_BATCHED_MIN;
 }
 
-// Register blocks per thread column of the register-resident kernels.
-int rr_r(int n) { return (n + RR_T - 1) / RR_T; }
-
-template <int R>
-void rr_rebuild_limits_r(size_t lds) {
-  raise_lds_limit((const void*)k_rebuild_rr<R>, lds);
-  raise_lds_limit((const void*)k_meas_rebuild_rr<R>, lds);
+// Shape of the register-resident kernels for order n: the full form (16 x 16
+// threads, R = ceil(n / 16) <= 4) for n <= 64, the panel form (32 x 32,
+// R = ceil(n / 32) <= 7) above.
+struct RrCfg {
+  int t = 0, r = 0;
+  bool full = false;
+};
+RrCfg rr_cfg(int n) {
+  RrCfg c;
+  c.full = n <= RR_NFULL;
+  c.t = c.full ? RR_TS : RR_T;
+  c.r = (n + c.t - 1) / c.t;
+  return c;
 }
 
-void rr_rebuild_limits(int r, size_t lds) {
-  switch (r) {
-    case 3: rr_rebuild_limits_r<3>(lds); break;
-    case 4: rr_rebuild_limits_r<4>(lds); break;
-    case 5: rr_rebuild_limits_r<5>(lds); break;
-    case 6: rr_rebuild_limits_r<6>(lds); break;
-    case 7: rr_rebuild_limits_r<7>(lds); break;
-    default: throw Fail(MVMC_E_INVALID_SHAPE, "register-resident path covers 64 < n <= 224");
+// Calls f.template operator()<T, R, FULL>() for the configuration's
+// compiled instance.
+template <class F>
+void rr_dispatch(const RrCfg& c, F&& f) {
+  if (c.full) {
+    switch (c.r) {
+      case 1: f.template operator()<RR_TS, 1, true>(); return;
+      case 2: f.template operator()<RR_TS, 2, true>(); return;
+      case 3: f.template operator()<RR_TS, 3, true>(); return;
+      case 4: f.template operator()<RR_TS, 4, true>(); return;
+    }
+  } else {
+    switch (c.r) {
+      case 3: f.template operator()<RR_T, 3, false>(); return;
+      case 4: f.template operator()<RR_T, 4, false>(); return;
+      case 5: f.template operator()<RR_T, 5, false>(); return;
+      case 6: f.template operator()<RR_T, 6, false>(); return;
+      case 7: f.template operator()<RR_T, 7, false>(); return;
+    }
   }
+  throw Fail(MVMC_E_INVALID_SHAPE, "register-resident path covers 2 <= n <= 224");
+}
+
+// LDS limits of the rebuild kernels and their resident workgroups per CU.
+int rr_rebuild_prepare(const RrCfg& c, size_t lds) {
+  int occ = 1;
+  rr_dispatch(c, [&]<int T, int R, bool FULL>() {
+    raise_lds_limit((const void*)k_rebuild_rr<T, R, FULL>, lds);
+    raise_lds_limit((const void*)k_meas_rebuild_rr<T, R, FULL>, lds);
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_rebuild_rr<T, R, FULL>, T * T, lds));
+  });
+  return std::max(occ, 1);
 }
 
 // Periodic rebuild (s < 0) or measurement rebuild of saved sample s with the
-// register-resident kernels, one 1024-thread workgroup per slot.
-template <int R>
-void rr_launch_rebuild_r(int grid, size_t lds, hipStream_t st, const EnsView& v, int s) {
-  if (s < 0)
-    hipLaunchKernelGGL(k_rebuild_rr<R>, dim3(grid), dim3(RR_T * RR_T), lds, st, v);
-  else
-    hipLaunchKernelGGL(k_meas_rebuild_rr<R>, dim3(grid), dim3(RR_T * RR_T), lds, st, v, s);
-}
-
-void rr_launch_rebuild(int r, int grid, size_t lds, hipStream_t st, const EnsView& v, int s) {
-  switch (r) {
-    case 3: rr_launch_rebuild_r<3>(grid, lds, st, v, s); break;
-    case 4: rr_launch_rebuild_r<4>(grid, lds, st, v, s); break;
-    case 5: rr_launch_rebuild_r<5>(grid, lds, st, v, s); break;
-    case 6: rr_launch_rebuild_r<6>(grid, lds, st, v, s); break;
-    case 7: rr_launch_rebuild_r<7>(grid, lds, st, v, s); break;
-    default: throw Fail(MVMC_E_INVALID_SHAPE, "register-resident path covers 64 < n <= 224");
-  }
+// register-resident kernels, one T x T workgroup per slot.
+void rr_launch_rebuild(const RrCfg& c, int grid, size_t lds, hipStream_t st, const EnsView& v, int s) {
+  rr_dispatch(c, [&]<int T, int R, bool FULL>() {
+    if (s < 0)
+      hipLaunchKernelGGL((k_rebuild_rr<T, R, FULL>), dim3(grid), dim3(T * T), lds, st, v);
+    else
+      hipLaunchKernelGGL((k_meas_rebuild_rr<T, R, FULL>), dim3(grid), dim3(T * T), lds, st, v, s);
+  });
 }
 
 template <class T>
@@ -653,13 +681,13 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
   E->grid_rb[0] = E->grid_rebuild;
   E->grid_rb[1] = (int)std::min<long long>(walkers, (long long)occ_r1 * ncu);
   size_t work_per_slot = pfinv_work_doubles(v.n, v.nqp);
-  if (rr_enabled() && v.n > 64 && v.n <= RR_NMAX) {
+  if (rr_enabled() && v.n >= 2 && v.n <= RR_NMAX) {
     // ei, rs, pfn, then the register-resident carve; 2 n^2 doubles of work
     E->lds_rr = 2 * rnd16((size_t)v.n * 4) + rnd16((size_t)v.nqp * 8) + rr_lds_bytes(v.n);
     if (E->lds_rr <= (size_t)maxlds) {
-      rr_rebuild_limits(rr_r(v.n), E->lds_rr);
+      const int occ = rr_rebuild_prepare(rr_cfg(v.n), E->lds_rr);
       E->rb_nvar = 3;
-      E->grid_rb[2] = (int)std::min<long long>(walkers, ncu);
+      E->grid_rb[2] = (int)std::min<long long>(walkers, (long long)occ * ncu);
       work_per_slot = std::max(work_per_slot, (size_t)2 * v.n * v.n);
     }
   }
@@ -701,7 +729,7 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
 
 void launch_rebuild(mvmc_ens* E, int var) {
   if (var == 2)
-    rr_launch_rebuild(rr_r(E->v.n), E->grid_rb[2], E->lds_rr, E->st, E->v, -1);
+    rr_launch_rebuild(rr_cfg(E->v.n), E->grid_rb[2], E->lds_rr, E->st, E->v, -1);
   else if (var == 1)
     hipLaunchKernelGGL((k_rebuild<2, 16>), dim3(E->grid_rb[1]), dim3(256), E->lds_rebuild, E->st, E->v);
   else
@@ -711,7 +739,7 @@ void launch_rebuild(mvmc_ens* E, int var) {
 
 void launch_meas_rebuild(mvmc_ens* E, int s) {
   if (E->rb_variant == 2)
-    rr_launch_rebuild(rr_r(E->v.n), E->grid_rb[2], E->lds_rr, E->st, E->v, s);
+    rr_launch_rebuild(rr_cfg(E->v.n), E->grid_rb[2], E->lds_rr, E->st, E->v, s);
   else if (E->rb_variant == 1)
     hipLaunchKernelGGL((k_meas_rebuild<2, 16, false>), dim3(E->grid_rb[1]), dim3(256), E->lds_rebuild, E->st, E->v,
                        GroupArgs{}, s);
@@ -1321,31 +1349,23 @@ struct GroupPart {
   std::vector<double> weight; // per-run work estimate the ranges are cut by
 };
 
-template <int R>
-void launch_pfinv_rr_r(const double* in, double* out, int n, int count, double* gs, int grid, double* pf, int* status,
-                       hipStream_t st) {
-  const size_t lds = rr_lds_bytes(n);
-  raise_lds_limit((const void*)k_pfinv_rr<R>, lds);
-  hipLaunchKernelGGL(k_pfinv_rr<R>, dim3(grid), dim3(RR_T * RR_T), lds, st, in, out, n, count, gs, pf, status);
-}
-
 void launch_pfinv_rr(const double* in, double* out, int n, int count, double* pf, int* status, hipStream_t st) {
   int dev = 0, ncu = 0;
   HIPCHK(hipGetDevice(&dev));
   HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  const int grid = std::min(count, std::max(ncu, 1));
-  double* gs = nullptr;
-  HIPCHK(hipMallocAsync((void**)&gs, sizeof(double) * 2 * (size_t)n * n * grid, st));
-  switch ((n + RR_T - 1) / RR_T) {
-    case 3: launch_pfinv_rr_r<3>(in, out, n, count, gs, grid, pf, status, st); break;
-    case 4: launch_pfinv_rr_r<4>(in, out, n, count, gs, grid, pf, status, st); break;
-    case 5: launch_pfinv_rr_r<5>(in, out, n, count, gs, grid, pf, status, st); break;
-    case 6: launch_pfinv_rr_r<6>(in, out, n, count, gs, grid, pf, status, st); break;
-    case 7: launch_pfinv_rr_r<7>(in, out, n, count, gs, grid, pf, status, st); break;
-    default: throw Fail(MVMC_E_INVALID_SHAPE, "register-resident path covers 64 < n <= 224");
-  }
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipFreeAsync(gs, st));
+  const RrCfg c = rr_cfg(n);
+  const size_t lds = rr_lds_bytes(n);
+  rr_dispatch(c, [&]<int T, int R, bool FULL>() {
+    raise_lds_limit((const void*)k_pfinv_rr<T, R, FULL>, lds);
+    int occ = 1;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_pfinv_rr<T, R, FULL>, T * T, lds));
+    const int grid = (int)std::min<long long>(count, (long long)std::max(occ, 1) * std::max(ncu, 1));
+    double* gs = nullptr;
+    HIPCHK(hipMallocAsync((void**)&gs, sizeof(double) * 2 * (size_t)n * n * grid, st));
+    hipLaunchKernelGGL((k_pfinv_rr<T, R, FULL>), dim3(grid), dim3(T * T), lds, st, in, out, n, count, gs, pf, status);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipFreeAsync(gs, st));
+  });
 }
 
 }  // namespace
@@ -1645,7 +1665,7 @@ int mvmc_pfinv_f64(const double* planes, int n, int count, double* pf, double* i
     if (count == 0) return;
     if (!planes || !pf || !inv || !status) throw Fail(MVMC_E_INVALID_SHAPE, "null buffer");
     hipStream_t st = (hipStream_t)stream;
-    if (rr_enabled() && n > 64 && n <= RR_NMAX) {
+    if (rr_batched(n)) {
       launch_pfinv_rr(planes, inv, n, count, pf, status, st);
       return;
     }

@@ -2187,13 +2187,14 @@ __global__ void k_cg_dir(int na, const double* sc, const int* ctl, const double*
 // Batched Pfaffian + inverse (BatchedEngine): planes [count][n][n]
 // column-major; one workgroup per plane batch of np planes.
 // ---------------------------------------------------------------------------
-// Batched Pfaffian + inverse of 64 < n <= RR_NMAX planes, one plane per
-// 1024-thread workgroup at a time (planes blockIdx.x, + gridDim.x, ...),
-// register-resident (pfinv_rr.h); gs: 2 n^2 doubles of scratch per
-// workgroup.  Results and statuses as k_pfinv_batched; in == out works in
-// place (the plane is read whole before it is written).
-template <int R>
-__global__ void __launch_bounds__(1024, 1) k_pfinv_rr(const double* in, double* out, int n, int count, double* gs_base,
+// Batched Pfaffian + inverse of n <= RR_NMAX planes, one plane per T x T
+// workgroup at a time (planes blockIdx.x, + gridDim.x, ...), register-
+// resident (pfinv_rr.h: the panel form, T = 32, for n > 64; the full form,
+// T = 16, for n <= 64); gs: 2 n^2 doubles of scratch per workgroup.
+// Results and statuses as k_pfinv_batched; in == out works in place (the
+// plane is read whole before it is written).
+template <int T, int R, bool FULL>
+__global__ void __launch_bounds__(T * T) k_pfinv_rr(const double* in, double* out, int n, int count, double* gs_base,
                                                    double* pf, int* status) {
   extern __shared__ __align__(16) char lds[];
   Carver c{lds};
@@ -2205,7 +2206,7 @@ __global__ void __launch_bounds__(1024, 1) k_pfinv_rr(const double* in, double* 
     double pfv = 0.0;
     int info = 0;
     bool az = false;
-    rr_pfinv<R>([&](int i, int j) { return A[i + (size_t)j * n]; }, n, O, 1.0, gs, L, pfv, info, false, az);
+    rr_pfinv<T, R, FULL>([&](int i, int j) { return A[i + (size_t)j * n]; }, n, O, 1.0, gs, L, pfv, info, false, az);
     if (threadIdx.x == 0) {
       status[p] = info;
       pf[p] = info > 0 ? 0.0 : pfv;
@@ -2217,12 +2218,12 @@ __global__ void __launch_bounds__(1024, 1) k_pfinv_rr(const double* in, double* 
 }
 
 // Periodic rebuild (engine.rs:454-473) of the handed-over walkers with the
-// register-resident Pfaffian + inverse, one walker per 1024-thread
-// workgroup, its NQP planes in turn, gathered straight from the Slater table
+// register-resident Pfaffian + inverse, one walker per T x T workgroup, its
+// NQP planes in turn, gathered straight from the Slater table
 // (A(r, c) = -S[rs_c][rs_r], pfaffian.rs:1107-1188).  A plane that is all
 // zero or fails stops the walker's rebuild (its Pfaffians are then not
 // replaced; the accept window restarts either way).
-template <int R>
+template <int T, int R, bool FULL>
 __device__ __forceinline__ int rr_calc_m_all(const EnsView& v, const int* ei, int* rs, double* planes, double* gs,
                                              RrLds& L, double* pfn) {
   const int tid = threadIdx.x;
@@ -2246,8 +2247,8 @@ __device__ __forceinline__ int rr_calc_m_all(const EnsView& v, const int* ei, in
     double pfv = 0.0;
     int info = 0;
     bool az = false;
-    rr_pfinv<R>([&](int i, int j) { return -S[(size_t)rs[j] * n2 + rs[i]]; }, n, planes + (size_t)q * n * n, -1.0,
-                gs, L, pfv, info, true, az);
+    rr_pfinv<T, R, FULL>([&](int i, int j) { return -S[(size_t)rs[j] * n2 + rs[i]]; }, n,
+                         planes + (size_t)q * n * n, -1.0, gs, L, pfv, info, true, az);
     if (az || info != 0) return 1;
     if (tid == 0) pfn[q] = pfv;
   }
@@ -2255,8 +2256,8 @@ __device__ __forceinline__ int rr_calc_m_all(const EnsView& v, const int* ei, in
   return 0;
 }
 
-template <int R>
-__global__ void __launch_bounds__(1024, 1) k_rebuild_rr(EnsView v) {
+template <int T, int R, bool FULL>
+__global__ void __launch_bounds__(T * T) k_rebuild_rr(EnsView v) {
   extern __shared__ __align__(16) char lds[];
   Carver c{lds};
   int* ei = c.take<int>(v.n);
@@ -2278,7 +2279,7 @@ __global__ void __launch_bounds__(1024, 1) k_rebuild_rr(EnsView v) {
     for (int i = tid; i < v.n; i += blockDim.x) ei[i] = v.ele_idx[(size_t)w * v.n + i];
     __syncthreads();
     double* planes = v.inv + (size_t)w * v.nqp * v.n * v.n;
-    const int bad = rr_calc_m_all<R>(v, ei, rs, planes, gs, L, pfn);
+    const int bad = rr_calc_m_all<T, R, FULL>(v, ei, rs, planes, gs, L, pfn);
     copy_inv32(v, w);
     if (tid == 0) {
       if (!bad) {
@@ -2294,8 +2295,9 @@ __global__ void __launch_bounds__(1024, 1) k_rebuild_rr(EnsView v) {
 
 // Measurement rebuild of saved sample s (as k_meas_rebuild) with the
 // register-resident Pfaffian + inverse, walkers blockIdx.x, + gridDim.x, ...
-template <int R>
-__global__ void __launch_bounds__(1024, 1) k_meas_rebuild_rr(EnsView v, int s) {
+// (T x T workgroups)
+template <int T, int R, bool FULL>
+__global__ void __launch_bounds__(T * T) k_meas_rebuild_rr(EnsView v, int s) {
   extern __shared__ __align__(16) char lds[];
   Carver c{lds};
   int* ei = c.take<int>(v.n);
@@ -2318,7 +2320,7 @@ __global__ void __launch_bounds__(1024, 1) k_meas_rebuild_rr(EnsView v, int s) {
     int valid = flag;
     if (valid) {
       double* planes = v.inv + (size_t)w * v.nqp * v.n * v.n;
-      valid = !rr_calc_m_all<R>(v, ei, rs, planes, gs, L, pfn);
+      valid = !rr_calc_m_all<T, R, FULL>(v, ei, rs, planes, gs, L, pfn);
     }
     if (tid == 0) {
       v.s_valid[ws] = valid;

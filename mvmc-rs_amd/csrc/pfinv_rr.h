@@ -1,21 +1,24 @@
 // Register-resident skew-symmetric Pfaffian + inverse of one plane per
-// workgroup, for 64 < n <= RR_NMAX: the C PfaPack order of dsktf2
+// workgroup, for n <= RR_NMAX: the C PfaPack order of dsktf2
 // (crates/pfapack/src/ltl.rs:61-191, :683-705), utu2pfa (utu2.rs:52-93)
-// and the panel path of utu2inv (utu2.rs:338-513 with the scalar trtri /
-// trmm forms of backend.rs:233-300), every element receiving exactly the
-// operation sequence of that order (bit-identical to the sequential loops;
-// built with -ffp-contract=off).
+// and utu2inv (utu2.rs:338-513 with the scalar trtri / trmm forms of
+// backend.rs:233-300), every element receiving exactly the operation
+// sequence of that order (bit-identical to the sequential loops; built with
+// -ffp-contract=off).
 //
 // Layout: T x T threads, thread (a, b) = (tid % T, tid / T) owns the
-// elements (i, j) = (a + T ri, b + T cj) of the plane.  Only the strict upper
-// triangle is ever needed (the elimination reads it, the inverse's panel
-// path forms the upper triangle and fills the lower by skew symmetry), and
-// (i, j) with cj > ri is always upper, cj < ri never, cj == ri iff b > a: a
-// thread keeps the R(R+1)/2 slots cj >= ri in registers.  Steps exchange
-// rows and columns through LDS vectors; the trtri result M and the solved C
-// go to a per-workgroup global scratch (row-major) from which the product
-// reads one row of each per step.  The matrix is read once (gathered from
-// the source) and the inverse written once.
+// elements (i, j) = (a + T ri, b + T cj) of the plane.  Panel form (n > 64,
+// T = 32): only the strict upper triangle is ever needed (the elimination
+// reads it, the inverse's panel path forms the upper triangle and fills the
+// lower by skew symmetry), and (i, j) with cj > ri is always upper, cj < ri
+// never, cj == ri iff b > a: a thread keeps the R(R+1)/2 slots cj >= ri in
+// registers.  Full form (n <= 64, T = 16, the whole plane held on chip):
+// utu2inv's small-matrix path multiplies the whole solved matrix, so a
+// thread keeps all R^2 slots (the elimination still uses the upper ones).
+// Steps exchange rows and columns through LDS vectors; the trtri result M
+// and the solved C go to a per-workgroup global scratch (row-major) from
+// which the product reads one row of each per step.  The matrix is read once
+// (gathered from the source) and the inverse written once.
 #pragma once
 #include "common.h"
 
@@ -23,15 +26,17 @@
 
 namespace mvmc_hip {
 
-constexpr int RR_T = 32;      // threads per side (1024 per workgroup)
+constexpr int RR_T = 32;      // threads per side of the panel form (1024 per workgroup)
+constexpr int RR_TS = 16;     // threads per side of the full form (256 per workgroup)
 constexpr int RR_NMAX = 224;  // largest n (R = 7 slots per side)
+constexpr int RR_NFULL = 64;  // largest n of the full form (utu2inv's small-matrix path)
 
 // LDS of one plane: 9 vectors of n doubles, 2 of n ints, scalars.
 struct RrLds {
   double *v0, *v1, *v2, *v3, *buf0, *buf1, *buf2, *buf3, *sup;  // sup: the superdiagonal A(i, i+1) after the LTL
   int *piv, *perm;
   int* si;     // [8] ints: 0 kp, 1 zero pivot, 2 info
-  double* sd;  // [8] doubles: 0 alpha, 1 max |a|^2 per wave (reduced), ...
+  double* sd;  // [32] doubles: 0, 1 alpha by step parity, 1 the Pfaffian, 16.. max |a|^2 per wave
 };
 
 __host__ __device__ inline size_t rr_lds_bytes(int n) {
@@ -57,10 +62,11 @@ __device__ inline RrLds carve_rr(Carver& c, int n) {
   return L;
 }
 
-// Slot of (ri, cj), cj >= ri, in the packed upper block triangle.
-template <int R>
+// Slot of block (ri, cj): packed upper block triangle (cj >= ri) in the
+// panel form, row-major R x R in the full form.
+template <int R, bool FULL>
 __host__ __device__ constexpr int rr_slot(int ri, int cj) {
-  return ri * R - ri * (ri - 1) / 2 + (cj - ri);
+  return FULL ? ri * R + cj : ri * R - ri * (ri - 1) / 2 + (cj - ri);
 }
 
 // C IDAMAX order of the pivot search: first index of the largest |x|; a NaN
@@ -76,12 +82,12 @@ __device__ inline double rr_key(double v, int i) {
 // (column-major, element (r, c) at r + c n) holds sign * A^{-1}.  allzero
 // (when check_zero) reports max |a|^2 < 1e-28 over the whole plane, in which
 // case nothing else is done.  gs: 2 n^2 doubles of global scratch.  Every
-// thread of the T x T workgroup (blockDim.x == RR_T * RR_T) calls it.
-template <int R, class Src>
+// thread of the T x T workgroup (blockDim.x == T * T) calls it.  FULL: the
+// full form (n <= 64), else the panel form (n > 64).
+template <int T, int R, bool FULL, class Src>
 __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sign, double* gs, RrLds& L, double& pf,
                                          int& info, bool check_zero, bool& allzero) {
-  constexpr int T = RR_T;
-  constexpr int NS = R * (R + 1) / 2;
+  constexpr int NS = FULL ? R * R : R * (R + 1) / 2;
   const int tid = threadIdx.x, NT = T * T;
   const int a0 = tid % T, b0 = tid / T;
   // per-step copies of (a, b) the compiler cannot prove loop-invariant: the
@@ -131,16 +137,22 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
         const double y = src(j, i);
         mx = fmax(mx, fmax(x * x, y * y));
       }
-      c[rr_slot<R>(ri, cj)] = x;
+      c[rr_slot<R, FULL>(ri, cj)] = x;
     }
+  if (FULL) {
+#pragma unroll
+    for (int ri = 1; ri < R; ++ri)
+#pragma unroll
+      for (int cj = 0; cj < ri; ++cj) c[rr_slot<R, FULL>(ri, cj)] = 0.0;
+  }
   for (int i = tid; i < n; i += NT) L.piv[i] = i + 1;
   if (tid == 0) L.si[2] = 0;
   if (check_zero) {
     for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
-    if ((tid & 63) == 0) L.buf0[tid >> 6] = mx;
+    if ((tid & 63) == 0) L.sd[16 + (tid >> 6)] = mx;
     __syncthreads();
     double m = 0.0;
-    for (int k = 0; k < NT / 64; ++k) m = fmax(m, L.buf0[k]);
+    for (int k = 0; k < NT / 64; ++k) m = fmax(m, L.sd[16 + k]);
     allzero = m < 1.0e-28;
     __syncthreads();
     if (allzero) return;
@@ -159,7 +171,7 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
     for (int cj = 0; cj < R; ++cj)
       if (cj == kc) {
 #pragma unroll
-        for (int ri = 0; ri <= cj; ++ri) f(a + T * ri, c[rr_slot<R>(ri, cj)]);
+        for (int ri = 0; ri <= cj; ++ri) f(a + T * ri, c[rr_slot<R, FULL>(ri, cj)]);
       }
   };
   auto for_row = [&](int row, auto f) {
@@ -169,7 +181,7 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
     for (int ri = 0; ri < R; ++ri)
       if (ri == kr) {
 #pragma unroll
-        for (int cj = ri; cj < R; ++cj) f(b + T * cj, c[rr_slot<R>(ri, cj)]);
+        for (int cj = ri; cj < R; ++cj) f(b + T * cj, c[rr_slot<R, FULL>(ri, cj)]);
       }
   };
 
@@ -193,7 +205,7 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
         for (int ri = 0; ri <= cj; ++ri) {
           const int i = a + T * ri;
           if (i <= kk) {
-            const double key = rr_key(fabs(c[rr_slot<R>(ri, cj)]), i);
+            const double key = rr_key(fabs(c[rr_slot<R, FULL>(ri, cj)]), i);
             if (key > best) { best = key; bi = i; }
           }
         }
@@ -277,7 +289,7 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
 #pragma unroll
             for (int ri = 0; ri <= cj; ++ri) {
               const int i = a + T * ri;
-              double& x = c[rr_slot<R>(ri, cj)];
+              double& x = c[rr_slot<R, FULL>(ri, cj)];
               if (i < j) x = (x + xs[i] * t1) - ys[i] * t2;
               else if (i == j) x = 0.0;
             }
@@ -302,7 +314,7 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
 #pragma unroll
     for (int cj = ri; cj < R; ++cj) {
       const int i = a + T * ri, j = b + T * cj;
-      if (j == i + 1 && j < n) L.sup[i] = c[rr_slot<R>(ri, cj)];
+      if (j == i + 1 && j < n) L.sup[i] = c[rr_slot<R, FULL>(ri, cj)];
     }
   __syncthreads();
   if (tid == 0) {
@@ -322,7 +334,8 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
   }
 
   RR_PHASE(2);
-  // ---- utu2inv, panel path (n > 64) ----
+  // ---- utu2inv (panel path for n > 64; the full form differs from the
+  // solve on) ----
   // trtri('U','U') on B(i, j) = A(i, j+1), 0 <= i < j <= n-2: at step k the
   // column k of the inverse is final (Bn(i, k) = -acc(i, k), i < k) and
   // every acc(i, j), i < k < j, adds Bn(i, k) B(k, j) with row k of B still
@@ -349,7 +362,7 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
             if (T * cj + T > k + 2) {
               const int i = a + T * ri, jj = b + T * cj;
               if (i < k && jj >= k + 2 && jj < n) {
-                double& x = c[rr_slot<R>(ri, cj)];
+                double& x = c[rr_slot<R, FULL>(ri, cj)];
                 x = x + U[i] * Bk[jj - 1];
               }
             }
@@ -369,21 +382,21 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
 #pragma unroll
     for (int cj = ri; cj < R; ++cj) {
       const int i = a + T * ri, jj = b + T * cj;
-      if (i + 1 < jj && jj < n) gm[(size_t)i * n + (jj - 1)] = c[rr_slot<R>(ri, cj)];
+      if (i + 1 < jj && jj < n) gm[(size_t)i * n + (jj - 1)] = c[rr_slot<R, FULL>(ri, cj)];
     }
   __syncthreads();
   RR_PHASE(4);
   // skew-tridiagonal solve T C = M, all columns at once, row by row through
   // LDS (vt[i] = -A(i, i+1) = -sup[i]); every row of C goes to gc row-major
-  // and the strict upper part into the registers.
+  // and into the registers (panel form: the strict upper part).
   auto keep_row = [&](int r, const double* row) {
 #pragma unroll
     for (int ri = 0; ri < R; ++ri)
       if (a + T * ri == r) {
 #pragma unroll
-        for (int cj = ri; cj < R; ++cj) {
+        for (int cj = FULL ? 0 : ri; cj < R; ++cj) {
           const int j = b + T * cj;
-          if (r < j && j < n) c[rr_slot<R>(ri, cj)] = row[j];
+          if ((FULL || r < j) && j < n) c[rr_slot<R, FULL>(ri, cj)] = row[j];
         }
       }
   };
@@ -452,9 +465,12 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
   }
   __syncthreads();
   RR_PHASE(5);
-  // panel product, upper part: C'(i, j) = C(i, j) + sum_{k < i} M(k, i) C(k, j)
-  // (k ascending, C as solved): right-looking over k with rows k of M and C
-  // from the scratch
+  // product C'(i, j) = C(i, j) + sum_{k < i} M(k, i) C(k, j) (k ascending,
+  // C as solved): right-looking over k with rows k of M and C from the
+  // scratch.  Panel form: the upper part (the lower follows by skew
+  // symmetry).  Full form: every element (the small-matrix path multiplies
+  // the column-permuted C; a column's result depends on that column alone,
+  // so the permutation moves to the output gather).
   {
     // rows k of M and C, loaded two steps ahead by thread j < n
     const int j = tid;
@@ -480,10 +496,10 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
       for (int ri = 0; ri < R; ++ri)
         if (T * ri + T > k + 1) {
 #pragma unroll
-          for (int cj = ri; cj < R; ++cj) {
+          for (int cj = FULL ? 0 : ri; cj < R; ++cj) {
             const int i = a + T * ri, jj = b + T * cj;
-            if (k < i && i < jj && jj < n) {
-              double& x = c[rr_slot<R>(ri, cj)];
+            if (k < i && (FULL || i < jj) && jj < n && i < n) {
+              double& x = c[rr_slot<R, FULL>(ri, cj)];
               x = x + Mr[i] * Cr[jj];
             }
           }
@@ -501,21 +517,24 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
       if (t != j) { const int x = L.perm[j]; L.perm[j] = L.perm[t]; L.perm[t] = x; }
     }
   }
-  // C' upper to the scratch (row-major), then the output gather: out(r, c) =
-  // sign * F(perm[r], perm[c]), F the skew completion of C' (0 on the diagonal)
+  // C' to the scratch (row-major: panel form the upper part, full form
+  // all), then the output gather: out(r, c) = sign * F(perm[r], perm[c]),
+  // F = C' (full form) or the skew completion of C' (panel form, 0 on the
+  // diagonal)
 #pragma unroll
   for (int ri = 0; ri < R; ++ri)
 #pragma unroll
-    for (int cj = ri; cj < R; ++cj) {
+    for (int cj = FULL ? 0 : ri; cj < R; ++cj) {
       const int i = a + T * ri, j = b + T * cj;
-      if (i < j && j < n) gc[(size_t)i * n + j] = c[rr_slot<R>(ri, cj)];
+      if ((FULL || i < j) && i < n && j < n) gc[(size_t)i * n + j] = c[rr_slot<R, FULL>(ri, cj)];
     }
   __syncthreads();
   for (int e = tid; e < n * n; e += NT) {
     const int r = e % n, cc = e / n;
     const int u = L.perm[r], v = L.perm[cc];
     double x = 0.0;
-    if (u < v) x = gc[(size_t)u * n + v];
+    if (FULL) x = gc[(size_t)u * n + v];
+    else if (u < v) x = gc[(size_t)u * n + v];
     else if (u > v) x = -gc[(size_t)v * n + u];
     out[e] = sign != 1.0 ? -x : x;
   }
