@@ -151,6 +151,16 @@ void cg_apply_dev(int na, int rows, const P* pack, const double* mean, const dou
                      g, out, wk.ctl);
 }
 
+// The same product from the explicit Gram G = O^T O (FP32, full symmetric
+// na x na; the precision form).
+void cg_apply_gram(int na, const float* G, const double* mean, const double* diag, const double* xv,
+                   const double* g, double* out, double inv_w, double shift, CgWork& wk, hipStream_t st) {
+  double* coef = wk.sc + 5;
+  hipLaunchKernelGGL(k_gram_coef, dim3(1), dim3(256), 0, st, mean, xv, na, coef, wk.ctl);
+  hipLaunchKernelGGL(k_gram_apply, dim3((na + 3) / 4), dim3(256), 0, st, G, na, xv, mean, diag, coef, inv_w, shift,
+                     g, out, wk.ctl);
+}
+
 // C 1.4.0 StochasticOptCG_Main / PreconCG_DiagScale (sr_cg.rs:700-880).
 // Iterations run on the device as a HIP graph of one residual-refresh period
 // (CG_PERIOD iterations, the refresh S x at its last one), captured once per
@@ -163,7 +173,7 @@ constexpr int CG_PERIOD = 20;
 template <class P>
 long long cg_solve_dev(int na, int W, int NS, const P* pack, const double* mean, const double* diag,
                        const double* g, double inv_w, double shift, double tol, int max_it, bool diag_scale,
-                       double* x_out, CgWork& wk, hipStream_t st_in) {
+                       double* x_out, CgWork& wk, hipStream_t st_in, const float* gram = nullptr) {
   const int rows = W * NS;
   wk.ensure(na, rows);
   // graph capture needs a created stream: the null stream's work is ordered
@@ -174,12 +184,17 @@ long long cg_solve_dev(int na, int W, int NS, const P* pack, const double* mean,
   const double threshold = tol * tol * (double)na * (double)na;
   const double pfac = 1.0 + shift;
   const dim3 blk(256);
+  // S x: one pass over the packed samples, or (gram set) one over their Gram
+  auto apply = [&](const double* xv, const double* gv, double* out) {
+    if (gram) cg_apply_gram(na, gram, mean, diag, xv, gv, out, inv_w, shift, wk, st);
+    else cg_apply_dev(na, rows, pack, mean, diag, xv, gv, out, inv_w, shift, wk, st);
+  };
   HIPCHK(hipMemsetAsync(wk.ctl, 0, sizeof(int) * 4, st));
   HIPCHK(hipMemcpyAsync(wk.g, g, sizeof(double) * na, hipMemcpyDeviceToDevice, st));
   hipLaunchKernelGGL(k_fill, dim3(nblk(na)), blk, 0, st, wk.x, na, 0.0);
   HIPCHK(hipMemcpyAsync(wk.r, g, sizeof(double) * na, hipMemcpyDeviceToDevice, st));
   if (diag_scale)
-    cg_apply_dev(na, rows, pack, mean, diag, wk.x, wk.g, wk.r, inv_w, shift, wk, st);
+    apply(wk.x, wk.g, wk.r);
   else
     HIPCHK(hipMemcpyAsync(wk.d, g, sizeof(double) * na, hipMemcpyDeviceToDevice, st));
   hipLaunchKernelGGL(k_cg_init, dim3(1), blk, 0, st, na, wk.r, wk.d, diag, pfac, diag_scale ? 1 : 0, wk.sc, wk.ctl);
@@ -188,12 +203,12 @@ long long cg_solve_dev(int na, int W, int NS, const P* pack, const double* mean,
   auto issue_period = [&]() {
     for (int j = 0; j < CG_PERIOD; ++j) {
       hipLaunchKernelGGL(k_cg_head, dim3(1), dim3(64), 0, st, j, max_it, threshold, wk.sc, wk.ctl);
-      cg_apply_dev(na, rows, pack, mean, diag, wk.d, nullptr, wk.q, inv_w, shift, wk, st);
+      apply(wk.d, nullptr, wk.q);
       hipLaunchKernelGGL(k_cg_alpha, dim3(1), blk, 0, st, j, na, wk.d, wk.q, wk.sc, wk.ctl);
       const int refresh = (j == CG_PERIOD - 1);  // (iter + 1) % 20 == 0 with periods starting at multiples of 20
       hipLaunchKernelGGL(k_cg_update, dim3(nblk(na)), blk, 0, st, na, wk.sc, wk.ctl, wk.d, wk.x, wk.q, wk.r,
                          refresh);
-      if (refresh) cg_apply_dev(na, rows, pack, mean, diag, wk.x, wk.g, wk.r, inv_w, shift, wk, st);
+      if (refresh) apply(wk.x, wk.g, wk.r);
       hipLaunchKernelGGL(k_cg_beta, dim3(1), blk, 0, st, j, na, wk.r, wk.q, diag, pfac, diag_scale ? 1 : 0, wk.sc,
                          wk.ctl);
       hipLaunchKernelGGL(k_cg_dir, dim3(nblk(na)), blk, 0, st, na, wk.sc, wk.ctl, diag_scale ? wk.q : wk.r, wk.d);
@@ -443,6 +458,9 @@ struct mvmc_ens {
   rocblas_handle blas = nullptr;
   double* d_S = nullptr;
   size_t S_cap = 0;
+  // precision form: FP32 Gram of the packed samples for the CG product
+  void* d_gram = nullptr;
+  size_t gram_cap = 0;
   int* d_info = nullptr;
   double* d_grad = nullptr;  // [size]
   // adjoint VJP: per-slot Slater-table cotangents [adj_slots][nqp][n2^2] and
@@ -460,6 +478,7 @@ struct mvmc_ens {
   ~mvmc_ens() {
     cg.release();
     if (d_S) (void)hipFree(d_S);
+    if (d_gram) (void)hipFree(d_gram);
     if (blas) (void)rocblas_destroy_handle(blas);
     for (void* p : allocs) (void)hipFree(p);
     if (st) (void)hipStreamDestroy(st);
@@ -1252,9 +1271,26 @@ int sr_impl(mvmc_ens* E, long long* info_out, bool force_direct = false) {
   HIPCHK(hipEventCreate(&c0));
   HIPCHK(hipEventCreate(&c1));
   HIPCHK(hipEventRecord(c0, st));
+  // precision form: the CG product from the FP32-MFMA Gram of the FP32
+  // samples (one MFMA pass, then na^2 per product instead of rows * na),
+  // when the Gram fits in 2 GiB
+  const float* gram = nullptr;
+  if (f32 && (size_t)na * na * sizeof(float) <= ((size_t)2 << 30)) {
+    const size_t gb = (size_t)na * na * sizeof(float);
+    if (E->gram_cap < gb) {
+      if (E->d_gram) (void)hipFree(E->d_gram);
+      E->d_gram = nullptr;
+      HIPCHK(hipMalloc(&E->d_gram, gb));
+      E->gram_cap = gb;
+    }
+    hipLaunchKernelGGL((k_gram_mfma<float, float>), dim3((unsigned)gm_tiles(na)), dim3(256), 0, st,
+                       (const float*)E->d_pack, (int)rows, na, 1.0, (float*)E->d_gram, 1);
+    HIPCHK(hipGetLastError());
+    gram = (const float*)E->d_gram;
+  }
   auto solve = [&](bool dscale) {
     return f32 ? cg_solve_dev(na, v.W, v.nsample, (const float*)E->d_pack, E->d_cmean, E->d_cdiag, E->d_cg, inv_w,
-                              m.mp.dsr_opt_sta_del, m.mp.dsr_opt_cg_tol, max_it, dscale, E->d_cx, E->cg, st)
+                              m.mp.dsr_opt_sta_del, m.mp.dsr_opt_cg_tol, max_it, dscale, E->d_cx, E->cg, st, gram)
                : cg_solve_dev(na, v.W, v.nsample, (const double*)E->d_pack, E->d_cmean, E->d_cdiag, E->d_cg, inv_w,
                               m.mp.dsr_opt_sta_del, m.mp.dsr_opt_cg_tol, max_it, dscale, E->d_cx, E->cg, st);
   };

@@ -142,8 +142,8 @@ __host__ __device__ inline long long gm_tiles(int na) {
   return nt * (nt + 1) / 2;
 }
 
-template <class P>
-__global__ void __launch_bounds__(256) k_gram_mfma(const P* pack, int rows, int na, double scale, double* G,
+template <class P, class OutT = double>
+__global__ void __launch_bounds__(256) k_gram_mfma(const P* pack, int rows, int na, double scale, OutT* G,
                                                    int full) {
   using Op = GramMfma<P>;
   using acc_t = typename Op::acc_t;
@@ -199,11 +199,51 @@ __global__ void __launch_bounds__(256) k_gram_mfma(const P* pack, int rows, int 
         const int i = i0 + 32 * wi + 16 * a + Op::row(lane, e);
         const int j = j0 + 32 * wj + 16 * b + (lane & 15);
         if (i < na && j < na) {
-          const double x = scale * (double)acc[a][b][e];
+          const OutT x = (OutT)(scale * (double)acc[a][b][e]);
           G[i + (size_t)j * na] = x;
           if (full && ti != tj) G[j + (size_t)i * na] = x;
         }
       }
+}
+
+// S x of the sampled operator from its Gram (the precision form of the
+// SR-CG product, Gram by FP32 MFMA): out_i = inv_w sum_j G[j + i na] x_j
+// - coef mean_i + shift diag_i x_i (minus from g when g is set), G the full
+// symmetric Gram (column i read as row i), one wave per row, FP64
+// accumulation of the FP32 Gram entries; coef = mean . x (k_gram_coef).
+// ctl[0] set: the solve has ended (as the other CG kernels).
+__global__ void __launch_bounds__(256) k_gram_apply(const float* G, int na, const double* x, const double* mean,
+                                                    const double* diag, const double* coef, double inv_w,
+                                                    double shift, const double* g, double* out, const int* ctl) {
+  if (ctl && ctl[0]) return;
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= na) return;
+  const float* col = G + (size_t)i * na;
+  double acc = 0.0;
+  for (int j = lane; j < na; j += 64) acc += (double)col[j] * x[j];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if (lane == 0) {
+    const double z = inv_w * acc - coef[0] * mean[i] + shift * diag[i] * x[i];
+    out[i] = g ? g[i] - z : z;
+  }
+}
+
+// coef = mean . x (one 256-thread block, FP64).
+__global__ void __launch_bounds__(256) k_gram_coef(const double* mean, const double* x, int na, double* coef,
+                                                   const int* ctl) {
+  if (ctl && ctl[0]) return;
+  __shared__ double red[256];
+  double p = 0.0;
+  for (int i = threadIdx.x; i < na; i += 256) p += mean[i] * x[i];
+  red[threadIdx.x] = p;
+  __syncthreads();
+  for (int h = 128; h >= 1; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) coef[0] = red[0];
 }
 
 // sr_s_g, centered() (stage_backend.rs:167-184): thread p < np forms the

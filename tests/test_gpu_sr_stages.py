@@ -218,3 +218,30 @@ def test_gram_mfma_f32(M, n, samples):
     bound = 4 * (samples + 2) * np.finfo(np.float32).eps * (np.abs(s64).T @ np.abs(s64))
     assert np.array_equal(G, G.T)
     assert np.all(np.abs(G - ref) <= bound), np.abs(G - ref).max()
+
+
+def test_sr_cg_fp32_gram_product(M, tmp_path):
+    """SR-CG of one measurement twice: the FP64 path (products as passes over
+    the FP64 samples), then the precision form (samples packed in FP32, their
+    Gram once on FP32 MFMA with FP32 accumulation, every product from the
+    Gram); the two updates within 1e-3 relative (8x8 lattice, 512 samples,
+    CG to its iteration limit)."""
+    import lattice
+    info = lattice.write_workload("square8", str(tmp_path / "sq8"), sr_steps=1, warmup=4)
+    ens = M.Ensemble(info["namelist"], info["initial"], walkers=512)
+    ens.sample()
+    ens.measure()
+    p0 = ens.params()
+    ens.sr_cg()
+    t64 = ens.timings()
+    d64 = np.concatenate(ens.params()) - np.concatenate(p0)
+    ens.set_params(*p0)
+    ens.set_precision("mixed")
+    ens.sr_cg()
+    t32 = ens.timings()
+    d32 = np.concatenate(ens.params()) - np.concatenate(p0)
+    rel = np.linalg.norm(d32 - d64) / np.linalg.norm(d64)
+    log(f"SR-CG square8 512 samples: FP32-Gram update vs FP64 relative {rel:.2e}; "
+        f"{t64['cg_iter']:.0f} / {t32['cg_iter']:.0f} iterations, {t64['cg_ms']:.1f} / {t32['cg_ms']:.1f} ms")
+    assert 0 < rel <= 1e-3
+    ens.close()
