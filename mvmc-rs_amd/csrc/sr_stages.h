@@ -105,7 +105,9 @@ __global__ void __launch_bounds__(256) k_gram_complex(const double* store, int n
 // store.  One 256-thread workgroup per 64 x 64 tile of the upper triangle
 // (tiles (ti, tj), ti <= tj, enumerated column by column); each wave owns a
 // 32 x 32 quarter as 2 x 2 blocks of 16 x 16 accumulators; rows stream
-// through LDS GM_KC at a time and every block consumes them 4 per MFMA
+// through LDS GM_KC at a time (double-buffered: the next chunk's global
+// loads are in flight while the current one feeds the MFMAs, one barrier
+// per chunk) and every block consumes them 4 per MFMA
 // (v_mfma_f64_16x16x4f64, or v_mfma_f32_16x16x4f32 with FP32 accumulation
 // for a float store).  MFMA operand layout (16 x 16 x 4, one block): lane l
 // supplies A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator element e of
@@ -147,8 +149,10 @@ __global__ void __launch_bounds__(256) k_gram_mfma(const P* pack, int rows, int 
                                                    int full) {
   using Op = GramMfma<P>;
   using acc_t = typename Op::acc_t;
-  __shared__ P sA[GM_KC][GM_LD];
-  __shared__ P sB[GM_KC][GM_LD];
+  // two LDS buffers: the chunk in use and the next one, stored from the
+  // registers its global loads landed in while the MFMAs ran
+  __shared__ P sA[2][GM_KC][GM_LD];
+  __shared__ P sB[2][GM_KC][GM_LD];
   // tile (ti, tj), ti <= tj, from the column-by-column index t
   const long long t = blockIdx.x;
   int tj = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
@@ -164,31 +168,49 @@ __global__ void __launch_bounds__(256) k_gram_mfma(const P* pack, int rows, int 
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
   const int lr = tid >> 4, lc = (tid & 15) * 4;  // loader: row lr of the chunk, 4 columns from lc
-  for (int r0 = 0; r0 < rows; r0 += GM_KC) {
+  P ra[4], rb[4];
+  auto load = [&](int r0) {
     const int r = r0 + lr;
     const bool rin = r < rows;
     const P* row = pack + (size_t)(rin ? r : 0) * na;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int ci = i0 + lc + u, cj = j0 + lc + u;
-      sA[lr][lc + u] = (rin && ci < na) ? row[ci] : (P)0;
-      sB[lr][lc + u] = (rin && cj < na) ? row[cj] : (P)0;
+      ra[u] = (rin && ci < na) ? row[ci] : (P)0;
+      rb[u] = (rin && cj < na) ? row[cj] : (P)0;
     }
-    __syncthreads();
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      sA[buf][lr][lc + u] = ra[u];
+      sB[buf][lr][lc + u] = rb[u];
+    }
+  };
+  load(0);
+  store(0);
+  __syncthreads();
+  int buf = 0;
+  for (int r0 = 0; r0 < rows; r0 += GM_KC) {
+    const bool more = r0 + GM_KC < rows;
+    if (more) load(r0 + GM_KC);
 #pragma unroll
     for (int k0 = 0; k0 < GM_KC; k0 += 4) {
       const int k = k0 + (lane >> 4), m = lane & 15;
       P av[2], bv[2];
 #pragma unroll
-      for (int a = 0; a < 2; ++a) av[a] = sA[k][32 * wi + 16 * a + m];
+      for (int a = 0; a < 2; ++a) av[a] = sA[buf][k][32 * wi + 16 * a + m];
 #pragma unroll
-      for (int b = 0; b < 2; ++b) bv[b] = sB[k][32 * wj + 16 * b + m];
+      for (int b = 0; b < 2; ++b) bv[b] = sB[buf][k][32 * wj + 16 * b + m];
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = Op::op(av[a], bv[b], acc[a][b]);
     }
+    // the other buffer was last read before the previous barrier
+    if (more) store(buf ^ 1);
     __syncthreads();
+    buf ^= 1;
   }
 #pragma unroll
   for (int a = 0; a < 2; ++a)
