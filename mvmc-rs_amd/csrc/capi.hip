@@ -24,6 +24,7 @@
 #include "../host/model.h"
 #include "ensemble.hip"
 #include "pfinv_c128_dev.h"
+#include "pfinv_rr_c128.h"
 #include "sr_stages.h"
 
 using namespace mvmc_hip;
@@ -1385,6 +1386,49 @@ struct GroupPart {
   std::vector<double> weight; // per-run work estimate the ranges are cut by
 };
 
+// Complex planes: the same shapes, full form n <= 64, panel form n <= 160
+// (two doubles per slot), each in both LTL orders.
+template <int ORDER, class F>
+void rr_dispatch_c(const RrCfg& c, F&& f) {
+  if (c.full) {
+    switch (c.r) {
+      case 1: f.template operator()<RR_TS, 1, true, ORDER>(); return;
+      case 2: f.template operator()<RR_TS, 2, true, ORDER>(); return;
+      case 3: f.template operator()<RR_TS, 3, true, ORDER>(); return;
+      case 4: f.template operator()<RR_TS, 4, true, ORDER>(); return;
+    }
+  } else {
+    switch (c.r) {
+      case 3: f.template operator()<RR_T, 3, false, ORDER>(); return;
+      case 4: f.template operator()<RR_T, 4, false, ORDER>(); return;
+      case 5: f.template operator()<RR_T, 5, false, ORDER>(); return;
+    }
+  }
+  throw Fail(MVMC_E_INVALID_SHAPE, "complex register-resident path covers 2 <= n <= 160");
+}
+
+void launch_pfinv_rr_c128(const cz* in, cz* out, int n, int count, int order, cz* pf, int* status, hipStream_t st) {
+  int dev = 0, ncu = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const RrCfg c = rr_cfg(n);
+  const size_t lds = rr_lds_bytes<cz>(n);
+  auto go = [&]<int T, int R, bool FULL, int ORDER>() {
+    raise_lds_limit((const void*)k_pfinv_rr_c128<T, R, FULL, ORDER>, lds);
+    int occ = 1;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_pfinv_rr_c128<T, R, FULL, ORDER>, T * T, lds));
+    const int grid = (int)std::min<long long>(count, (long long)std::max(occ, 1) * std::max(ncu, 1));
+    cz* gs = nullptr;
+    HIPCHK(hipMallocAsync((void**)&gs, sizeof(cz) * 2 * (size_t)n * n * grid, st));
+    hipLaunchKernelGGL((k_pfinv_rr_c128<T, R, FULL, ORDER>), dim3(grid), dim3(T * T), lds, st, in, out, n, count, gs,
+                       pf, status);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipFreeAsync(gs, st));
+  };
+  if (order == 0) rr_dispatch_c<0>(c, go);
+  else rr_dispatch_c<1>(c, go);
+}
+
 void launch_pfinv_rr(const double* in, double* out, int n, int count, double* pf, int* status, hipStream_t st) {
   int dev = 0, ncu = 0;
   HIPCHK(hipGetDevice(&dev));
@@ -1766,6 +1810,10 @@ int mvmc_pfinv_c128(const double* planes, int n, int count, int order, double* p
       throw Fail(MVMC_E_INVALID_SHAPE, "complex buffers must be 16-byte aligned");
     hipStream_t st = (hipStream_t)stream;
     const size_t nn = (size_t)n * n;
+    if (rr_enabled() && n <= RR_C_NMAX) {
+      launch_pfinv_rr_c128((const cz*)planes, (cz*)inv, n, count, order, (cz*)pf, status, st);
+      return;
+    }
     if ((const double*)inv != planes)
       HIPCHK(hipMemcpyAsync(inv, planes, sizeof(cz) * nn * count, hipMemcpyDeviceToDevice, st));
     int dev = 0, cus = 0;

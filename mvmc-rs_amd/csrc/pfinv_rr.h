@@ -31,33 +31,43 @@ constexpr int RR_TS = 16;     // threads per side of the full form (256 per work
 constexpr int RR_NMAX = 224;  // largest n (R = 7 slots per side)
 constexpr int RR_NFULL = 64;  // largest n of the full form (utu2inv's small-matrix path)
 
-// LDS of one plane: 9 vectors of n doubles, 2 of n ints, scalars.
-struct RrLds {
-  double *v0, *v1, *v2, *v3, *buf0, *buf1, *buf2, *buf3, *sup;  // sup: the superdiagonal A(i, i+1) after the LTL
+// LDS of one plane: 11 vectors of n scalars (real or complex), 2 of n
+// ints, scalars.
+template <class S>
+struct RrLdsT {
+  S *v0, *v1, *v2, *v3, *buf0, *buf1, *buf2, *buf3;
+  S* sup;          // the superdiagonal A(i, i+1) after the LTL
+  S *rneg, *rpos;  // complex solve: reciprocals of -vt[i] and vt[i]
   int *piv, *perm;
-  int* si;     // [8] ints: 0 kp, 1 zero pivot, 2 info
-  double* sd;  // [32] doubles: 0, 1 alpha by step parity, 1 the Pfaffian, 16.. max |a|^2 per wave
+  int* si;     // [16] ints: 2 info, 4..7 pivot results by step parity
+  S* sa;       // [4] scalars: 0, 1 alpha by step parity, 2 the Pfaffian
+  double* sd;  // [32] doubles: 16.. max |a|^2 per wave
 };
+using RrLds = RrLdsT<double>;
 
+template <class S = double>
 __host__ __device__ inline size_t rr_lds_bytes(int n) {
-  return 9 * rnd16((size_t)n * 8) + 2 * rnd16((size_t)n * 4) + 16 * 4 + 32 * 8;
+  return 11 * rnd16((size_t)n * sizeof(S)) + 2 * rnd16((size_t)n * 4) + 16 * 4 + rnd16(4 * sizeof(S)) + 32 * 8;
 }
 
-template <class Carver>
-__device__ inline RrLds carve_rr(Carver& c, int n) {
-  RrLds L;
-  L.v0 = c.template take<double>(n);
-  L.v1 = c.template take<double>(n);
-  L.v2 = c.template take<double>(n);
-  L.v3 = c.template take<double>(n);
-  L.buf0 = c.template take<double>(n);
-  L.buf1 = c.template take<double>(n);
-  L.buf2 = c.template take<double>(n);
-  L.buf3 = c.template take<double>(n);
-  L.sup = c.template take<double>(n);
+template <class S = double, class Carver>
+__device__ inline RrLdsT<S> carve_rr(Carver& c, int n) {
+  RrLdsT<S> L;
+  L.v0 = c.template take<S>(n);
+  L.v1 = c.template take<S>(n);
+  L.v2 = c.template take<S>(n);
+  L.v3 = c.template take<S>(n);
+  L.buf0 = c.template take<S>(n);
+  L.buf1 = c.template take<S>(n);
+  L.buf2 = c.template take<S>(n);
+  L.buf3 = c.template take<S>(n);
+  L.sup = c.template take<S>(n);
+  L.rneg = c.template take<S>(n);
+  L.rpos = c.template take<S>(n);
   L.piv = c.template take<int>(n);
   L.perm = c.template take<int>(n);
   L.si = c.template take<int>(16);
+  L.sa = c.template take<S>(4);
   L.sd = c.template take<double>(32);
   return L;
 }
@@ -76,6 +86,31 @@ __device__ inline double rr_key(double v, int i) {
   return v;
 }
 
+// Arithmetic of the real planes (dsktf2 / utu2 with real operands): every
+// operation is the one the sequential loops perform.
+struct RrReal {
+  using S = double;
+  static constexpr bool kRecip = false;  // the solve divides by vt
+  static __device__ __forceinline__ S zero() { return 0.0; }
+  static __device__ __forceinline__ S one() { return 1.0; }
+  static __device__ __forceinline__ S neg(S x) { return -x; }
+  static __device__ __forceinline__ S add(S x, S y) { return x + y; }
+  static __device__ __forceinline__ S sub(S x, S y) { return x - y; }
+  static __device__ __forceinline__ S mul(S x, S y) { return x * y; }
+  static __device__ __forceinline__ double mag2(S x) { return x * x; }
+  static __device__ __forceinline__ double key(S x, int i) { return rr_key(fabs(x), i); }
+  static __device__ __forceinline__ S alpha(S pivot) { return 1.0 / pivot; }
+  // rank-2 update of one element: (c + x_i t1) - y_i t2
+  static __device__ __forceinline__ S update(S c, S xi, S yi, S t1, S t2) { return (c + xi * t1) - yi * t2; }
+  static __device__ __forceinline__ S scale(S x, S alpha) { return x * alpha; }
+  static __device__ __forceinline__ S sign(bool neg) { return neg ? (0.0 - 1.0) : 1.0; }
+  static __device__ __forceinline__ bool finite(S x) { return isfinite(x); }
+  // solve: num / -vt (forward), num / vt (backward)
+  static __device__ __forceinline__ S div_negvt(S num, S vt, S) { return num / -vt; }
+  static __device__ __forceinline__ S div_vt(S num, S vt, S) { return num / vt; }
+  static __device__ __forceinline__ S recip(S x) { return 1.0 / x; }
+};
+
 // One plane.  src(i, j) gives A(i, j) (any i, j < n).  On exit: info (0 ok,
 // k > 0 zero pivot at 1-based row k, -1 non-finite Pfaffian), pf (the
 // Pfaffian; on a zero pivot not formed), and for info == 0 the plane `out`
@@ -84,9 +119,11 @@ __device__ inline double rr_key(double v, int i) {
 // case nothing else is done.  gs: 2 n^2 doubles of global scratch.  Every
 // thread of the T x T workgroup (blockDim.x == T * T) calls it.  FULL: the
 // full form (n <= 64), else the panel form (n > 64).
-template <int T, int R, bool FULL, class Src>
-__device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sign, double* gs, RrLds& L, double& pf,
-                                         int& info, bool check_zero, bool& allzero) {
+template <int T, int R, bool FULL, class Ar = RrReal, class Src>
+__device__ __forceinline__ void rr_pfinv(Src src, int n, typename Ar::S* out, double sign, typename Ar::S* gs,
+                                         RrLdsT<typename Ar::S>& L, typename Ar::S& pf, int& info, bool check_zero,
+                                         bool& allzero) {
+  using S = typename Ar::S;
   constexpr int NS = FULL ? R * R : R * (R + 1) / 2;
   const int tid = threadIdx.x, NT = T * T;
   const int a0 = tid % T, b0 = tid / T;
@@ -100,7 +137,7 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
     b = b0;        \
     asm volatile("" : "+v"(a), "+v"(b)); \
   } while (0)
-  double c[NS];
+  S c[NS];
 #ifdef MVMC_PHASE_CLOCKS
   long long rr_t0 = wall_clock64(), rr_ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   long long rr_s0 = 0, rr_sub[4] = {0, 0, 0, 0};
@@ -131,11 +168,13 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
 #pragma unroll
     for (int cj = ri; cj < R; ++cj) {
       const int i = a + T * ri, j = b + T * cj;
-      double x = 0.0;
+      S x = Ar::zero();
       if (i < n && j < n && i <= j) {
         x = src(i, j);
-        const double y = src(j, i);
-        mx = fmax(mx, fmax(x * x, y * y));
+        if (check_zero) {
+          const S y = src(j, i);
+          mx = fmax(mx, fmax(Ar::mag2(x), Ar::mag2(y)));
+        }
       }
       c[rr_slot<R, FULL>(ri, cj)] = x;
     }
@@ -143,7 +182,7 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
 #pragma unroll
     for (int ri = 1; ri < R; ++ri)
 #pragma unroll
-      for (int cj = 0; cj < ri; ++cj) c[rr_slot<R, FULL>(ri, cj)] = 0.0;
+      for (int cj = 0; cj < ri; ++cj) c[rr_slot<R, FULL>(ri, cj)] = Ar::zero();
   }
   for (int i = tid; i < n; i += NT) L.piv[i] = i + 1;
   if (tid == 0) L.si[2] = 0;
@@ -205,7 +244,7 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
         for (int ri = 0; ri <= cj; ++ri) {
           const int i = a + T * ri;
           if (i <= kk) {
-            const double key = rr_key(fabs(c[rr_slot<R, FULL>(ri, cj)]), i);
+            const double key = Ar::key(c[rr_slot<R, FULL>(ri, cj)], i);
             if (key > best) { best = key; bi = i; }
           }
         }
@@ -245,58 +284,58 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
       // interchange rows/columns kk0 and kp in the upper triangle with the
       // sign twiddles: stage column kk0 (rows < kk0), column kp (rows < kp),
       // row kp (columns > kp), row kk0 (columns >= k0)
-      for_col(kk0, [&](int i, double& x) { if (i < kk0) L.v0[i] = x; });
-      for_col(kp, [&](int i, double& x) { if (i < kp) L.v1[i] = x; });
-      for_row(kp, [&](int j, double& x) { if (j > kp && j < n) L.v2[j] = x; });
-      for_row(kk0, [&](int j, double& x) { if (j >= k0 && j < n) L.v3[j] = x; });
+      for_col(kk0, [&](int i, S& x) { if (i < kk0) L.v0[i] = x; });
+      for_col(kp, [&](int i, S& x) { if (i < kp) L.v1[i] = x; });
+      for_row(kp, [&](int j, S& x) { if (j > kp && j < n) L.v2[j] = x; });
+      for_row(kk0, [&](int j, S& x) { if (j >= k0 && j < n) L.v3[j] = x; });
       __syncthreads();
       RR_SUB(1);
       // write-back; the four groups target disjoint elements
-      for_col(kk0, [&](int i, double& x) {
+      for_col(kk0, [&](int i, S& x) {
         if (i < kp) x = L.v1[i];
-        else if (i == kp) x = -L.v0[kp];
-        else if (i < kk0) x = -L.v2[i];
+        else if (i == kp) x = Ar::neg(L.v0[kp]);
+        else if (i < kk0) x = Ar::neg(L.v2[i]);
       });
-      for_col(kp, [&](int i, double& x) { if (i < kp) x = L.v0[i]; });
-      for_row(kp, [&](int j, double& x) {
-        if (j > kp && j < kk0) x = -L.v0[j];
+      for_col(kp, [&](int i, S& x) { if (i < kp) x = L.v0[i]; });
+      for_row(kp, [&](int j, S& x) {
+        if (j > kp && j < kk0) x = Ar::neg(L.v0[j]);
         else if (j >= k0 && j < n) x = L.v3[j];
       });
-      for_row(kk0, [&](int j, double& x) { if (j >= k0 && j < n) x = L.v2[j]; });
+      for_row(kk0, [&](int j, S& x) { if (j >= k0 && j < n) x = L.v2[j]; });
     }
     if (tid == 0) L.piv[kk0] = kp + 1;
     if (kk0 >= 1) {
       // rank-2 update of the leading kk0 x kk0 block with x = column k0,
       // y = column kk0 (as swapped), alpha = 1 / A(kk0, k0)
-      double* xs = par ? L.buf0 : L.buf2;
-      double* ys = par ? L.buf1 : L.buf3;
-      for_col(k0, [&](int i, double& x) {
+      S* xs = par ? L.buf0 : L.buf2;
+      S* ys = par ? L.buf1 : L.buf3;
+      for_col(k0, [&](int i, S& x) {
         if (i < kk0) xs[i] = x;
-        else if (i == kk0) L.sd[par] = 1.0 / x;
+        else if (i == kk0) L.sa[par] = Ar::alpha(x);
       });
-      for_col(kk0, [&](int i, double& x) { if (i < kk0) ys[i] = x; });
+      for_col(kk0, [&](int i, S& x) { if (i < kk0) ys[i] = x; });
       pivot_search(kk0, par ^ 1);
       __syncthreads();
       RR_SUB(2);
-      const double alpha = L.sd[par];
+      const S alpha = L.sa[par];
 #pragma unroll
       for (int cj = 0; cj < R; ++cj)
         if (T * cj < kk0) {
           const int j = b + T * cj;
           if (j < kk0) {
-            const double t1 = alpha * ys[j];
-            const double t2 = alpha * xs[j];
+            const S t1 = Ar::mul(alpha, ys[j]);
+            const S t2 = Ar::mul(alpha, xs[j]);
 #pragma unroll
             for (int ri = 0; ri <= cj; ++ri) {
               const int i = a + T * ri;
-              double& x = c[rr_slot<R, FULL>(ri, cj)];
-              if (i < j) x = (x + xs[i] * t1) - ys[i] * t2;
-              else if (i == j) x = 0.0;
+              S& x = c[rr_slot<R, FULL>(ri, cj)];
+              if (i < j) x = Ar::update(x, xs[i], ys[i], t1, t2);
+              else if (i == j) x = Ar::zero();
             }
           }
         }
       // scale column k0 (its pre-scale values were staged in xs)
-      for_col(k0, [&](int i, double& x) { if (i < kk0) x *= alpha; });
+      for_col(k0, [&](int i, S& x) { if (i < kk0) x = Ar::scale(x, alpha); });
       RR_SUB(3);
     }
   }
@@ -304,7 +343,7 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
   RR_PHASE(1);
   info = L.si[2];
   if (info != 0) {
-    pf = 0.0;
+    pf = Ar::zero();
     return;
   }
   // ---- utu2pfa: product of A(i, i+1), i even, sign by the count of moves ----
@@ -318,17 +357,22 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
     }
   __syncthreads();
   if (tid == 0) {
-    double p = 1.0;
-    for (int i = 0; i + 1 < n; i += 2) p *= L.sup[i];
+    S p = Ar::one();
+    for (int i = 0; i + 1 < n; i += 2) p = Ar::mul(p, L.sup[i]);
     bool neg = false;
     for (int k = 0; k < n; ++k)
       if (L.piv[k] != k + 1) neg = !neg;
-    const double s = neg ? (0.0 - 1.0) : 1.0;
-    L.sd[1] = s * p;
+    L.sa[2] = Ar::mul(Ar::sign(neg), p);
   }
+  if (Ar::kRecip)
+    for (int i = tid; i + 1 < n; i += NT) {
+      const S vt = Ar::neg(L.sup[i]);
+      L.rneg[i] = Ar::recip(Ar::neg(vt));
+      L.rpos[i] = Ar::recip(vt);
+    }
   __syncthreads();
-  pf = L.sd[1];
-  if (!isfinite(pf)) {
+  pf = L.sa[2];
+  if (!Ar::finite(pf)) {
     info = -1;
     return;
   }
@@ -343,15 +387,15 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
   // ascending order, as the row-chain loops do.
   for (int k = 1; k <= n - 2; ++k) {
     RR_FRESH();
-    double* U = (k & 1) ? L.v0 : L.v2;   // column k of the inverse, rows < k
-    double* Bk = (k & 1) ? L.v1 : L.v3;  // row k of B (columns > k)
-    for_col(k + 1, [&](int i, double& x) {
+    S* U = (k & 1) ? L.v0 : L.v2;   // column k of the inverse, rows < k
+    S* Bk = (k & 1) ? L.v1 : L.v3;  // row k of B (columns > k)
+    for_col(k + 1, [&](int i, S& x) {
       if (i < k) {
-        x = -x;
+        x = Ar::neg(x);
         U[i] = x;
       }
     });
-    for_row(k, [&](int jj, double& x) { if (jj >= k + 2 && jj < n) Bk[jj - 1] = x; });
+    for_row(k, [&](int jj, S& x) { if (jj >= k + 2 && jj < n) Bk[jj - 1] = x; });
     __syncthreads();
     if (k <= n - 3) {
 #pragma unroll
@@ -362,8 +406,8 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
             if (T * cj + T > k + 2) {
               const int i = a + T * ri, jj = b + T * cj;
               if (i < k && jj >= k + 2 && jj < n) {
-                double& x = c[rr_slot<R, FULL>(ri, cj)];
-                x = x + U[i] * Bk[jj - 1];
+                S& x = c[rr_slot<R, FULL>(ri, cj)];
+                x = Ar::add(x, Ar::mul(U[i], Bk[jj - 1]));
               }
             }
         }
@@ -373,9 +417,9 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
   RR_PHASE(3);
   // M (unit upper, M(i, j) = Bn(i, j) for i < j <= n-2, M(i, n-1) = 0 for
   // i < n-1) to gs row-major: gm[i * n + j]
-  double* gm = gs;
-  double* gc = gs + (size_t)n * n;
-  for (int e = tid; e < n * n; e += NT) gm[e] = (e / n == e % n) ? 1.0 : 0.0;
+  S* gm = gs;
+  S* gc = gs + (size_t)n * n;
+  for (int e = tid; e < n * n; e += NT) gm[e] = (e / n == e % n) ? Ar::one() : Ar::zero();
   __syncthreads();
 #pragma unroll
   for (int ri = 0; ri < R; ++ri)
@@ -389,7 +433,7 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
   // skew-tridiagonal solve T C = M, all columns at once, row by row through
   // LDS (vt[i] = -A(i, i+1) = -sup[i]); every row of C goes to gc row-major
   // and into the registers (panel form: the strict upper part).
-  auto keep_row = [&](int r, const double* row) {
+  auto keep_row = [&](int r, const S* row) {
 #pragma unroll
     for (int ri = 0; ri < R; ++ri)
       if (a + T * ri == r) {
@@ -406,61 +450,61 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
     const int j = tid;
     const bool col = j < n;
     // forward: c[1] = b[0] / -vt[0]; c[i+1] = (b[i] - c[i-1] vt[i-1]) / -vt[i]
-    double m0 = col ? gm[j] : 0.0;
-    double m1 = (col && 2 < n) ? gm[(size_t)2 * n + j] : 0.0;
-    double m2 = (col && 4 < n) ? gm[(size_t)4 * n + j] : 0.0;
+    S m0 = col ? gm[j] : Ar::zero();
+    S m1 = (col && 2 < n) ? gm[(size_t)2 * n + j] : Ar::zero();
+    S m2 = (col && 4 < n) ? gm[(size_t)4 * n + j] : Ar::zero();
     if (col) {
-      const double vt0 = -L.sup[0];
-      const double x = m0 / -vt0;
+      const S vt0 = Ar::neg(L.sup[0]);
+      const S x = Ar::div_negvt(m0, vt0, L.rneg[0]);
       L.buf0[j] = x;
       gc[(size_t)1 * n + j] = x;
     }
     __syncthreads();
     keep_row(1, L.buf0);
-    double* prev = L.buf0;
-    double* next = L.buf1;
+    S* prev = L.buf0;
+    S* next = L.buf1;
     for (int i = 2; i < n; i += 2) {
-      const double mi = m1;
+      const S mi = m1;
       m1 = m2;
-      m2 = (col && i + 4 < n) ? gm[(size_t)(i + 4) * n + j] : 0.0;
-      const double vtm = -L.sup[i - 1], vti = -L.sup[i];
+      m2 = (col && i + 4 < n) ? gm[(size_t)(i + 4) * n + j] : Ar::zero();
+      const S vtm = Ar::neg(L.sup[i - 1]), vti = Ar::neg(L.sup[i]);
       if (col) {
-        const double x = (mi - prev[j] * vtm) / -vti;
+        const S x = Ar::div_negvt(Ar::sub(mi, Ar::mul(prev[j], vtm)), vti, L.rneg[i]);
         next[j] = x;
         gc[(size_t)(i + 1) * n + j] = x;
       }
       __syncthreads();
       RR_FRESH();
       keep_row(i + 1, next);
-      double* t = prev; prev = next; next = t;
+      S* t = prev; prev = next; next = t;
     }
     __syncthreads();
     // backward: c[n-2] = b[n-1] / vt[n-2]; c[i-1] = (b[i] + c[i+1] vt[i]) / vt[i-1]
-    m0 = col ? gm[(size_t)(n - 1) * n + j] : 0.0;
-    m1 = (col && n - 3 >= 1) ? gm[(size_t)(n - 3) * n + j] : 0.0;
-    m2 = (col && n - 5 >= 1) ? gm[(size_t)(n - 5) * n + j] : 0.0;
+    m0 = col ? gm[(size_t)(n - 1) * n + j] : Ar::zero();
+    m1 = (col && n - 3 >= 1) ? gm[(size_t)(n - 3) * n + j] : Ar::zero();
+    m2 = (col && n - 5 >= 1) ? gm[(size_t)(n - 5) * n + j] : Ar::zero();
     if (col) {
-      const double vtl = -L.sup[n - 2];
-      const double x = m0 / vtl;
+      const S vtl = Ar::neg(L.sup[n - 2]);
+      const S x = Ar::div_vt(m0, vtl, L.rpos[n - 2]);
       prev[j] = x;
       gc[(size_t)(n - 2) * n + j] = x;
     }
     __syncthreads();
     keep_row(n - 2, prev);
     for (int i = n - 3; i >= 1; i -= 2) {
-      const double mi = m1;
+      const S mi = m1;
       m1 = m2;
-      m2 = (col && i - 4 >= 1) ? gm[(size_t)(i - 4) * n + j] : 0.0;
-      const double vti = -L.sup[i], vtm = -L.sup[i - 1];
+      m2 = (col && i - 4 >= 1) ? gm[(size_t)(i - 4) * n + j] : Ar::zero();
+      const S vti = Ar::neg(L.sup[i]), vtm = Ar::neg(L.sup[i - 1]);
       if (col) {
-        const double x = (mi + prev[j] * vti) / vtm;
+        const S x = Ar::div_vt(Ar::add(mi, Ar::mul(prev[j], vti)), vtm, L.rpos[i - 1]);
         next[j] = x;
         gc[(size_t)(i - 1) * n + j] = x;
       }
       __syncthreads();
       RR_FRESH();
       keep_row(i - 1, next);
-      double* t = prev; prev = next; next = t;
+      S* t = prev; prev = next; next = t;
     }
   }
   __syncthreads();
@@ -475,12 +519,12 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
     // rows k of M and C, loaded two steps ahead by thread j < n
     const int j = tid;
     const bool col = j < n;
-    double pm0 = col ? gm[j] : 0.0, pc0 = col ? gc[j] : 0.0;
-    double pm1 = (col && 1 < n) ? gm[(size_t)n + j] : 0.0, pc1 = (col && 1 < n) ? gc[(size_t)n + j] : 0.0;
+    S pm0 = col ? gm[j] : Ar::zero(), pc0 = col ? gc[j] : Ar::zero();
+    S pm1 = (col && 1 < n) ? gm[(size_t)n + j] : Ar::zero(), pc1 = (col && 1 < n) ? gc[(size_t)n + j] : Ar::zero();
     for (int k = 0; k + 2 < n; ++k) {
       RR_FRESH();
-      double* Mr = (k & 1) ? L.v0 : L.v2;
-      double* Cr = (k & 1) ? L.v1 : L.v3;
+      S* Mr = (k & 1) ? L.v0 : L.v2;
+      S* Cr = (k & 1) ? L.v1 : L.v3;
       if (col) {
         Mr[j] = pm0;
         Cr[j] = pc0;
@@ -499,8 +543,8 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
           for (int cj = FULL ? 0 : ri; cj < R; ++cj) {
             const int i = a + T * ri, jj = b + T * cj;
             if (k < i && (FULL || i < jj) && jj < n && i < n) {
-              double& x = c[rr_slot<R, FULL>(ri, cj)];
-              x = x + Mr[i] * Cr[jj];
+              S& x = c[rr_slot<R, FULL>(ri, cj)];
+              x = Ar::add(x, Ar::mul(Mr[i], Cr[jj]));
             }
           }
         }
@@ -532,11 +576,11 @@ __device__ __forceinline__ void rr_pfinv(Src src, int n, double* out, double sig
   for (int e = tid; e < n * n; e += NT) {
     const int r = e % n, cc = e / n;
     const int u = L.perm[r], v = L.perm[cc];
-    double x = 0.0;
+    S x = Ar::zero();
     if (FULL) x = gc[(size_t)u * n + v];
     else if (u < v) x = gc[(size_t)u * n + v];
-    else if (u > v) x = -gc[(size_t)v * n + u];
-    out[e] = sign != 1.0 ? -x : x;
+    else if (u > v) x = Ar::neg(gc[(size_t)v * n + u]);
+    out[e] = sign != 1.0 ? Ar::neg(x) : x;
   }
   __syncthreads();
   RR_PHASE(7);
