@@ -72,7 +72,9 @@ int mvmc_pfinv_f64_noalloc(const double* planes, int n, int count, double* pf, d
  * order 1: zsktf2_c_compat (ltl.rs:54-59, the order of the production
  * calc_m_all_complex, crates/mvmc-core/src/pfaffian.rs:383-398).
  * inv == planes factorises in place.  Device pointers; asynchronous on
- * `stream` (NULL = default stream). */
+ * `stream` (NULL = default stream).  n <= 160: one plane per workgroup held
+ * in registers; larger n: one plane per 256-lane workgroup through a global
+ * work plane (MVMC_PFINV_RR=0 selects the latter for every n); same bits. */
 int mvmc_pfinv_c128(const double* planes, int n, int count, int order, double* pf, double* inv, int* status,
                     void* stream);
 
