@@ -137,6 +137,7 @@ struct GramMfma<float> {
   static __device__ __forceinline__ int row(int lane, int e) { return 4 * (lane >> 4) + e; }
 };
 constexpr int GM_T = 64, GM_KC = 16;
+constexpr int GM_XCDS = 8;  // XCDs of an MI355X (workgroup dispatch order)
 constexpr int GM_LD = 80;  // LDS row stride (elements): the 4 k-rows of one operand fetch fall in two bank halves
 
 __host__ __device__ inline long long gm_tiles(int na) {
@@ -153,8 +154,15 @@ __global__ void __launch_bounds__(256) k_gram_mfma(const P* pack, int rows, int 
   // registers its global loads landed in while the MFMAs ran
   __shared__ P sA[2][GM_KC][GM_LD];
   __shared__ P sB[2][GM_KC][GM_LD];
+  // XCD-aware tile order: workgroups are dealt round-robin over the 8 XCDs
+  // (blockIdx.x % 8), so XCD x gets the x-th contiguous range of the
+  // column-by-column tile order and its L2 serves the column tiles its
+  // neighbours in that range share
+  const long long ntiles = gridDim.x;
+  const long long xcd = blockIdx.x % GM_XCDS, local = blockIdx.x / GM_XCDS;
+  const long long base = ntiles / GM_XCDS, rem = ntiles % GM_XCDS;
+  const long long t = xcd * base + (xcd < rem ? xcd : rem) + local;
   // tile (ti, tj), ti <= tj, from the column-by-column index t
-  const long long t = blockIdx.x;
   int tj = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
   while ((long long)(tj + 1) * (tj + 2) / 2 <= t) ++tj;
   while ((long long)tj * (tj + 1) / 2 > t) --tj;
