@@ -32,9 +32,12 @@ that already differ and are reported for speed.
 
 value  = saved samples / second of the whole timed SR steps (all walkers).
 roofline: dominant kernel = the sampling kernel k_block; algorithmic bytes
-per launch = proposals * NQP * 2n*8 (inverse row + Slater row) + accepted
-moves * NQP * 2n^2*8 (inverse read + write), divided by the launch's
-average device time (HIP events on the kernel's stream).  traffic = HBM
+per launch = proposals * NQP * 2n*8 (inverse row + Slater row) + (accepted
+moves + flush passes) * NQP * n^2*8 (one read of the inverse planes each) +
+write passes * NQP * n^2*8 (the planes written back: every second accepted
+hop, and each flush of the pending updates before an exchange move or at
+the end of a block), divided by the launch's average device time (HIP
+events on the kernel's stream).  traffic = HBM
 bytes per launch from the profiled counter passes recorded under profiles/
 (bytes per proposal and per accepted move), applied to this run's counts.
 cpu_baseline: the FP64 oracle (same algorithm and operation order) on a
@@ -348,7 +351,8 @@ def main():
         step_times.append(dt)
         rows.append(row)
         kernel.append((after["block_ms"], after["block_launches"], after["proposals"] - before["proposals"],
-                       after["accepted"] - before["accepted"]))
+                       after["accepted"] - before["accepted"], after["block_write_passes"],
+                       after["block_flush_passes"]))
         ll = ens.launch_log()
         launch_logs.append(ll)
         log(f"step {s}: E/site = {row[0] / info['nsite']:.8f}  {dt:.2f} s  (sampling {after['sample_s'] - before['sample_s']:.2f} s, "
@@ -368,7 +372,9 @@ def main():
     block_ms = sum(k[0] for k in kernel)
     props = sum(k[2] for k in kernel)
     accs = sum(k[3] for k in kernel)
-    bytes_total = props * nqp * 2 * n * 8 + accs * nqp * 2 * n * n * 8
+    wpasses = sum(k[4] for k in kernel)
+    fpasses = sum(k[5] for k in kernel)
+    bytes_total = props * nqp * 2 * n * 8 + (accs + fpasses + wpasses) * nqp * n * n * 8
     avg_launch_s = block_ms / 1e3 / max(launches, 1)
     achieved = bytes_total / max(launches, 1) / avg_launch_s / 1e9
     tmodel = traffic_model(args.workload)
@@ -405,6 +411,7 @@ def main():
                      "traffic": traffic_per_launch(tmodel, props, accs, launches, bytes_total),
                      "kernel": "k_block", "avg_launch_ms": avg_launch_s * 1e3, "launches": launches,
                      "algorithmic_bytes_per_launch": bytes_total / max(launches, 1),
+                     "accepted_moves": accs, "write_passes": wpasses, "flush_passes": fpasses,
                      "walkers_per_launch": [[int(x) for x in ll[:, 3]] for ll in launch_logs]},
     }
     if tmodel is not None:

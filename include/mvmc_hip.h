@@ -215,7 +215,7 @@ int mvmc_ens_get_walker(mvmc_ens* e, int walker, int* saved_ele_idx, long long* 
 int mvmc_ens_get_tables(mvmc_ens* e, int walker, double* inv, double* pf);
 int mvmc_ens_get_config(mvmc_ens* e, int walker, int* ele_idx, int* ele_cfg, int* ele_num, int* proj_cnt);
 int mvmc_ens_get_cg(mvmc_ens* e, double* solution, int* n_active);
-/* timings[24]: sampling s, measurement s, SR s (wall, summed over calls),
+/* timings[26]: sampling s, measurement s, SR s (wall, summed over calls),
  * proposals, accepted, device ms of the last sampling call's k_block
  * launches, their number, device ms of its periodic-rebuild launches,
  * walkers those rebuilt; device ms of the last measurement call's rebuilds
@@ -227,8 +227,13 @@ int mvmc_ens_get_cg(mvmc_ens* e, double* solution, int* n_active);
  * one 1024-thread workgroup per CU, 64 < n <= 224) and the device ms per
  * rebuilt walker measured for variants 0, 1 and 2; device ms of the last
  * mvmc_ens_vjp_adjoint call and of its plane rebuilds; device ms of the
- * last direct SR's Gram kernel; one reserved. */
-int mvmc_ens_timings(mvmc_ens* e, double* timings24);
+ * last direct SR's Gram kernel; passes of the last sampling call's k_block
+ * launches that wrote a walker's inverse planes back (every second accepted
+ * hop, and the flushes), and of them the flushes: passes that accept no
+ * move (before an exchange move, at the end of a block); one reserved.  An
+ * accepted hop reads the planes once and writes them back every second
+ * time. */
+int mvmc_ens_timings(mvmc_ens* e, double* timings26);
 /* Device phase clocks of the sampling kernel, summed over walkers (ticks of
  * the 100 MHz constant clock): 0 candidate, 1 projection deltas + ratio,
  * 2 inverse/Slater row staging, 3 Pfaffian ratio dots, 4 Metropolis,

@@ -447,6 +447,9 @@ struct mvmc_ens {
   double t_sample = 0, t_measure = 0, t_sr = 0, proposals = 0, accepted = 0;
   float block_ms = 0;
   int block_launches = 0;
+  // passes of the last sample call's k_block launches over a walker's planes
+  // that write them back, and those of them that accept no move (flushes)
+  double block_write_passes = 0, block_flush_passes = 0;
   // per k_block launch of the last sample call: device ms, proposals,
   // accepted moves and walkers that advanced in it
   std::vector<double> launch_log;
@@ -720,8 +723,8 @@ void create_impl(mvmc_ens* E, const char* namelist, const char* initial_def, int
                       std::max(std::max(E->grid_rb[0], E->grid_rb[1]), E->grid_rb[2]));
   v.work = dalloc<double>(A, (size_t)E->slots * work_per_slot);
   v.qorb = dalloc<double>(A, (size_t)E->slots * v.nqp * std::max(v.nslater, 1));
-  v.pend = dalloc<double>(A, (size_t)E->slots * 2 * v.nqp * v.n);
-  v.nflag = dalloc<int>(A, 4);
+  v.pend = dalloc<double>(A, (size_t)E->slots * 2 * PEND_MAX * v.nqp * v.n);
+  v.nflag = dalloc<int>(A, NFLAGS);
   v.rlist = dalloc<int>(A, (size_t)walkers);
 #ifdef MVMC_PHASE_CLOCKS
   v.clk = dalloc<long long>(A, (size_t)E->slots * 32);
@@ -793,14 +796,14 @@ void sample_impl(mvmc_ens* E) {
   HIPCHK(hipEventCreate(&r1));
   double block_ms = 0, rebuild_ms = 0;
   int launches = 0, rebuilds = 0;
-  long long rebuilt = 0;
+  long long rebuilt = 0, wpasses = 0, fpasses = 0;
   E->launch_log.clear();
   std::vector<long long> cnt_prev((size_t)v.W * 4), cnt_now((size_t)v.W * 4);
   E->d2h(cnt_prev.data(), v.counter, sizeof(long long) * cnt_prev.size());
   for (int out = 0; out < n_out; ++out) {
     hipLaunchKernelGGL(k_reset_resume, dim3(nblk(v.W)), dim3(256), 0, E->st, v);
     for (;;) {
-      HIPCHK(hipMemsetAsync(v.nflag, 0, 4 * sizeof(int), E->st));
+      HIPCHK(hipMemsetAsync(v.nflag, 0, NFLAGS * sizeof(int), E->st));
       HIPCHK(hipEventRecord(e0, E->st));
       if (E->precision == 1)
         hipLaunchKernelGGL((k_block<float, false>), dim3(E->grid_samp), dim3(256), E->lds_samp, E->st, v, GroupArgs{},
@@ -810,9 +813,12 @@ void sample_impl(mvmc_ens* E) {
                            out, n_in);
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(e1, E->st));
-      int nf = 0;
-      HIPCHK(hipMemcpyAsync(&nf, v.nflag, sizeof(int), hipMemcpyDeviceToHost, E->st));
+      int flags[NFLAGS] = {};
+      HIPCHK(hipMemcpyAsync(flags, v.nflag, sizeof(flags), hipMemcpyDeviceToHost, E->st));
       HIPCHK(hipStreamSynchronize(E->st));
+      const int nf = flags[0];
+      wpasses += flags[3];
+      fpasses += flags[4];
       float ms = 0;
       HIPCHK(hipEventElapsedTime(&ms, e0, e1));
       block_ms += ms;
@@ -867,6 +873,8 @@ void sample_impl(mvmc_ens* E) {
   for (hipEvent_t ev : {e0, e1, r0, r1}) (void)hipEventDestroy(ev);
   E->block_ms = block_ms;
   E->block_launches = launches;
+  E->block_write_passes = (double)wpasses;
+  E->block_flush_passes = (double)fpasses;
   E->rebuild_ms = rebuild_ms;
   E->rebuild_launches = rebuilds;
   E->rebuilt = (double)rebuilt;
@@ -1455,7 +1463,7 @@ struct mvmc_group {
   hipStream_t st = nullptr;
   std::vector<void*> allocs;
   EnsView* d_views = nullptr;
-  int* d_flags = nullptr;  // [nruns][4]: the runs' walker counters (their v.nflag points here)
+  int* d_flags = nullptr;  // [nruns][NFLAGS]: the runs' walker counters (their v.nflag points here)
   size_t lds_samp = 0, lds_meas = 0, lds_rebuild = 0;
   GroupPart samp, meas, rb;  // slot ranges of the placement, measurement and measurement-rebuild launches
   // shared walker queue of the sweep and periodic-rebuild launches
@@ -1534,11 +1542,11 @@ void group_create_impl(mvmc_group* G, int nruns, const char* const* namelists, c
       throw Fail(MVMC_E_INPUT, "runs of a group must share NVMCSample");
   }
   HIPCHK(hipStreamCreateWithFlags(&G->st, hipStreamNonBlocking));
-  G->d_flags = dalloc<int>(G->allocs, 4 * (size_t)nruns);
+  G->d_flags = dalloc<int>(G->allocs, NFLAGS * (size_t)nruns);
   std::vector<EnsView> views;
   for (int r = 0; r < nruns; ++r) {
     mvmc_ens* E = G->runs[r];
-    E->v.nflag = G->d_flags + 4 * r;
+    E->v.nflag = G->d_flags + NFLAGS * r;
     views.push_back(E->v);
     G->lds_samp = std::max(G->lds_samp, E->lds_samp);
     G->lds_meas = std::max(G->lds_meas, E->lds_meas);
@@ -1584,7 +1592,7 @@ void group_create_impl(mvmc_group* G, int nruns, const char* const* namelists, c
   HIPCHK(hipMemcpyAsync(G->d_qrun, order.data(), sizeof(int) * nruns, hipMemcpyHostToDevice, G->st));
   for (mvmc_ens* E : G->runs) {
     G->work_stride = std::max<long long>(G->work_stride, (long long)pfinv_work_doubles(E->v.n, E->v.nqp));
-    G->pend_stride = std::max<long long>(G->pend_stride, 2LL * E->v.nqp * E->v.n);
+    G->pend_stride = std::max<long long>(G->pend_stride, 2LL * PEND_MAX * E->v.nqp * E->v.n);
   }
   const size_t slots = (size_t)std::max(G->samp.capacity, G->rb.capacity);
   G->d_work = dalloc<double>(G->allocs, slots * G->work_stride);
@@ -2214,7 +2222,8 @@ int mvmc_ens_timings(mvmc_ens* e, double* t) {
   t[17] = e->rb_walkers[0] > 0 ? e->rb_ms[0] / e->rb_walkers[0] : 0.0;
   t[18] = e->rb_walkers[1] > 0 ? e->rb_ms[1] / e->rb_walkers[1] : 0.0;
   t[19] = e->rb_walkers[2] > 0 ? e->rb_ms[2] / e->rb_walkers[2] : 0.0;
-  t[20] = e->adj_ms; t[21] = e->adj_rebuild_ms; t[22] = e->gram_ms; t[23] = 0.0;
+  t[20] = e->adj_ms; t[21] = e->adj_rebuild_ms; t[22] = e->gram_ms; t[23] = e->block_write_passes;
+  t[24] = e->block_flush_passes; t[25] = 0.0;
   return MVMC_OK;
 }
 

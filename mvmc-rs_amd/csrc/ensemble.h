@@ -5,6 +5,8 @@
 
 namespace mvmc_hip {
 
+constexpr int NFLAGS = 8;  // ints of an ensemble's walker counters (EnsView::nflag)
+
 struct EnsView {
   // dimensions
   int W;          // walkers
@@ -47,8 +49,10 @@ struct EnsView {
   double* log_ip;     // [W]
   int* wstate;        // [W][8]: 0 burn flag, 1 n_accept window, 2 n_out, 3 status,
                       //   4 next proposal of the outer step, 5 rebuild pending, 6 outer step done, 7 spare
-  int* nflag;         // [4]: 0 walkers handed to the rebuild kernel by the last k_block launch,
-                      //   1 next walker taken by a k_block slot, 2 next hand-over entry taken by k_rebuild
+  int* nflag;         // [NFLAGS]: 0 walkers handed to the rebuild kernel by the last k_block launch,
+                      //   1 next walker taken by a k_block slot, 2 next hand-over entry taken by k_rebuild,
+                      //   3 passes of the last k_block launch that wrote a walker's planes back,
+                      //   4 those of them that accepted no move (pending updates flushed)
   int* rlist;         // [W] walkers handed to the rebuild kernel, in hand-over order
   long long* counter; // [W][4]
   // saved samples
@@ -61,7 +65,7 @@ struct EnsView {
   // per-slot scratch (persistent grids: slot = blockIdx.x)
   double* work;       // [slots][nqp][n*n]
   double* qorb;       // [slots][nqp][nslater]
-  double* pend;       // [slots][2][nqp][n]: vec1 / vec2 of the not yet applied hop update
+  double* pend;       // [slots][nqp][n][PEND_MAX][2]: vec1 / vec2 of the hop updates not yet written back
   int* status;        // [1] sticky
   long long* clk;     // [slots][32] phase clocks (MVMC_PHASE_CLOCKS builds), else null
   double* gscr;       // [slots][gstride] per-term Green ratios of the measurement

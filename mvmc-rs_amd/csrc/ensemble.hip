@@ -651,20 +651,37 @@ struct PlaneIO<float> {
   }
 };
 
+// Accepted hop updates not yet written back to the stored planes: up to
+// PEND_MAX of them, in acceptance order.  Update k is the pair of vectors
+// (vec1, vec2) of updates.rs:1184-1224 and the row pm[k] it replaces; the
+// pairs live in the slot's scratch as pend[(q * n + i) * PEND_MAX + k] =
+// {vec1[q][i], vec2[q][i]}, so one 16-byte load gives a row's two values.
+// An element of the current inverse is its stored value with the pending
+// updates applied in order (apply_pend), which is bit for bit the eagerly
+// updated element; so the pass of an accepted hop, which has to read every
+// element anyway, writes the planes back only when the list is full.
+constexpr int PEND_MAX = 2;
+struct PendList {
+  int np;             // pending updates (uniform over the workgroup)
+  int pm[PEND_MAX];   // their rows
+};
+
 // One streaming pass over a walker's planes.  A lane owns CPL = 2 * EPV
 // contiguous columns of one plane (q = tid / LP; columns
 // LP*CPL*g + CPL*(tid % LP) + c), read and written as two 16-byte vectors
 // per row, and issues the loads of STREAM_ROWS rows before using any of
 // them: each lane keeps STREAM_ROWS rows of its columns in flight, which is
 // what bounds a pass over HBM at this occupancy (the FP32 form therefore
-// covers twice the columns per round trip).
-//   pm >= 0 : apply the pending update (pv1, pv2, row pm) and write back,
-//             in the storage type's arithmetic;
-//   ACC     : out[q][j] = sum_i -X[i][j] * b[q][i] in FP64, serial in i
-//             (the update_one_real vec1 = -(X^T b)), X as stored.
+// covers twice the columns per round trip).  The pending values of a tile's
+// rows (cache hits) are loaded before the next tile's rows, so waiting for
+// them does not wait for that tile.
+//   P.np > 0 : apply the pending updates, in order, in the storage type's
+//              arithmetic; store = write the result back;
+//   ACC      : out[q][j] = sum_i -X[i][j] * b[q][i] in FP64, serial in i
+//              (the update_one_real vec1 = -(X^T b)), X the current inverse.
 constexpr int STREAM_ROWS = 4;
 template <bool ACC, class T>
-__device__ void stream_planes(T* planes, int n, int nqp, const double* pv1, const double* pv2, int pm,
+__device__ void stream_planes(T* planes, int n, int nqp, const double2* pend, const PendList& P, bool store,
                               const double* b, double* out) {
   using IO = PlaneIO<T>;
   constexpr int EPV = IO::EPV, NV = 2, CPL = NV * EPV;
@@ -678,69 +695,79 @@ __device__ void stream_planes(T* planes, int n, int nqp, const double* pv1, cons
   const __amdgpu_buffer_rsrc_t rs =
       __builtin_amdgcn_make_buffer_rsrc(planes, 0, (int)((size_t)nqp * nn * ES), BUF_WORD3);
   const int rowb = n * ES;
+  const int np = P.np;
   for (int q0 = 0; q0 < nqp; q0 += PG) {
     const int q = q0 + qo;
     const bool on = qo < PG && q < nqp;
     const int qoff = (on ? q : 0) * nn * ES;
-    const double* p1 = pv1 + (on ? q : 0) * n;
-    const double* p2 = pv2 + (on ? q : 0) * n;
+    const double2* pq = pend + (size_t)(on ? q : 0) * n * PEND_MAX;
     const double* bq = b + (on ? q : 0) * n;
     for (int g0 = 0; g0 < n; g0 += span) {
       const int jb = g0 + CPL * jl;  // first column of the lane
       int cnt[NV];                   // elements of vector m inside the row
       double acc[CPL];
-      T c1[CPL], c2[CPL];
+      T c1[PEND_MAX][CPL], c2[PEND_MAX][CPL];
 #pragma unroll
       for (int m = 0; m < NV; ++m) cnt[m] = on ? max(0, min(EPV, n - (jb + EPV * m))) : 0;
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
         const bool in = c % EPV < cnt[c / EPV];
         acc[c] = 0.0;
-        c1[c] = (pm >= 0 && in) ? (T)p1[jb + c] : (T)0;
-        c2[c] = (pm >= 0 && in) ? (T)p2[jb + c] : (T)0;
+#pragma unroll
+        for (int k = 0; k < PEND_MAX; ++k) {
+          double2 d = {0.0, 0.0};
+          if (k < np && in) d = pq[(jb + c) * PEND_MAX + k];
+          c1[k][c] = (T)d.x;
+          c2[k][c] = (T)d.y;
+        }
       }
       const int vbase = qoff + jb * ES;
-      // Two tiles in flight: the loads of tile i0 + STREAM_ROWS (inverse
-      // rows, pending-vector and b rows) are issued before tile i0 is
-      // updated and stored (a load after a buffer store could not be moved
-      // above it by the compiler).
+      // Two tiles of plane rows in flight: the loads of tile i0 +
+      // STREAM_ROWS are issued before tile i0 is updated and stored (a load
+      // after a buffer store could not be moved above it by the compiler).
       u32x4_t x[STREAM_ROWS][NV], xn[STREAM_ROWS][NV];
-      T ra[STREAM_ROWS], rb[STREAM_ROWS], ran[STREAM_ROWS], rbn[STREAM_ROWS];
-      double rsv[STREAM_ROWS], rsvn[STREAM_ROWS];
-      auto load_tile = [&](int i0, u32x4_t (&tx)[STREAM_ROWS][NV], T (&ta)[STREAM_ROWS], T (&tb)[STREAM_ROWS],
-                           double (&ts)[STREAM_ROWS]) {
+      auto load_tile = [&](int i0, u32x4_t (&tx)[STREAM_ROWS][NV]) {
 #pragma unroll
         for (int r = 0; r < STREAM_ROWS; ++r)
           if (i0 + r < n) {
 #pragma unroll
             for (int m = 0; m < NV; ++m)
               if (cnt[m] > 0) tx[r][m] = IO::load(rs, vbase + 16 * m, (i0 + r) * rowb, cnt[m]);
-            if (pm >= 0) {
-              ta[r] = (T)p1[i0 + r];
-              tb[r] = (T)p2[i0 + r];
-            }
-            if (ACC) ts[r] = bq[i0 + r];
           }
       };
-      load_tile(0, x, ra, rb, rsv);
+      load_tile(0, x);
       for (int i0 = 0; i0 < n; i0 += STREAM_ROWS) {
-        if (i0 + STREAM_ROWS < n) load_tile(i0 + STREAM_ROWS, xn, ran, rbn, rsvn);
+        double2 pr[STREAM_ROWS][PEND_MAX];
+        double rsv[STREAM_ROWS];
+#pragma unroll
+        for (int r = 0; r < STREAM_ROWS; ++r)
+          if (i0 + r < n) {
+#pragma unroll
+            for (int k = 0; k < PEND_MAX; ++k)
+              if (k < np) pr[r][k] = pq[(i0 + r) * PEND_MAX + k];
+            if (ACC) rsv[r] = bq[i0 + r];
+          }
+        if (i0 + STREAM_ROWS < n) load_tile(i0 + STREAM_ROWS, xn);
 #pragma unroll
         for (int r = 0; r < STREAM_ROWS; ++r) {
           const int i = i0 + r;
           if (i < n) {
-            if (pm >= 0) {
-              const T a0 = ra[r], b0 = rb[r];
+            if (np > 0) {
 #pragma unroll
               for (int m = 0; m < NV; ++m)
                 if (cnt[m] > 0) {
 #pragma unroll
                   for (int h = 0; h < EPV; ++h)
-                    if (h < cnt[m])
-                      IO::set(x[r][m], h,
-                              apply_pend(IO::get(x[r][m], h), a0, b0, c1[EPV * m + h], c2[EPV * m + h], i,
-                                         jb + EPV * m + h, pm));
-                  IO::store(x[r][m], rs, vbase + 16 * m, i * rowb, cnt[m]);
+                    if (h < cnt[m]) {
+                      T e = IO::get(x[r][m], h);
+#pragma unroll
+                      for (int k = 0; k < PEND_MAX; ++k)
+                        if (k < np)
+                          e = apply_pend(e, (T)pr[r][k].x, (T)pr[r][k].y, c1[k][EPV * m + h], c2[k][EPV * m + h], i,
+                                         jb + EPV * m + h, P.pm[k]);
+                      IO::set(x[r][m], h, e);
+                    }
+                  if (store) IO::store(x[r][m], rs, vbase + 16 * m, i * rowb, cnt[m]);
                 }
             }
             if (ACC) {
@@ -754,13 +781,9 @@ __device__ void stream_planes(T* planes, int n, int nqp, const double* pv1, cons
           }
         }
 #pragma unroll
-        for (int r = 0; r < STREAM_ROWS; ++r) {
+        for (int r = 0; r < STREAM_ROWS; ++r)
 #pragma unroll
           for (int m = 0; m < NV; ++m) x[r][m] = xn[r][m];
-          ra[r] = ran[r];
-          rb[r] = rbn[r];
-          rsv[r] = rsvn[r];
-        }
       }
       if (ACC) {
 #pragma unroll
@@ -768,6 +791,20 @@ __device__ void stream_planes(T* planes, int n, int nqp, const double* pv1, cons
           if (c % EPV < cnt[c / EPV]) out[q * n + jb + c] = acc[c];
       }
     }
+  }
+}
+
+// Apply every pending update to the stored planes and empty the list;
+// passes (if not null) = {write passes, flush passes} counters.
+template <class T>
+__device__ inline void flush_pending(T* planes, int n, int nqp, const double2* pend, PendList& P, int* passes) {
+  if (P.np > 0) {
+    stream_planes<false>(planes, n, nqp, pend, P, true, nullptr, nullptr);
+    if (passes && threadIdx.x == 0) {
+      atomicAdd(passes, 1);
+      atomicAdd(passes + 1, 1);
+    }
+    P.np = 0;
   }
 }
 
@@ -820,8 +857,13 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v_arg, GroupArgs ga, i
   const int tid = threadIdx.x, nt = blockDim.x;
   int n = v.n, ne = v.ne, ns = v.ns, n2 = v.n2, nqp = v.nqp, nn = n * n;
   double* work = GROUP ? ga.work + (size_t)slot * ga.work_stride : v.work + (size_t)slot * pfinv_work_doubles(n, nqp);
-  double* pv1 = (GROUP ? ga.pend + (size_t)slot * ga.pend_stride : v.pend + (size_t)slot * 2 * nqp * n);  // pending hop update: vec1
-  double* pv2 = pv1 + (size_t)nqp * n;                      //                     vec2
+  // pending hop updates of the slot's walker
+  const double2* pend_c = (const double2*)(GROUP ? ga.pend + (size_t)slot * ga.pend_stride
+                                                 : v.pend + (size_t)slot * 2 * PEND_MAX * nqp * n);
+  double2* pend = const_cast<double2*>(pend_c);
+  // write and flush passes of a single-ensemble launch (nflag[3], nflag[4]);
+  // not counted in a group
+  int* const wpasses = GROUP ? nullptr : v_arg.nflag + 3;
   PHASE_INIT
   __shared__ int next_w;
   for (;;) {
@@ -842,7 +884,6 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v_arg, GroupArgs ga, i
         n_in = v.interval * (v.ns > 0 ? v.ns : 1);
         inv = sizeof(Elem) == sizeof(float) ? (Elem*)v.inv32 : (Elem*)v.inv;
         n = v.n; ne = v.ne; ns = v.ns; n2 = v.n2; nqp = v.nqp; nn = n * n;
-        pv2 = pv1 + (size_t)nqp * n;
         L = carve_sampler(v, lds);
         load_tables(v, L);
         __syncthreads();
@@ -851,7 +892,8 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v_arg, GroupArgs ga, i
       break;
     }
     int* ws = v.wstate + (size_t)w * 8;
-    int pm = -1;  // row of the pending update (-1: none); uniform over the workgroup
+    PendList P;  // hop updates not yet written back (uniform over the workgroup)
+    P.np = 0;
     if (tid == 0) {
       L.misc[M_NOUT] = ws[2];
       L.misc[M_NACC] = ws[1];
@@ -994,12 +1036,15 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v_arg, GroupArgs ga, i
           const int ksplit = nterms / 2;
           double lsum = 0.0;
           if (tid == 0) lsum = lds_serial_sum(L.terms, 0, ksplit, 0.0);
-          // row msa of the (pending-updated) inverse and the Slater row
-          // (waves 1..), loads of STAGE_U elements per lane issued together
+          // row msa of the current inverse (stored row + pending updates)
+          // and the Slater row (waves 1..), loads of STAGE_U elements per
+          // lane issued together
           constexpr int STAGE_U = 4;
           const int st_id = tid - 64, st_n = nt - 64;
+          const int np = P.np;
           for (int e0 = 0; e0 < nqp * n && tid >= 64; e0 += st_n * STAGE_U) {
-            double xv[STAGE_U], sv[STAGE_U], a1[STAGE_U], a2[STAGE_U];
+            double xv[STAGE_U], sv[STAGE_U];
+            double2 pj[STAGE_U][PEND_MAX];
 #pragma unroll
             for (int u = 0; u < STAGE_U; ++u) {
               const int e = e0 + st_id + u * st_n;
@@ -1008,7 +1053,9 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v_arg, GroupArgs ga, i
                 xv[u] = planes[(size_t)q * nn + (size_t)msa * n + j];
                 const int rsj = L.ele_idx[j] + (j < ne ? 0 : ns);
                 sv[u] = v.slater[((size_t)q * n2 + rsa) * n2 + rsj];
-                if (pm >= 0) { a1[u] = pv1[e]; a2[u] = pv2[e]; }
+#pragma unroll
+                for (int k = 0; k < PEND_MAX; ++k)
+                  if (k < np) pj[u][k] = pend_c[(size_t)e * PEND_MAX + k];
               }
             }
 #pragma unroll
@@ -1017,7 +1064,12 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v_arg, GroupArgs ga, i
               if (e < nqp * n) {
                 const int q = e / n, j = e % n;
                 double x = xv[u];
-                if (pm >= 0) x = apply_pend(x, pv1[q * n + msa], pv2[q * n + msa], a1[u], a2[u], msa, j, pm);
+#pragma unroll
+                for (int k = 0; k < PEND_MAX; ++k)
+                  if (k < np) {
+                    const double2 pi = pend_c[((size_t)q * n + msa) * PEND_MAX + k];
+                    x = apply_pend(x, pi.x, pi.y, pj[u][k].x, pj[u][k].y, msa, j, P.pm[k]);
+                  }
                 stage[e] = x;
                 slt[e] = sv[u];
               }
@@ -1034,9 +1086,8 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v_arg, GroupArgs ga, i
           if (tid == 0) L.dmisc[0] = lds_serial_sum(L.terms, 0, nterms, 0.0);
           // two-electron ratio (updates.rs:395-441, 1539-1622) on the
           // fully updated inverse
-          if (pm >= 0) {
-            stream_planes<false>(planes, n, nqp, pv1, pv2, pm, nullptr, nullptr);
-            pm = -1;
+          if (P.np > 0) {
+            flush_pending(planes, n, nqp, pend_c, P, wpasses);
             __syncthreads();
           }
           const int msa = mi + s * ne, msb = mj + t * ne;
@@ -1121,19 +1172,27 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v_arg, GroupArgs ga, i
         apply_clear_deltas(v, L, cnt, acc != 0);
         if (acc) {
           if (type == 0) {
-            // update_one_real_scalar (updates.rs:1184-1224), delayed by one
-            // accepted hop: one pass applies the pending update and forms
-            // vec1 = -(X^T slt) of this one; this update becomes pending.
+            // update_one_real_scalar (updates.rs:1184-1224), delayed: one
+            // pass forms vec1 = -(X^T slt) of this update from the current
+            // inverse (pending updates applied in registers), writing the
+            // planes back only when the pending list is full; this update
+            // joins the list.
             const int msa = mi + s * ne;
-            stream_planes<true>(planes, n, nqp, pv1, pv2, pm, slt, vec1);
+            const bool full = P.np == PEND_MAX;
+            stream_planes<true>(planes, n, nqp, pend_c, P, full, slt, vec1);
+            if (full && wpasses && tid == 0) atomicAdd(wpasses, 1);
             __syncthreads();
+            if (full) P.np = 0;
+            const int k = P.np;
             for (int e = tid; e < nqp * n; e += nt) {
               const int q = e / n;
               const double inv_a = -1.0 / vec1[q * n + msa];
-              pv2[e] = stage[e] * inv_a;
-              pv1[e] = vec1[e];
+              pend[(size_t)e * PEND_MAX + k] = make_double2(vec1[e], stage[e] * inv_a);
             }
-            pm = msa;
+#pragma unroll
+            for (int kk = 0; kk < PEND_MAX; ++kk)
+              if (kk == k) P.pm[kk] = msa;
+            P.np = k + 1;
             for (int q = tid; q < nqp; q += nt) L.pf[q] = L.pf_new[q];
             __syncthreads();
             PHASE(5);
@@ -1228,7 +1287,7 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v_arg, GroupArgs ga, i
     }
     if (stop_at >= 0) {
       // the planes are rebuilt from the configuration, so the pending
-      // update is dropped; the walker resumes at proposal stop_at
+      // updates are dropped; the walker resumes at proposal stop_at
       store_walker(v, w, L);
       for (int q = tid; q < nqp; q += nt) v.pf[(size_t)w * nqp + q] = L.pf[q];
       if (tid == 0) {
@@ -1243,11 +1302,8 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v_arg, GroupArgs ga, i
       PHASE(6);
       continue;
     }
-    // ---- end of block: apply the pending update, persist state, save sample ----
-    if (pm >= 0) {
-      stream_planes<false>(planes, n, nqp, pv1, pv2, pm, nullptr, nullptr);
-      pm = -1;
-    }
+    // ---- end of block: write the pending updates back, persist state, save sample ----
+    flush_pending(planes, n, nqp, pend_c, P, wpasses);
     store_walker(v, w, L);
     for (int q = tid; q < nqp; q += nt) v.pf[(size_t)w * nqp + q] = L.pf[q];
     if (tid == 0) {

@@ -308,14 +308,15 @@ class Ensemble:
         return st, x[:na.value].copy()
 
     def timings(self):
-        t = np.zeros(24)
+        t = np.zeros(26)
         lib().mvmc_ens_timings(self.h, _p(t))
         return {"sample_s": t[0], "measure_s": t[1], "sr_s": t[2], "proposals": t[3], "accepted": t[4],
                 "block_ms": t[5], "block_launches": t[6], "rebuild_ms": t[7], "rebuilt": t[8],
                 "meas_rebuild_ms": t[9], "measure_ms": t[10], "measured": t[11], "cg_ms": t[12],
                 "cg_iter": t[13], "cg_na": t[14], "cg_rows": t[15], "rebuild_variant": int(t[16]),
                 "rebuild_ms_per_walker": (t[17], t[18], t[19]), "adjoint_vjp_ms": t[20],
-                "adjoint_vjp_rebuild_ms": t[21], "gram_ms": t[22]}
+                "adjoint_vjp_rebuild_ms": t[21], "gram_ms": t[22], "block_write_passes": t[23],
+                "block_flush_passes": t[24]}
 
     PHASES = ("candidate", "projection", "staging", "ratio", "metropolis", "accept_pass", "rebuild", "exchange",
               "block_end")
