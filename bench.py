@@ -441,9 +441,10 @@ def main():
         sec["k_cg_* (d): SR-CG"] = {"iterations": tl["cg_iter"], "ms": tl["cg_ms"],
                                     "achieved_GBs": cb / (tl["cg_ms"] / 1e3) / 1e9,
                                     "frac_hbm": cb / (tl["cg_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS}
-    out["autotuned"] = {"rebuild_variant": ["3 workgroups/CU, 8-column close batches",
-                                            "2 workgroups/CU, 16-column close batches",
-                                            "not decided"][tl["rebuild_variant"] if tl["rebuild_variant"] >= 0 else 2],
+    variants = ["3 workgroups/CU, 8-column close batches", "2 workgroups/CU, 16-column close batches",
+                "register-resident, one 1024-thread workgroup per CU"]
+    rv = int(tl["rebuild_variant"])
+    out["autotuned"] = {"rebuild_variant": variants[rv] if 0 <= rv < len(variants) else "not decided",
                         "rebuild_ms_per_walker_measured": list(tl["rebuild_ms_per_walker"])}
     out["secondary_rooflines"] = sec
 
