@@ -100,6 +100,34 @@ def test_exchange_path_benchmark_lattices(M, tmp_path, name, W, warmup):
         O.set_math(0)
 
 
+@pytest.mark.parametrize("ex_path", [0, 1])
+def test_write_pass_accounting(M, tmp_path, ex_path):
+    """The sampler writes a walker's planes back at every second accepted hop
+    (the pending list of two is full) and when it flushes the list (before
+    an exchange proposal, at the end of a block); a periodic rebuild drops
+    the list.  Per segment of h accepted hops between flushes or rebuilds
+    that makes floor((h - 1) / 2) list-full writes, so over the sample call
+    hops/2 - segments <= writes - flushes <= hops/2, and every flush is a
+    write.  These counts give the sampler's algorithmic bytes."""
+    import lattice
+    W, warmup = 32, 4
+    info = lattice.write_workload("square8", str(tmp_path / "s8"), sr_steps=1, warmup=warmup, ex_update_path=ex_path)
+    log(f"square8 W={W} NExUpdatePath {ex_path}: write-pass accounting")
+    ens = M.Ensemble(info["namelist"], info["initial"], walkers=W)
+    ens.opt_step()
+    t = ens.timings()
+    c = np.array([ens.walker(w)["counter"] for w in range(W)], dtype=np.int64)
+    hops, ex_props = int(c[:, 1].sum()), int(c[:, 2].sum())
+    writes, flushes = int(t["block_write_passes"]), int(t["block_flush_passes"])
+    blocks = W * (warmup + 1)
+    segments = blocks + int(t["rebuilt"]) + ex_props
+    log(f"  hops {hops}, exchange proposals {ex_props}, writes {writes}, flushes {flushes}, rebuilt {t['rebuilt']}")
+    ens.close()
+    assert hops > 0 and 0 < flushes <= writes
+    assert flushes <= blocks + ex_props
+    assert hops / 2 - segments <= writes - flushes <= hops // 2
+
+
 def _random_skew(rng, n):
     a = rng.standard_normal((n, n))
     return a - a.T
