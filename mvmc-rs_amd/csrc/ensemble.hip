@@ -80,9 +80,13 @@ struct SlotTimer {
   const int slot = GROUP ? (int)blockIdx.x - (GARG).slot_start[grp_run_] : (int)blockIdx.x;             \
   const int nslots = GROUP ? (GARG).slot_start[grp_run_ + 1] - (GARG).slot_start[grp_run_] : (int)gridDim.x;
 
+// Row stride of the sampler's hop staging arrays in LDS: odd, so that lanes
+// reading the same element of different QP planes hit distinct banks.
+__host__ __device__ inline int hop_ld(int n) { return n | 1; }
+
 __host__ __device__ inline size_t union_doubles_sampler(const EnsView& v) {
   const size_t nq = (size_t)v.nqp * v.n;
-  size_t hop = 3 * nq;
+  size_t hop = 3 * (size_t)v.nqp * hop_ld(v.n);
   size_t ex = v.ex_path != 0 ? 5 * nq + 16 * (size_t)v.nqp : 0;
   size_t m = hop > ex ? hop : ex;
   size_t pfb = (pf_bytes(v.n, v.nqp) + 7) / 8;
@@ -678,11 +682,12 @@ struct PendList {
 //   P.np > 0 : apply the pending updates, in order, in the storage type's
 //              arithmetic; store = write the result back;
 //   ACC      : out[q][j] = sum_i -X[i][j] * b[q][i] in FP64, serial in i
-//              (the update_one_real vec1 = -(X^T b)), X the current inverse.
+//              (the update_one_real vec1 = -(X^T b)), X the current inverse;
+//              rows of b and out are ldb doubles apart.
 constexpr int STREAM_ROWS = 4;
 template <bool ACC, class T>
 __device__ void stream_planes(T* planes, int n, int nqp, const double2* pend, const PendList& P, bool store,
-                              const double* b, double* out) {
+                              const double* b, double* out, int ldb = 0) {
   using IO = PlaneIO<T>;
   constexpr int EPV = IO::EPV, NV = 2, CPL = NV * EPV;
   constexpr int ES = (int)sizeof(T);
@@ -701,7 +706,7 @@ __device__ void stream_planes(T* planes, int n, int nqp, const double2* pend, co
     const bool on = qo < PG && q < nqp;
     const int qoff = (on ? q : 0) * nn * ES;
     const double2* pq = pend + (size_t)(on ? q : 0) * n * PEND_MAX;
-    const double* bq = b + (on ? q : 0) * n;
+    const double* bq = b + (on ? q : 0) * ldb;
     for (int g0 = 0; g0 < n; g0 += span) {
       const int jb = g0 + CPL * jl;  // first column of the lane
       int cnt[NV];                   // elements of vector m inside the row
@@ -788,7 +793,7 @@ __device__ void stream_planes(T* planes, int n, int nqp, const double2* pend, co
       if (ACC) {
 #pragma unroll
         for (int c = 0; c < CPL; ++c)
-          if (c % EPV < cnt[c / EPV]) out[q * n + jb + c] = acc[c];
+          if (c % EPV < cnt[c / EPV]) out[q * ldb + jb + c] = acc[c];
       }
     }
   }
@@ -1027,9 +1032,13 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v_arg, GroupArgs ga, i
         // The projection sum is one serial chain: on the hop path lane 0 of
         // wave 0 forms it (in two parts, carried in a register) while the
         // other waves stage the rows and form the Pfaffian-ratio dots.
-        double* stage = L.uni;            // [nqp][n]
-        double* slt = L.uni + nqp * n;    // [nqp][n]
-        double* vec1 = L.uni + 2 * nqp * n;
+        // Rows of the hop staging arrays are hop_ld(n) (odd) doubles apart,
+        // so the ratio dots' lanes (one per QP plane, same k) read distinct
+        // LDS banks.
+        const int ld = hop_ld(n);
+        double* stage = L.uni;              // [nqp][ld]
+        double* slt = L.uni + nqp * ld;     // [nqp][ld]
+        double* vec1 = L.uni + 2 * nqp * ld;
         if (type == 0) {
           const int msa = mi + s * ne;
           const int rsa = L.ele_idx[msa] + s * ns;
@@ -1070,8 +1079,8 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v_arg, GroupArgs ga, i
                     const double2 pi = pend_c[((size_t)q * n + msa) * PEND_MAX + k];
                     x = apply_pend(x, pi.x, pi.y, pj[u][k].x, pj[u][k].y, msa, j, P.pm[k]);
                   }
-                stage[e] = x;
-                slt[e] = sv[u];
+                stage[q * ld + j] = x;
+                slt[q * ld + j] = sv[u];
               }
             }
           }
@@ -1079,7 +1088,7 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v_arg, GroupArgs ga, i
           PHASE(2);
           if (tid == 0) L.dmisc[0] = lds_serial_sum(L.terms, ksplit, nterms, lsum);
           for (int q = tid - 64; q < nqp && tid >= 64; q += nt - 64) {
-            const double ratio = lds_serial_dot(stage + q * n, slt + q * n, n);
+            const double ratio = lds_serial_dot(stage + q * ld, slt + q * ld, n);
             L.pf_new[q] = -ratio * L.pf[q];
           }
         } else {
@@ -1179,15 +1188,16 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v_arg, GroupArgs ga, i
             // joins the list.
             const int msa = mi + s * ne;
             const bool full = P.np == PEND_MAX;
-            stream_planes<true>(planes, n, nqp, pend_c, P, full, slt, vec1);
+            stream_planes<true>(planes, n, nqp, pend_c, P, full, slt, vec1, ld);
             if (full && wpasses && tid == 0) atomicAdd(wpasses, 1);
             __syncthreads();
             if (full) P.np = 0;
             const int k = P.np;
             for (int e = tid; e < nqp * n; e += nt) {
               const int q = e / n;
-              const double inv_a = -1.0 / vec1[q * n + msa];
-              pend[(size_t)e * PEND_MAX + k] = make_double2(vec1[e], stage[e] * inv_a);
+              const int j = e - q * n;
+              const double inv_a = -1.0 / vec1[q * ld + msa];
+              pend[(size_t)e * PEND_MAX + k] = make_double2(vec1[q * ld + j], stage[q * ld + j] * inv_a);
             }
 #pragma unroll
             for (int kk = 0; kk < PEND_MAX; ++kk)
