@@ -59,6 +59,22 @@ __device__ inline int queue_run(const GroupArgs& g, int e, int& w) {
   return g.qrun[k];
 }
 
+// View of run r in a group launch.  The host writes the views before the
+// launch and nothing writes them during it, so they are read through the
+// constant address space: the compiler then holds the fields in scalar
+// registers, reloading them when it needs the room, instead of keeping
+// vector copies live across the walker loop.
+__device__ inline EnsView group_view(const GroupArgs& g, int r) {
+  static_assert(sizeof(EnsView) % 4 == 0, "EnsView is copied as 32-bit words");
+  typedef const __attribute__((address_space(4))) uint32_t* ConstWords;
+  const ConstWords src = (ConstWords)(g.views + r);
+  EnsView v;
+  uint32_t* dst = reinterpret_cast<uint32_t*>(&v);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(EnsView) / 4); ++i) dst[i] = src[i];
+  return v;
+}
+
 // Kernel prologue of the walker kernels: `v` is the ensemble this workgroup
 // serves, `slot` its scratch slot and `nslots` the workgroups serving that
 // ensemble (the launch's own grid outside a group).
@@ -76,7 +92,7 @@ struct SlotTimer {
   SlotTimer slot_timer_(GROUP && threadIdx.x == 0 && (GARG).slot_time ? (GARG).slot_time + blockIdx.x     \
                                                                       : nullptr);                        \
   const int grp_run_ = GROUP ? group_run(GARG) : 0;                                                      \
-  const EnsView v = GROUP ? (GARG).views[grp_run_] : (VARG);                                             \
+  const EnsView v = GROUP ? group_view(GARG, grp_run_) : (VARG);                                         \
   const int slot = GROUP ? (int)blockIdx.x - (GARG).slot_start[grp_run_] : (int)blockIdx.x;             \
   const int nslots = GROUP ? (GARG).slot_start[grp_run_ + 1] - (GARG).slot_start[grp_run_] : (int)gridDim.x;
 
@@ -851,7 +867,7 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v_arg, GroupArgs ga, i
   if (GROUP) {
     int w0 = 0;
     cur_run = queue_run(ga, 0, w0);
-    v = ga.views[cur_run];
+    v = group_view(ga, cur_run);
   }
   // proposals per outer step (NVMCInterval x Nsite): each run's own in a group
   int n_in = GROUP ? v.interval * (v.ns > 0 ? v.ns : 1) : n_in_arg;
@@ -885,7 +901,7 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v_arg, GroupArgs ga, i
       const int r = queue_run(ga, qe, w);
       if (r != cur_run) {
         cur_run = r;
-        v = ga.views[r];
+        v = group_view(ga, r);
         n_in = v.interval * (v.ns > 0 ? v.ns : 1);
         inv = sizeof(Elem) == sizeof(float) ? (Elem*)v.inv32 : (Elem*)v.inv;
         n = v.n; ne = v.ne; ns = v.ns; n2 = v.n2; nqp = v.nqp; nn = n * n;
