@@ -66,6 +66,36 @@ def test_group_matches_runs_alone(M, tmp_path, graph_rounds):
     g.close()
 
 
+def test_group_exchange_sampler_matches_runs_alone(M, tmp_path):
+    """The group sampler with exchange moves (NExUpdatePath 1: hops and spin
+    exchanges, whose proposals flush the pending hop updates to the planes
+    first): one run per side L = 6, 8, 10, 12, 16 walkers each, two SR-CG
+    steps; each run's rows, parameters and walker states equal those of the
+    run stepped alone, with exchange moves proposed and accepted."""
+    import lattice
+    infos = lattice.write_uneven_batch(str(tmp_path / "bx"), walkers=16, runs_per_side=1, sr_steps=2, warmup=2,
+                                       ex_update_path=1)
+    R = len(infos)
+    W = [16] * R
+    log(f"exchange group of {R} runs, Nsize {[i['nsize'] for i in infos]}")
+    g = M.EnsembleGroup([i["namelist"] for i in infos], [i["initial"] for i in infos], W)
+    rows_g = [g.opt_step() for _ in range(2)]
+    for r, info in enumerate(infos):
+        e = M.Ensemble(info["namelist"], info["initial"], walkers=W[r])
+        rows_a = [e.opt_step() for _ in range(2)]
+        for k in range(2):
+            assert np.array_equal(rows_g[k][r], rows_a[k]), (r, k, rows_g[k][r], rows_a[k])
+        assert np.array_equal(np.concatenate(g.runs[r].params()), np.concatenate(e.params())), r
+        ex = 0
+        for w in range(W[r]):
+            a, b = _walker_state(g.runs[r], w), _walker_state(e, w)
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2:] == b[2:], (r, w)
+            ex += a[3][3]
+        assert ex > 0, ("no exchange accepted", r)
+        e.close()
+    g.close()
+
+
 def test_group_matches_oracle(M, tmp_path):
     """One run per side (Nsize 28..144), 2 walkers each, one SR-CG step as a
     group against the oracle run by run (device exp/log): zvo rows,
