@@ -412,6 +412,11 @@ def main():
                      "kernel": "k_block", "avg_launch_ms": avg_launch_s * 1e3, "launches": launches,
                      "algorithmic_bytes_per_launch": bytes_total / max(launches, 1),
                      "accepted_moves": accs, "write_passes": wpasses, "flush_passes": fpasses,
+                     # the same launches priced at the write-every-hop byte
+                     # count (one read + one write of the planes per accepted
+                     # move), for comparison with that form only
+                     "write_every_hop_equivalent_GBs": (props * nqp * 2 * n * 8 + accs * nqp * 2 * n * n * 8)
+                     / max(launches, 1) / avg_launch_s / 1e9,
                      "walkers_per_launch": [[int(x) for x in ll[:, 3]] for ll in launch_logs]},
     }
     if tmodel is not None:
