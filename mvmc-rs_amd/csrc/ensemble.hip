@@ -681,6 +681,21 @@ struct PlaneIO<float> {
 // updated element; so the pass of an accepted hop, which has to read every
 // element anyway, writes the planes back only when the list is full.
 constexpr int PEND_MAX = 2;
+// Storage of a pending pair: FP64; FP32 in the mixed-precision form, whose
+// stream applies the updates in FP32 anyway (the row staging of a proposal
+// then applies the FP32-rounded vectors in FP64).
+template <class T>
+struct PendPair;
+template <>
+struct PendPair<double> {
+  using type = double2;
+  static __device__ inline type make(double a, double b) { return make_double2(a, b); }
+};
+template <>
+struct PendPair<float> {
+  using type = float2;
+  static __device__ inline type make(double a, double b) { return make_float2((float)a, (float)b); }
+};
 struct PendList {
   int np;             // pending updates (uniform over the workgroup)
   int pm[PEND_MAX];   // their rows
@@ -702,9 +717,11 @@ struct PendList {
 //              rows of b and out are ldb doubles apart.
 constexpr int STREAM_ROWS = 4;
 template <bool ACC, class T>
-__device__ void stream_planes(T* planes, int n, int nqp, const double2* pend, const PendList& P, bool store,
+__device__ void stream_planes(T* planes, int n, int nqp, const typename PendPair<T>::type* pend, const PendList& P,
+                              bool store,
                               const double* b, double* out, int ldb = 0) {
   using IO = PlaneIO<T>;
+  using PV = typename PendPair<T>::type;
   constexpr int EPV = IO::EPV, NV = 2, CPL = NV * EPV;
   constexpr int ES = (int)sizeof(T);
   const int nt = blockDim.x, tid = threadIdx.x;
@@ -721,7 +738,7 @@ __device__ void stream_planes(T* planes, int n, int nqp, const double2* pend, co
     const int q = q0 + qo;
     const bool on = qo < PG && q < nqp;
     const int qoff = (on ? q : 0) * nn * ES;
-    const double2* pq = pend + (size_t)(on ? q : 0) * n * PEND_MAX;
+    const PV* pq = pend + (size_t)(on ? q : 0) * n * PEND_MAX;
     const double* bq = b + (on ? q : 0) * ldb;
     for (int g0 = 0; g0 < n; g0 += span) {
       const int jb = g0 + CPL * jl;  // first column of the lane
@@ -736,7 +753,7 @@ __device__ void stream_planes(T* planes, int n, int nqp, const double2* pend, co
         acc[c] = 0.0;
 #pragma unroll
         for (int k = 0; k < PEND_MAX; ++k) {
-          double2 d = {0.0, 0.0};
+          PV d = {0, 0};
           if (k < np && in) d = pq[(jb + c) * PEND_MAX + k];
           c1[k][c] = (T)d.x;
           c2[k][c] = (T)d.y;
@@ -758,7 +775,7 @@ __device__ void stream_planes(T* planes, int n, int nqp, const double2* pend, co
       };
       load_tile(0, x);
       for (int i0 = 0; i0 < n; i0 += STREAM_ROWS) {
-        double2 pr[STREAM_ROWS][PEND_MAX];
+        PV pr[STREAM_ROWS][PEND_MAX];
         double rsv[STREAM_ROWS];
 #pragma unroll
         for (int r = 0; r < STREAM_ROWS; ++r)
@@ -818,7 +835,8 @@ __device__ void stream_planes(T* planes, int n, int nqp, const double2* pend, co
 // Apply every pending update to the stored planes and empty the list;
 // passes (if not null) = {write passes, flush passes} counters.
 template <class T>
-__device__ inline void flush_pending(T* planes, int n, int nqp, const double2* pend, PendList& P, int* passes) {
+__device__ inline void flush_pending(T* planes, int n, int nqp, const typename PendPair<T>::type* pend, PendList& P,
+                                     int* passes) {
   if (P.np > 0) {
     stream_planes<false>(planes, n, nqp, pend, P, true, nullptr, nullptr);
     if (passes && threadIdx.x == 0) {
@@ -879,9 +897,10 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v_arg, GroupArgs ga, i
   int n = v.n, ne = v.ne, ns = v.ns, n2 = v.n2, nqp = v.nqp, nn = n * n;
   double* work = GROUP ? ga.work + (size_t)slot * ga.work_stride : v.work + (size_t)slot * pfinv_work_doubles(n, nqp);
   // pending hop updates of the slot's walker
-  const double2* pend_c = (const double2*)(GROUP ? ga.pend + (size_t)slot * ga.pend_stride
-                                                 : v.pend + (size_t)slot * 2 * PEND_MAX * nqp * n);
-  double2* pend = const_cast<double2*>(pend_c);
+  using PV = typename PendPair<Elem>::type;
+  const PV* pend_c = (const PV*)(GROUP ? ga.pend + (size_t)slot * ga.pend_stride
+                                       : v.pend + (size_t)slot * 2 * PEND_MAX * nqp * n);
+  PV* pend = const_cast<PV*>(pend_c);
   // write and flush passes of a single-ensemble launch (nflag[3], nflag[4]);
   // not counted in a group
   int* const wpasses = GROUP ? nullptr : v_arg.nflag + 3;
@@ -1069,7 +1088,7 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v_arg, GroupArgs ga, i
           const int np = P.np;
           for (int e0 = 0; e0 < nqp * n && tid >= 64; e0 += st_n * STAGE_U) {
             double xv[STAGE_U], sv[STAGE_U];
-            double2 pj[STAGE_U][PEND_MAX];
+            PV pj[STAGE_U][PEND_MAX];
 #pragma unroll
             for (int u = 0; u < STAGE_U; ++u) {
               const int e = e0 + st_id + u * st_n;
@@ -1092,8 +1111,8 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v_arg, GroupArgs ga, i
 #pragma unroll
                 for (int k = 0; k < PEND_MAX; ++k)
                   if (k < np) {
-                    const double2 pi = pend_c[((size_t)q * n + msa) * PEND_MAX + k];
-                    x = apply_pend(x, pi.x, pi.y, pj[u][k].x, pj[u][k].y, msa, j, P.pm[k]);
+                    const PV pi = pend_c[((size_t)q * n + msa) * PEND_MAX + k];
+                    x = apply_pend<double>(x, pi.x, pi.y, pj[u][k].x, pj[u][k].y, msa, j, P.pm[k]);
                   }
                 stage[q * ld + j] = x;
                 slt[q * ld + j] = sv[u];
@@ -1213,7 +1232,7 @@ __global__ void __launch_bounds__(256, 2) k_block(EnsView v_arg, GroupArgs ga, i
               const int q = e / n;
               const int j = e - q * n;
               const double inv_a = -1.0 / vec1[q * ld + msa];
-              pend[(size_t)e * PEND_MAX + k] = make_double2(vec1[q * ld + j], stage[q * ld + j] * inv_a);
+              pend[(size_t)e * PEND_MAX + k] = PendPair<Elem>::make(vec1[q * ld + j], stage[q * ld + j] * inv_a);
             }
 #pragma unroll
             for (int kk = 0; kk < PEND_MAX; ++kk)
