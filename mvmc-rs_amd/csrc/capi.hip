@@ -361,6 +361,7 @@ int rr_rebuild_prepare(const RrCfg& c, size_t lds) {
   rr_dispatch(c, [&]<int T, int R, bool FULL>() {
     raise_lds_limit((const void*)k_rebuild_rr<T, R, FULL>, lds);
     raise_lds_limit((const void*)k_meas_rebuild_rr<T, R, FULL>, lds);
+    raise_lds_limit((const void*)k_begin_rr<T, R, FULL>, lds);
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_rebuild_rr<T, R, FULL>, T * T, lds));
   });
   return std::max(occ, 1);
@@ -760,6 +761,28 @@ void launch_rebuild(mvmc_ens* E, int var) {
   HIPCHK(hipGetLastError());
 }
 
+// Start of a sampling call: with the register-resident rebuild chosen, the
+// walkers resuming from their burn configuration are rebuilt by k_begin_rr
+// and k_begin serves the rest (placement, failed or non-finite rebuilds);
+// otherwise k_begin serves every walker.
+void launch_begin(mvmc_ens* E) {
+  EnsView& v = E->v;
+  int marked = v.W;
+  if (E->rb_variant == 2) {
+    HIPCHK(hipMemsetAsync(v.nflag + 5, 0, sizeof(int), E->st));
+    rr_dispatch(rr_cfg(v.n), [&]<int T, int R, bool FULL>() {
+      hipLaunchKernelGGL((k_begin_rr<T, R, FULL>), dim3(E->grid_rb[2]), dim3(T * T), E->lds_rr, E->st, v);
+    });
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&marked, v.nflag + 5, sizeof(int), hipMemcpyDeviceToHost, E->st));
+    HIPCHK(hipStreamSynchronize(E->st));
+    if (marked == 0) return;
+  }
+  hipLaunchKernelGGL(k_begin<false>, dim3(E->grid_samp), dim3(256), E->lds_samp, E->st, v, GroupArgs{},
+                     E->rb_variant == 2 ? 1 : 0);
+  HIPCHK(hipGetLastError());
+}
+
 void launch_meas_rebuild(mvmc_ens* E, int s) {
   if (E->rb_variant == 2)
     rr_launch_rebuild(rr_cfg(E->v.n), E->grid_rb[2], E->lds_rr, E->st, E->v, s);
@@ -776,8 +799,7 @@ void sample_impl(mvmc_ens* E) {
   EnsView& v = E->v;
   const double t0 = now_s();
   E->upload_tables();
-  hipLaunchKernelGGL(k_begin<false>, dim3(E->grid_samp), dim3(256), E->lds_samp, E->st, v, GroupArgs{});
-  HIPCHK(hipGetLastError());
+  launch_begin(E);
   std::vector<int> ws((size_t)v.W * 8);
   HIPCHK(hipMemcpyAsync(ws.data(), v.wstate, sizeof(int) * ws.size(), hipMemcpyDeviceToHost, E->st));
   HIPCHK(hipStreamSynchronize(E->st));
@@ -1663,7 +1685,7 @@ void group_sample(mvmc_group* G) {
   }
   group_clear_times(G);
   hipLaunchKernelGGL(k_begin<true>, dim3(G->samp.start[R]), dim3(256), G->lds_samp, G->st, EnsView{},
-                     group_args(G, G->samp));
+                     group_args(G, G->samp), 0);
   HIPCHK(hipGetLastError());
   int n_out = 0;
   for (mvmc_ens* E : G->runs) {

@@ -48,11 +48,13 @@ struct EnsView {
   double* pf;         // [W][nqp]
   double* log_ip;     // [W]
   int* wstate;        // [W][8]: 0 burn flag, 1 n_accept window, 2 n_out, 3 status,
-                      //   4 next proposal of the outer step, 5 rebuild pending, 6 outer step done, 7 spare
+                      //   4 next proposal of the outer step, 5 rebuild pending, 6 outer step done,
+                      //   7 left by k_begin_rr to k_begin
   int* nflag;         // [NFLAGS]: 0 walkers handed to the rebuild kernel by the last k_block launch,
                       //   1 next walker taken by a k_block slot, 2 next hand-over entry taken by k_rebuild,
                       //   3 passes of the last k_block launch that wrote a walker's planes back,
-                      //   4 those of them that accepted no move (pending updates flushed)
+                      //   4 those of them that accepted no move (pending updates flushed),
+                      //   5 walkers k_begin_rr left to k_begin
   int* rlist;         // [W] walkers handed to the rebuild kernel, in hand-over order
   long long* counter; // [W][4]
   // saved samples

@@ -457,8 +457,9 @@ __device__ inline void copy_inv32(const EnsView& v, int w) {
   }
 }
 
+// only_marked: serve only the walkers k_begin_rr left to it (wstate[7] set).
 template <bool GROUP>
-__global__ void __launch_bounds__(256) k_begin(EnsView v_arg, GroupArgs ga) {
+__global__ void __launch_bounds__(256) k_begin(EnsView v_arg, GroupArgs ga, int only_marked) {
   ENS_SLOT(v_arg, ga)
   extern __shared__ __align__(16) char lds[];
   SampLds L = carve_sampler(v, lds);
@@ -467,6 +468,7 @@ __global__ void __launch_bounds__(256) k_begin(EnsView v_arg, GroupArgs ga) {
   char* pfscr = (char*)L.uni;
   for (int w = slot; w < v.W; w += nslots) {
     int* ws = v.wstate + (size_t)w * 8;
+    if (only_marked && ws[7] == 0) continue;
     double* planes = v.inv + (size_t)w * v.nqp * v.n * v.n;
     int* cnt = v.proj_cnt + (size_t)w * v.nproj;
     load_walker(v, w, L, true);
@@ -502,6 +504,7 @@ __global__ void __launch_bounds__(256) k_begin(EnsView v_arg, GroupArgs ga) {
       v.log_ip[w] = lip;
       ws[1] = 0;
       ws[2] = burn ? v.nsample + 1 : v.warmup + v.nsample;
+      ws[7] = 0;
       for (int k = 0; k < 4; ++k) v.counter[(size_t)w * 4 + k] = 0;
     }
     store_walker(v, w, L);
@@ -2389,6 +2392,56 @@ __global__ void __launch_bounds__(T * T) k_rebuild_rr(EnsView v) {
       }
       ws[1] = 0;
       ws[5] = 0;
+    }
+    __syncthreads();
+  }
+}
+
+// Start of a sampling call (k_begin) for the walkers that resume from their
+// burn configuration, the case of every call after the first: its rebuild
+// through the register-resident Pfaffian + inverse, walkers blockIdx.x,
+// + gridDim.x, ...  That phase draws no random numbers and keeps the
+// configuration, so a walker is done here when every plane factorises (no
+// AllZero plane, info 0: the planes and Pfaffians then equal k_begin's bit
+// for bit) and log|<x|psi>| is finite.  Any other walker is marked
+// (wstate[7] = 1, counted in nflag[5]) and left to k_begin, which repeats
+// the same rebuild and goes on from there.
+template <int T, int R, bool FULL>
+__global__ void __launch_bounds__(T * T) k_begin_rr(EnsView v) {
+  extern __shared__ __align__(16) char lds[];
+  Carver c{lds};
+  int* ei = c.take<int>(v.n);
+  int* rs = c.take<int>(v.n);
+  double* pfn = c.take<double>(v.nqp);
+  RrLds L = carve_rr(c, v.n);
+  const int tid = threadIdx.x;
+  double* gs = v.work + (size_t)blockIdx.x * 2 * v.n * v.n;
+  __shared__ double lip;
+  for (int w = blockIdx.x; w < v.W; w += gridDim.x) {
+    int* ws = v.wstate + (size_t)w * 8;
+    bool done = false;
+    if (ws[0]) {
+      for (int i = tid; i < v.n; i += blockDim.x) ei[i] = v.ele_idx[(size_t)w * v.n + i];
+      __syncthreads();
+      double* planes = v.inv + (size_t)w * v.nqp * v.n * v.n;
+      const int bad = rr_calc_m_all<T, R, FULL>(v, ei, rs, planes, gs, L, pfn);
+      if (tid == 0) lip = bad ? 0.0 : ddm::cr_log(fabs(ip_of(v, pfn)));
+      __syncthreads();
+      done = !bad && lip - lip == 0.0;
+    }
+    if (done) {
+      copy_inv32(v, w);
+      if (tid == 0) {
+        for (int q = 0; q < v.nqp; ++q) v.pf[(size_t)w * v.nqp + q] = pfn[q];
+        v.log_ip[w] = lip;
+        ws[1] = 0;
+        ws[2] = v.nsample + 1;
+        ws[7] = 0;
+        for (int k = 0; k < 4; ++k) v.counter[(size_t)w * 4 + k] = 0;
+      }
+    } else if (tid == 0) {
+      ws[7] = 1;
+      atomicAdd(v.nflag + 5, 1);
     }
     __syncthreads();
   }

@@ -99,38 +99,41 @@ def test_rr_full_form_bit_exact_vs_oracle(M, n):
 @pytest.mark.parametrize("case,W", [("chain16", 8), ("square8", 2), ("uneven10", 2), ("uneven12", 2),
                                     ("square16", 1)])
 def test_rr_rebuilds_in_the_sampler_vs_oracle(M, tmp_path, case, W, monkeypatch):
-    """The sampler's periodic rebuild and the measurement rebuild through the
-    register-resident kernels (rebuild variant 2, fixed with
-    MVMC_RB_VARIANT): the full form on the 16-site chain and the 8x8 lattice
-    (Nsize 16, 64), the panel form on uneven-batch runs of side 10 and 12
-    (Nsize 90, 112) and the 16x16 lattice (Nsize 224), one SR-CG step: zvo
-    row, parameters and every walker's state bit-exact against the oracle,
-    with periodic rebuilds done."""
+    """The sampler's periodic rebuild, the measurement rebuild and, from the
+    second step on, the rebuild of the walkers' burn configurations at the
+    start of a sampling call (k_begin_rr) through the register-resident
+    kernels (rebuild variant 2, fixed with MVMC_RB_VARIANT): the full form on
+    the 16-site chain and the 8x8 lattice (Nsize 16, 64), the panel form on
+    uneven-batch runs of side 10 and 12 (Nsize 90, 112) and the 16x16
+    lattice (Nsize 224), two SR-CG steps: zvo rows, parameters and every
+    walker's state bit-exact against the oracle after each step, with
+    periodic rebuilds done."""
     import lattice
     if case in ("chain16", "square8", "square16"):
-        info = lattice.write_workload(case, str(tmp_path / case), sr_steps=1, warmup=1 if case == "square16" else 4)
+        info = lattice.write_workload(case, str(tmp_path / case), sr_steps=2, warmup=1 if case == "square16" else 4)
     else:
         side = int(case[len("uneven"):])
-        info = [i for i in lattice.write_uneven_batch(str(tmp_path / "b"), walkers=W, runs_per_side=1, sr_steps=1,
+        info = [i for i in lattice.write_uneven_batch(str(tmp_path / "b"), walkers=W, runs_per_side=1, sr_steps=2,
                                                       warmup=2) if i["L"] == side][0]
         assert info["nsize"] > 64
     monkeypatch.setenv("MVMC_RB_VARIANT", "2")
     log(f"rr rebuild {case} W={W}")
     ens = M.Ensemble(info["namelist"], info["initial"], walkers=W)
-    row = ens.opt_step()
-    t = ens.timings()
-    log(f"  rebuilt {t['rebuilt']}, variant {t['rebuild_variant']}")
-    assert t["rebuild_variant"] == 2 and t["rebuilt"] > 0, t
     O.set_math(1)
     try:
         run = O.Run(info["namelist"], info["initial"], walkers=W)
-        run.step(do_sr=True)
-        assert np.array_equal(row, run.energies()[-1])
-        assert np.array_equal(np.concatenate(ens.params()), np.concatenate(run.params()))
-        for w in range(W):
-            sg, so = ens.walker(w), run.walker(w)
-            assert np.array_equal(sg["ele_idx"], so["ele_idx"]), w
-            assert np.array_equal(sg["rng"], so["rng"]) and sg["rng_idx"] == so["rng_idx"], w
+        for step in range(2):
+            row = ens.opt_step()
+            t = ens.timings()
+            log(f"  step {step}: rebuilt {t['rebuilt']}, variant {t['rebuild_variant']}")
+            assert t["rebuild_variant"] == 2 and t["rebuilt"] > 0, t
+            run.step(do_sr=True)
+            assert np.array_equal(row, run.energies()[-1]), step
+            assert np.array_equal(np.concatenate(ens.params()), np.concatenate(run.params())), step
+            for w in range(W):
+                sg, so = ens.walker(w), run.walker(w)
+                assert np.array_equal(sg["ele_idx"], so["ele_idx"]), (step, w)
+                assert np.array_equal(sg["rng"], so["rng"]) and sg["rng_idx"] == so["rng_idx"], (step, w)
     finally:
         O.set_math(0)
         ens.close()
